@@ -1,0 +1,97 @@
+/*
+ * jfx.h — C ABI of libjfx.so, the MI355X (gfx950) sort-centric algorithms
+ * layer behind jaxframes' sort_values / groupby().agg / merge / collect.
+ *
+ * The reference (jaxframes, pure Python on JAX) has no FFI: its "operator
+ * API" is the module surface of src/jaxframes/distributed/parallel_algorithms.py.
+ * Each entry point below names the reference function it replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 (JFX_OK) or a JFX_ERR_* code; jfx_last_error()
+ *    returns a thread-local message for the last failure on this thread.
+ *  - All data pointers are device pointers owned by the caller. The library
+ *    never allocates device memory: scratch lives in a caller-provided
+ *    workspace whose size comes from the matching *_workspace_size call.
+ *  - `stream` is a hipStream_t (NULL = default stream). Calls enqueue work on
+ *    it. Calls whose output size is data dependent (they write the size to a
+ *    host pointer) synchronise that stream once, as the reference does with
+ *    int(np.asarray(...)) at parallel_algorithms.py:286, 712, 772.
+ *  - Device-side safety: every cross-workgroup wait is bounded; a wait that
+ *    runs out raises a device error word, reported by jfx_device_error().
+ *  - Calls on different streams may overlap if they use different
+ *    workspaces; calls sharing a workspace must be ordered.
+ */
+#ifndef JFX_H
+#define JFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define JFX_OK 0
+#define JFX_ERR_INVALID 1      /* bad argument (null pointer, size, type) */
+#define JFX_ERR_HIP 2          /* a HIP runtime call failed */
+#define JFX_ERR_UNSUPPORTED 3  /* type/size combination not supported */
+#define JFX_ERR_WORKSPACE 4    /* workspace too small */
+#define JFX_ERR_DEVICE 5       /* a kernel raised the device error word */
+
+/* key / value type codes */
+#define JFX_U32 0
+#define JFX_I32 1
+#define JFX_F32 2
+#define JFX_U64 3
+#define JFX_I64 4
+#define JFX_F64 5
+
+/* aggregation bit flags for jfx_groupby */
+#define JFX_AGG_SUM 1
+#define JFX_AGG_COUNT 2
+#define JFX_AGG_MIN 4
+#define JFX_AGG_MAX 8
+#define JFX_AGG_MEAN 16
+
+/* comparison codes for jfx_filter_mask */
+#define JFX_CMP_GT 0
+#define JFX_CMP_GE 1
+#define JFX_CMP_LT 2
+#define JFX_CMP_LE 3
+#define JFX_CMP_EQ 4
+#define JFX_CMP_NE 5
+
+/* ---- library --------------------------------------------------------- */
+const char* jfx_last_error(void);
+int jfx_version(void);
+/* Synchronises `stream`, reads and clears the device error word. */
+int jfx_device_error(void* stream, uint32_t* out_code);
+
+/* ---- radix sort ------------------------------------------------------- */
+/* Workspace bytes for sorting n keys of key_type with pay_bytes (0, 4, 8)
+ * of payload per key. */
+int jfx_sort_workspace_size(int key_type, int pay_bytes, uint64_t n, uint64_t* out_bytes);
+
+/* Stable LSD radix sort of keys with an optional payload.
+ * Replaces ParallelRadixSort.sort (parallel_algorithms.py:463-581) and
+ * parallel_sort (parallel_algorithms.py:1079-1114).
+ *  - keys_in (n keys) is not modified; keys_out receives the sorted keys when
+ *    want_keys != 0 (it is also used as a ping-pong buffer, so it must hold n
+ *    keys and must not alias keys_in).
+ *  - pay_bytes 0: no payload. 4 or 8: pay_in is permuted with the keys into
+ *    pay_out; pay_bytes 4 with pay_in == NULL writes the uint32 row index
+ *    (the reference's jnp.arange payload).
+ *  - descending != 0 returns the reverse of the ascending order, so ties come
+ *    out in reverse original order (parallel_algorithms.py:552-555).
+ *  - digit_bits: 0 = autotuned, else 8 or 11.
+ *  - Synchronises the stream once to read the digit plan. */
+int jfx_sort(int key_type, const void* keys_in, void* keys_out, int want_keys,
+             int pay_bytes, const void* pay_in, void* pay_out, uint64_t n,
+             int descending, int digit_bits, void* workspace, uint64_t ws_bytes,
+             void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JFX_H */

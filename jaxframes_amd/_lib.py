@@ -1,0 +1,115 @@
+"""ctypes binding of libjfx.so (the HIP kernels behind include/jfx.h).
+
+The library is built in-tree by ``jaxframes_amd.build``. There is no CPU
+fallback: if the library or a GPU is missing, every op raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libjfx.so"
+
+JFX_U32, JFX_I32, JFX_F32, JFX_U64, JFX_I64, JFX_F64 = range(6)
+AGG_SUM, AGG_COUNT, AGG_MIN, AGG_MAX, AGG_MEAN = 1, 2, 4, 8, 16
+CMP_CODES = {">": 0, ">=": 1, "<": 2, "<=": 3, "==": 4, "!=": 5}
+
+_ERRNAMES = {1: "invalid argument", 2: "HIP error", 3: "unsupported", 4: "workspace too small",
+             5: "device error"}
+
+
+class JfxError(RuntimeError):
+    """A libjfx call failed."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+c_u64 = ctypes.c_uint64
+c_int = ctypes.c_int
+c_vp = ctypes.c_void_p
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+# name -> argtypes (every function returns int status)
+_SIGNATURES = {
+    "jfx_version": [],
+    "jfx_device_error": [c_vp, c_u32p],
+    "jfx_sort_workspace_size": [c_int, c_int, c_u64, c_u64p],
+    "jfx_sort": [c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_u64, c_int, c_int, c_vp, c_u64,
+                 c_vp],
+}
+
+
+def lib():
+    """Load libjfx.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise JfxError(f"{LIB_PATH} is missing: run `python -m jaxframes_amd.build` "
+                           "(there is no CPU fallback)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.jfx_last_error.restype = ctypes.c_char_p
+        L.jfx_last_error.argtypes = []
+        for name, args in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return ["jfx_last_error", *_SIGNATURES.keys()]
+
+
+def check(status: int, what: str):
+    if status != 0:
+        msg = lib().jfx_last_error().decode(errors="replace")
+        raise JfxError(f"{what}: {_ERRNAMES.get(status, status)}: {msg}")
+
+
+def call(name: str, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+def stream_handle(device=None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise JfxError("no ROCm GPU is visible; jaxframes_amd runs on MI355X only")
+
+
+_ws_cache: dict = {}
+
+
+def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
+    """A cached device byte buffer of at least nbytes for (device, stream)."""
+    key = (device.index if device.index is not None else torch.cuda.current_device(),
+           stream_handle(device))
+    buf = _ws_cache.get(key)
+    if buf is None or buf.numel() < nbytes:
+        cap = max(int(nbytes * 1.25), 1 << 20)
+        buf = torch.empty(cap, dtype=torch.uint8, device=device)
+        _ws_cache[key] = buf
+    return buf
+
+
+def device_error() -> int:
+    code = ctypes.c_uint32(0)
+    call("jfx_device_error", c_vp(stream_handle()), ctypes.byref(code))
+    return code.value
+
+
+def ptr(t: torch.Tensor | None) -> c_vp:
+    return c_vp(0 if t is None else t.data_ptr())

@@ -1,0 +1,53 @@
+// Host-side runtime shared by the C-ABI entry points: error reporting,
+// workspace carving, a pinned staging buffer per thread, the device error word.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+#include "../../include/jfx.h"
+
+namespace jfx {
+
+int set_error(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define JFX_HIP(call)                                   \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return ::jfx::hip_fail(_e, #call); \
+  } while (0)
+
+#define JFX_CHECK(cond, code, msg)                      \
+  do {                                                  \
+    if (!(cond)) return ::jfx::set_error((code), (msg)); \
+  } while (0)
+
+// Bump allocator over a caller-provided workspace (256-byte aligned pieces).
+struct Carver {
+  char* base;
+  uint64_t cap;
+  uint64_t off = 0;
+  Carver(void* b, uint64_t c) : base(static_cast<char*>(b)), cap(c) {}
+  template <typename T> T* take(uint64_t count) {
+    off = (off + 255) & ~uint64_t(255);
+    T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
+    off += count * sizeof(T);
+    return p;
+  }
+  bool fits() const { return off <= cap; }
+  uint64_t used() const { return (off + 255) & ~uint64_t(255); }
+};
+
+// A per-thread pinned host buffer for small device-to-host reads.
+void* pinned_scratch(size_t bytes);
+
+// Device error word (one per device, in device memory owned by the library
+// image as a __device__ variable).
+uint32_t* device_error_word();
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace jfx
