@@ -90,6 +90,39 @@ int jfx_sort(int key_type, const void* keys_in, void* keys_out, int want_keys,
              int descending, int digit_bits, void* workspace, uint64_t ws_bytes,
              void* stream);
 
+/* ---- groupby (sort + head-flag segmented reduction) ------------------- */
+/* Group outputs (out_keys, out_sum, out_count, out_min, out_max, out_mean)
+ * are caller buffers with room for n groups; any of them may be NULL to skip
+ * that aggregation. Types: out_keys has key_type; out_sum/min/max have
+ * val_type; out_count is int64; out_mean is float32 for 4-byte value types
+ * and float64 for 8-byte ones. Float sums are TwoSum-compensated in the value
+ * type; integer sums wrap in the value type like the reference's
+ * segment_sum. Groups come out in ascending key order. */
+
+int jfx_segreduce_workspace_size(int val_type, uint64_t n, uint64_t* out_bytes);
+
+/* Segmented reduction over already-sorted keys. Replaces the head-flag /
+ * cumsum / segment_* steps of SortBasedGroupBy.groupby_aggregate
+ * (parallel_algorithms.py:765-801) and _apply_aggregation_vectorized
+ * (parallel_algorithms.py:626-660). Row i's value is vals[perm[i]] when perm
+ * (uint32 row index, e.g. the sort's payload) is non-NULL, else vals[i].
+ * When out_ngroups is non-NULL the stream is synchronised and G written. */
+int jfx_segreduce(int key_type, const void* sorted_keys, int val_type, const void* vals,
+                  const uint32_t* perm, uint64_t n, void* out_keys, void* out_sum,
+                  int64_t* out_count, void* out_min, void* out_max, void* out_mean,
+                  uint64_t* out_ngroups, void* workspace, uint64_t ws_bytes, void* stream);
+
+int jfx_groupby_workspace_size(int key_type, int val_type, uint64_t n, uint64_t* out_bytes);
+
+/* One-column groupby: radix sort of the keys carrying the values as payload,
+ * then jfx_segreduce. Replaces groupby_aggregate (parallel_algorithms.py:
+ * 1117-1148) for a single aggregated column. Synchronises once for the digit
+ * plan and once to read G into *out_ngroups. */
+int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
+                void* out_keys, void* out_sum, int64_t* out_count, void* out_min, void* out_max,
+                void* out_mean, uint64_t* out_ngroups, int digit_bits, void* workspace,
+                uint64_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,12 @@ _SIGNATURES = {
     "jfx_sort_workspace_size": [c_int, c_int, c_u64, c_u64p],
     "jfx_sort": [c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_u64, c_int, c_int, c_vp, c_u64,
                  c_vp],
+    "jfx_segreduce_workspace_size": [c_int, c_u64, c_u64p],
+    "jfx_segreduce": [c_int, c_vp, c_int, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                      c_u64p, c_vp, c_u64, c_vp],
+    "jfx_groupby_workspace_size": [c_int, c_int, c_u64, c_u64p],
+    "jfx_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64p,
+                    c_int, c_vp, c_u64, c_vp],
 }
 
 

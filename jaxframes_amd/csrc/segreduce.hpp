@@ -1,0 +1,140 @@
+// Segmented-reduction state for the groupby aggregations.
+//
+// One state per (partial) group: compensated sum (s, c), row count, min, max.
+// Floating sums use TwoSum (Knuth) compensation: every add and every combine
+// keeps the rounding error in c, so s + c carries about twice the working
+// precision. Integer sums wrap in the value type, as XLA's int segment_sum.
+#pragma once
+
+#include <math.h>
+
+#include "common.hpp"
+
+namespace jfx {
+
+template <typename V> struct ValTraits;
+template <> struct ValTraits<float> {
+  using U = uint32_t;
+  static constexpr bool is_float = true;
+  __device__ static float lowest() { return -INFINITY; }
+  __device__ static float highest() { return INFINITY; }
+};
+template <> struct ValTraits<double> {
+  using U = uint64_t;
+  static constexpr bool is_float = true;
+  __device__ static double lowest() { return -INFINITY; }
+  __device__ static double highest() { return INFINITY; }
+};
+template <> struct ValTraits<int32_t> {
+  using U = uint32_t;
+  static constexpr bool is_float = false;
+  __device__ static int32_t lowest() { return INT32_MIN; }
+  __device__ static int32_t highest() { return INT32_MAX; }
+};
+template <> struct ValTraits<int64_t> {
+  using U = uint64_t;
+  static constexpr bool is_float = false;
+  __device__ static int64_t lowest() { return INT64_MIN; }
+  __device__ static int64_t highest() { return INT64_MAX; }
+};
+template <> struct ValTraits<uint32_t> {
+  using U = uint32_t;
+  static constexpr bool is_float = false;
+  __device__ static uint32_t lowest() { return 0u; }
+  __device__ static uint32_t highest() { return 0xFFFFFFFFu; }
+};
+template <> struct ValTraits<uint64_t> {
+  using U = uint64_t;
+  static constexpr bool is_float = false;
+  __device__ static uint64_t lowest() { return 0ull; }
+  __device__ static uint64_t highest() { return ~0ull; }
+};
+
+// Exact rounding error of a + b (TwoSum, no branches, no FMA involved).
+template <typename V>
+__device__ __forceinline__ V two_sum(V a, V b, V& err) {
+  const V s = a + b;
+  const V bp = s - a;
+  const V ap = s - bp;
+  err = (a - ap) + (b - bp);
+  return s;
+}
+
+// NaN-propagating min / max (numpy minimum/maximum semantics).
+template <typename V> __device__ __forceinline__ V vmin(V a, V b) {
+  if constexpr (ValTraits<V>::is_float) {
+    if (a != a) return a;
+    if (b != b) return b;
+  }
+  return b < a ? b : a;
+}
+template <typename V> __device__ __forceinline__ V vmax(V a, V b) {
+  if constexpr (ValTraits<V>::is_float) {
+    if (a != a) return a;
+    if (b != b) return b;
+  }
+  return b > a ? b : a;
+}
+
+template <typename V>
+struct AggState {
+  V s;
+  V c;
+  uint32_t n;
+  V mn;
+  V mx;
+
+  __device__ __forceinline__ static AggState identity() {
+    AggState r;
+    r.s = V(0);
+    r.c = V(0);
+    r.n = 0u;
+    r.mn = ValTraits<V>::highest();
+    r.mx = ValTraits<V>::lowest();
+    return r;
+  }
+  __device__ __forceinline__ void add(V x) {
+    if constexpr (ValTraits<V>::is_float) {
+      V e;
+      s = two_sum(s, x, e);
+      c += e;
+    } else {
+      using U = typename ValTraits<V>::U;
+      s = (V)((U)s + (U)x);
+    }
+    n += 1u;
+    mn = vmin(mn, x);
+    mx = vmax(mx, x);
+  }
+  // this := this (earlier rows) then b (later rows)
+  __device__ __forceinline__ void merge(const AggState& b) {
+    if constexpr (ValTraits<V>::is_float) {
+      V e;
+      s = two_sum(s, b.s, e);
+      c = c + b.c + e;
+    } else {
+      using U = typename ValTraits<V>::U;
+      s = (V)((U)s + (U)b.s);
+    }
+    n += b.n;
+    mn = vmin(mn, b.mn);
+    mx = vmax(mx, b.mx);
+  }
+  __device__ __forceinline__ V sum() const {
+    if constexpr (ValTraits<V>::is_float) return s + c;
+    else return s;
+  }
+};
+
+template <typename V>
+__device__ __forceinline__ AggState<V> shfl_up_state(const AggState<V>& x, int off) {
+  AggState<V> r;
+  r.s = __shfl_up(x.s, off, 64);
+  r.c = __shfl_up(x.c, off, 64);
+  r.n = __shfl_up(x.n, off, 64);
+  r.mn = __shfl_up(x.mn, off, 64);
+  r.mx = __shfl_up(x.mx, off, 64);
+  return r;
+}
+
+}  // namespace jfx

@@ -1,0 +1,203 @@
+"""Sort-based groupby on the GPU: radix sort + head-flag segmented reduction.
+
+Mirrors SortBasedGroupBy.groupby_aggregate (parallel_algorithms.py:733-803),
+_apply_aggregation_vectorized (parallel_algorithms.py:626-660) and
+groupby_aggregate_multi_column (parallel_algorithms.py:662-731).
+
+Output conventions (value-equal to the reference; dtypes noted where they
+differ): unique keys ascending; sum and min/max keep the value dtype; mean is
+float32 for 4-byte value types and float64 for 8-byte ones; count is int64
+(the reference returns it in the value dtype). Float sums are
+TwoSum-compensated float32/float64 sums, not bit-equal to XLA's scatter-add.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._tensors import key_code, to_device, widen_key
+from .sorting import radix_sort
+
+AGGS = ("sum", "mean", "count", "min", "max")
+
+_VAL_CODE = {
+    torch.float32: _lib.JFX_F32, torch.float64: _lib.JFX_F64,
+    torch.int32: _lib.JFX_I32, torch.int64: _lib.JFX_I64,
+    torch.uint32: _lib.JFX_U32, torch.uint64: _lib.JFX_U64,
+}
+_VAL_WIDEN = {
+    torch.int8: torch.int32, torch.int16: torch.int32, torch.uint8: torch.int32,
+    torch.uint16: torch.int32, torch.bool: torch.int32,
+    torch.float16: torch.float32, torch.bfloat16: torch.float32,
+}
+
+
+def _check_aggs(agg_funcs):
+    for col, fn in agg_funcs.items():
+        if fn not in AGGS:
+            raise ValueError(f"Unsupported aggregation function: {fn}")
+
+
+def _value_tensor(v: torch.Tensor):
+    if v.dtype in _VAL_CODE:
+        return v, v.dtype
+    if v.dtype in _VAL_WIDEN:
+        return v.to(_VAL_WIDEN[v.dtype]), v.dtype
+    raise TypeError(f"unsupported value dtype {v.dtype}")
+
+
+def _mean_dtype(v: torch.Tensor):
+    return torch.float32 if v.element_size() == 4 else torch.float64
+
+
+def _alloc_out(fn, v, cap, dev):
+    if fn in ("sum", "min", "max"):
+        return torch.empty(cap, dtype=v.dtype, device=dev)
+    if fn == "count":
+        return torch.empty(cap, dtype=torch.int64, device=dev)
+    return torch.empty(cap, dtype=_mean_dtype(v), device=dev)
+
+
+def _out_ptrs(fn, out):
+    p = {"sum": None, "count": None, "min": None, "max": None, "mean": None}
+    p[fn] = out
+    return [_lib.ptr(p[k]) for k in ("sum", "count", "min", "max", "mean")]
+
+
+def _shrink(t: torch.Tensor, g: int) -> torch.Tensor:
+    v = t[:g]
+    return v.clone() if g < t.shape[0] // 2 else v
+
+
+def _finish_col(fn, out, orig_dtype, g):
+    r = _shrink(out, g)
+    if fn in ("sum", "min", "max") and r.dtype != orig_dtype:
+        r = r.to(orig_dtype)
+    return r
+
+
+def _segreduce_ws(vcode, n, dev):
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_segreduce_workspace_size", vcode, n, ctypes.byref(nb))
+    return _lib.workspace(nb.value, dev)
+
+
+def groupby_aggregate(keys, values: dict, agg_funcs: dict, *, digit_bits: int = 0):
+    """Single-key groupby; returns (unique_keys, {col: aggregated})."""
+    _check_aggs(agg_funcs)
+    _lib.require_gpu()
+    k = to_device(keys)
+    dev = k.device
+    n = k.shape[0]
+    k, korig = widen_key(k)
+    kc = key_code(k)
+    cols = {c: _value_tensor(to_device(values[c], dev)) for c in agg_funcs}
+    for c, (v, _) in cols.items():
+        if v.shape[0] != n:
+            raise ValueError(f"column {c} has {v.shape[0]} rows, keys have {n}")
+    if n == 0:
+        return k[:0].to(korig), {c: v[:0] for c, (v, _) in cols.items()}
+    stream = ctypes.c_void_p(_lib.stream_handle(dev))
+    g = ctypes.c_uint64(0)
+    out_keys = torch.empty(n, dtype=k.dtype, device=dev)
+    results = {}
+    if len(agg_funcs) == 1:
+        (col, fn), = agg_funcs.items()
+        v, vorig = cols[col]
+        vcode = _VAL_CODE[v.dtype]
+        nb = ctypes.c_uint64(0)
+        _lib.call("jfx_groupby_workspace_size", kc, vcode, n, ctypes.byref(nb))
+        ws = _lib.workspace(nb.value, dev)
+        out = _alloc_out(fn, v, n, dev)
+        _lib.call("jfx_groupby", kc, _lib.ptr(k), vcode, _lib.ptr(v), n, _lib.ptr(out_keys),
+                  *_out_ptrs(fn, out), ctypes.byref(g), digit_bits, _lib.ptr(ws), ws.numel(),
+                  stream)
+        G = g.value
+        results[col] = _finish_col(fn, out, vorig, G)
+    else:
+        sk, _, perm = radix_sort(k, None, True, return_index=True, digit_bits=digit_bits)
+        perm32 = perm.to(torch.int32)
+        G = None
+        for i, (col, fn) in enumerate(agg_funcs.items()):
+            v, vorig = cols[col]
+            vcode = _VAL_CODE[v.dtype]
+            ws = _segreduce_ws(vcode, n, dev)
+            out = _alloc_out(fn, v, n, dev)
+            _lib.call("jfx_segreduce", kc, _lib.ptr(sk), vcode, _lib.ptr(v), _lib.ptr(perm32), n,
+                      _lib.ptr(out_keys if i == 0 else None), *_out_ptrs(fn, out),
+                      ctypes.byref(g) if i == 0 else None, _lib.ptr(ws), ws.numel(), stream)
+            if i == 0:
+                G = g.value
+            results[col] = (fn, out, vorig)
+        results = {c: _finish_col(fn, out, vorig, G) for c, (fn, out, vorig) in results.items()}
+    uk = _shrink(out_keys, G)
+    if uk.dtype != korig:
+        uk = uk.to(korig)
+    return uk, results
+
+
+def groupby_stats(keys, values, *, digit_bits: int = 0):
+    """All five aggregations of one value column in one sort + one reduction:
+    returns (unique_keys, {'sum','count','min','max','mean'}), used by the
+    serving and benchmark paths."""
+    _lib.require_gpu()
+    k = to_device(keys)
+    dev = k.device
+    n = k.shape[0]
+    k, korig = widen_key(k)
+    kc = key_code(k)
+    v, vorig = _value_tensor(to_device(values, dev))
+    vcode = _VAL_CODE[v.dtype]
+    outs = {f: _alloc_out(f, v, max(n, 1), dev) for f in ("sum", "count", "min", "max", "mean")}
+    out_keys = torch.empty(max(n, 1), dtype=k.dtype, device=dev)
+    if n == 0:
+        return k[:0].to(korig), {f: o[:0] for f, o in outs.items()}
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_groupby_workspace_size", kc, vcode, n, ctypes.byref(nb))
+    ws = _lib.workspace(nb.value, dev)
+    g = ctypes.c_uint64(0)
+    _lib.call("jfx_groupby", kc, _lib.ptr(k), vcode, _lib.ptr(v), n, _lib.ptr(out_keys),
+              *[_lib.ptr(outs[f]) for f in ("sum", "count", "min", "max", "mean")],
+              ctypes.byref(g), digit_bits, _lib.ptr(ws), ws.numel(),
+              ctypes.c_void_p(_lib.stream_handle(dev)))
+    G = g.value
+    res = {f: _finish_col(f, o, vorig, G) for f, o in outs.items()}
+    uk = _shrink(out_keys, G)
+    return (uk.to(korig) if uk.dtype != korig else uk), res
+
+
+def groupby_aggregate_multi_column(keys: list, key_names: list, values: dict, agg_funcs: dict):
+    """Multi-key groupby (parallel_algorithms.py:662-731): rows are ordered by a
+    stable lexicographic sort (last key first) and grouped where any key
+    changes; returns ({key_name: unique}, {col: aggregated})."""
+    from .multikey import lexsort_perm, multikey_heads
+    _check_aggs(agg_funcs)
+    ks = [to_device(k) for k in keys]
+    dev = ks[0].device
+    n = ks[0].shape[0]
+    if n == 0:
+        return ({nm: k[:0] for nm, k in zip(key_names, ks)},
+                {c: to_device(values[c], dev)[:0] for c in agg_funcs})
+    perm = lexsort_perm(ks)
+    sorted_keys = [k[perm] for k in ks]
+    group_id, starts = multikey_heads(sorted_keys)
+    G = starts.shape[0]
+    uniq = {nm: sk[starts] for nm, sk in zip(key_names, sorted_keys)}
+    res = {}
+    gid32 = group_id.to(torch.int32)
+    for col, fn in agg_funcs.items():
+        v, vorig = _value_tensor(to_device(values[col], dev))
+        vcode = _VAL_CODE[v.dtype]
+        ws = _segreduce_ws(vcode, n, dev)
+        out = _alloc_out(fn, v, n, dev)
+        g = ctypes.c_uint64(0)
+        # the dense group id is a sorted int32 key with the same groups
+        _lib.call("jfx_segreduce", _lib.JFX_I32, _lib.ptr(gid32), vcode, _lib.ptr(v),
+                  _lib.ptr(perm.to(torch.int32)), n, None, *_out_ptrs(fn, out), ctypes.byref(g),
+                  _lib.ptr(ws), ws.numel(), ctypes.c_void_p(_lib.stream_handle(dev)))
+        assert g.value == G
+        res[col] = _finish_col(fn, out, vorig, G)
+    return uniq, res

@@ -1,0 +1,161 @@
+"""GPU parity of the sort-based groupby against the oracle.
+
+Keys, counts, min and max are bit-exact; integer sums are exact (wrapping in
+the value dtype); float sums are TwoSum-compensated and must agree with the
+FP64 oracle to a relative error of at most SUM_REL_TOL (float32 values) per
+group, or to at most SUM_NORM_TOL of sum(|x|) over the group.
+"""
+
+import numpy as np
+import pandas as pd
+import pytest
+import torch
+
+import oracle
+from jaxframes_amd import grouping
+from jaxframes_amd._tensors import to_numpy
+from tests._util import cuda
+
+pytestmark = pytest.mark.gpu
+
+SUM_REL_TOL = 1e-6      # float32 sums: max relative error vs FP64 per group
+SUM_NORM_TOL = 2e-7     # float32 sums: max |err| / sum|x| per group
+SUM64_NORM_TOL = 1e-15  # float64 sums
+
+
+def _oracle_groups(keys, vals):
+    sk, sv, perm = oracle.radix_sort(keys, vals)
+    starts = np.flatnonzero(oracle.head_flags(sk))
+    return sk, sv, starts
+
+
+def _check_float_sums(got, sv, starts, dtype):
+    rel, nrm = oracle.sum_error_metrics(got, sv, starts)
+    if dtype == np.float32:
+        assert nrm <= SUM_NORM_TOL, (rel, nrm)
+        ref = oracle.segment_sums_f64(sv, starts)
+        # relative error only where the sum is not dominated by cancellation
+        absx = np.add.reduceat(np.abs(sv.astype(np.float64)), starts)
+        ok = np.abs(ref) > 1e-3 * absx
+        r = np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-300)
+        assert (r[ok] <= SUM_REL_TOL).all(), r[ok].max()
+    else:
+        assert nrm <= SUM64_NORM_TOL, (rel, nrm)
+
+
+@pytest.mark.parametrize("n", [1, 5, 8191, 8192, 8193, 100000, 1000003])
+@pytest.mark.parametrize("card", [1, 7, 1000, 100000, 10**9])
+def test_groupby_stats_int32_f32(n, card):
+    rng = np.random.default_rng(n + card)
+    keys = rng.integers(0, card, n).astype(np.int32) if card < 2**31 else \
+        rng.integers(-2**31, 2**31 - 1, n).astype(np.int32)
+    vals = rng.standard_normal(n).astype(np.float32)
+    uk, res = grouping.groupby_stats(torch.from_numpy(keys).cuda(), torch.from_numpy(vals).cuda())
+    sk, sv, starts = _oracle_groups(keys, vals)
+    ek = sk[starts]
+    assert np.array_equal(uk.cpu().numpy(), ek)
+    counts = np.diff(np.append(starts, n))
+    assert np.array_equal(res["count"].cpu().numpy(), counts)
+    assert np.array_equal(res["min"].cpu().numpy(), np.minimum.reduceat(sv, starts))
+    assert np.array_equal(res["max"].cpu().numpy(), np.maximum.reduceat(sv, starts))
+    _check_float_sums(res["sum"].cpu().numpy(), sv, starts, np.float32)
+    ref_mean = oracle.segment_sums_f64(sv, starts) / counts
+    np.testing.assert_allclose(res["mean"].cpu().numpy(), ref_mean, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("kdt", [np.int64, np.uint64, np.float32, np.float64, np.uint32])
+@pytest.mark.parametrize("vdt", [np.float64, np.int32, np.int64, np.float32])
+def test_groupby_types(kdt, vdt):
+    rng = np.random.default_rng(11)
+    n = 150001
+    base = rng.integers(0, 5000, n)
+    keys = (base * 1000003).astype(kdt) if np.dtype(kdt).kind != "f" else (base / 7.0).astype(kdt)
+    vals = (rng.standard_normal(n) * 1000).astype(vdt)
+    uk, res = grouping.groupby_stats(cuda(keys), cuda(vals))
+    sk, sv, starts = _oracle_groups(keys, vals)
+    assert np.array_equal(to_numpy(uk), sk[starts])
+    assert np.array_equal(res["count"].cpu().numpy(), np.diff(np.append(starts, n)))
+    assert np.array_equal(res["min"].cpu().numpy(), np.minimum.reduceat(sv, starts))
+    assert np.array_equal(res["max"].cpu().numpy(), np.maximum.reduceat(sv, starts))
+    if np.dtype(vdt).kind == "i":
+        exp = np.add.reduceat(sv.astype(np.int64), starts).astype(vdt)
+        assert np.array_equal(res["sum"].cpu().numpy(), exp)
+    else:
+        _check_float_sums(res["sum"].cpu().numpy(), sv, starts, vdt)
+
+
+def test_groupby_float_keys_nan_and_zero():
+    # heads use `!=`: every NaN key is its own group, -0.0 and 0.0 share one
+    keys = np.array([0.0, np.nan, -0.0, 1.0, np.nan, 1.0, -0.0], dtype=np.float32)
+    vals = np.arange(7, dtype=np.float32)
+    uk, res = grouping.groupby_stats(torch.from_numpy(keys).cuda(), torch.from_numpy(vals).cuda())
+    ek, eres = oracle.groupby_aggregate(keys, {"v": vals}, {"v": "count"})
+    got = uk.cpu().numpy()
+    assert got.shape == ek.shape
+    assert np.array_equal(np.isnan(got), np.isnan(ek))
+    assert np.array_equal(got[~np.isnan(got)], ek[~np.isnan(ek)])
+    assert np.array_equal(res["count"].cpu().numpy(), eres["v"])
+
+
+def test_groupby_aggregate_reference_vectors():
+    # test_parallel_algorithms.py:90-152
+    cuda = lambda x: torch.tensor(x, device="cuda")
+    uk, r = grouping.groupby_aggregate(cuda([1, 2, 1, 3, 2, 1, 3]),
+                                       {"col1": cuda([10, 20, 30, 40, 50, 60, 70])}, {"col1": "sum"})
+    assert uk.tolist() == [1, 2, 3] and r["col1"].tolist() == [100, 70, 110]
+    uk, r = grouping.groupby_aggregate(cuda([1, 2, 1, 2, 1]),
+                                       {"col1": cuda([10.0, 20.0, 30.0, 40.0, 50.0])}, {"col1": "mean"})
+    assert uk.tolist() == [1, 2]
+    np.testing.assert_allclose(r["col1"].cpu().numpy(), [30.0, 30.0], rtol=1e-6)
+    uk, r = grouping.groupby_aggregate(cuda([1, 2, 1, 2, 1]),
+                                       {"col1": cuda([10, 25, 30, 15, 20]), "col2": cuda([5, 8, 2, 9, 7])},
+                                       {"col1": "max", "col2": "min"})
+    assert uk.tolist() == [1, 2] and r["col1"].tolist() == [30, 25] and r["col2"].tolist() == [2, 8]
+    uk, r = grouping.groupby_aggregate(cuda([1, 2, 1, 3, 2, 1]),
+                                       {"col1": cuda([10, 20, 30, 40, 50, 60])}, {"col1": "count"})
+    assert uk.tolist() == [1, 2, 3] and r["col1"].tolist() == [3, 2, 1]
+    with pytest.raises(ValueError):
+        grouping.groupby_aggregate(cuda([1]), {"c": cuda([1])}, {"c": "median"})
+
+
+def test_groupby_multi_column_aggs_vs_pandas():
+    rng = np.random.default_rng(2)
+    n = 200000
+    keys = rng.integers(0, 3000, n).astype(np.int64)
+    a = rng.standard_normal(n).astype(np.float32)
+    b = rng.integers(-1000, 1000, n).astype(np.int32)
+    c = rng.standard_normal(n)
+    uk, r = grouping.groupby_aggregate(torch.from_numpy(keys).cuda(),
+                                       {"a": torch.from_numpy(a).cuda(), "b": torch.from_numpy(b).cuda(),
+                                        "c": torch.from_numpy(c).cuda()},
+                                       {"a": "sum", "b": "max", "c": "count"})
+    df = pd.DataFrame({"k": keys, "a": a, "b": b, "c": c}).groupby("k").agg({"a": "sum", "b": "max", "c": "count"})
+    assert np.array_equal(uk.cpu().numpy(), df.index.values)
+    assert np.array_equal(r["b"].cpu().numpy(), df["b"].values)
+    assert np.array_equal(r["c"].cpu().numpy(), df["c"].values)
+    np.testing.assert_allclose(r["a"].cpu().numpy(), df["a"].values, rtol=2e-4, atol=1e-4)
+
+
+def test_groupby_config1_vs_pandas():
+    # 1M rows, int32 keys uniform in [0,1000), float32 N(0,1)
+    rng = np.random.default_rng(0)
+    n = 1_000_000
+    keys = rng.integers(0, 1000, n, dtype=np.int32)
+    vals = rng.standard_normal(n, dtype=np.float32)
+    uk, res = grouping.groupby_stats(torch.from_numpy(keys).cuda(), torch.from_numpy(vals).cuda())
+    df = pd.DataFrame({"k": keys, "v": vals}).groupby("k")["v"].agg(["sum", "count", "mean"])
+    assert np.array_equal(uk.cpu().numpy(), df.index.values)
+    assert np.array_equal(res["count"].cpu().numpy(), df["count"].values)
+    sk, sv, starts = _oracle_groups(keys, vals)
+    _check_float_sums(res["sum"].cpu().numpy(), sv, starts, np.float32)
+
+
+def test_groupby_empty():
+    uk, res = grouping.groupby_aggregate(torch.empty(0, dtype=torch.int32, device="cuda"),
+                                         {"v": torch.empty(0, device="cuda")}, {"v": "sum"})
+    assert uk.numel() == 0 and res["v"].numel() == 0
+
+
+def test_no_device_error_after_groupby():
+    from jaxframes_amd import _lib
+    assert _lib.device_error() == 0

@@ -7,6 +7,7 @@ import torch
 import oracle
 from jaxframes_amd import sorting
 from jaxframes_amd._tensors import to_numpy
+from tests._util import cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -37,9 +38,7 @@ def _keys(rng, dtype, n, kind):
 
 def _check(keys_np, ascending=True, payload=None, want_index=True):
     ek, _, eperm = oracle.radix_sort(keys_np, None, ascending)
-    t = torch.from_numpy(keys_np).cuda() if keys_np.dtype not in (np.uint32, np.uint64) else \
-        torch.from_numpy(keys_np.view(np.int32 if keys_np.dtype == np.uint32 else np.int64)).cuda().view(
-            torch.uint32 if keys_np.dtype == np.uint32 else torch.uint64)
+    t = cuda(keys_np)
     if payload is None:
         sk, _, idx = sorting.radix_sort(t, None, ascending, return_index=True)
         got_k = to_numpy(sk)
@@ -82,7 +81,7 @@ def test_sort_digit_skip_and_wide_layout():
     k = _keys(rng, np.uint64, 200003, "wide_gaps")
     _check(k)
     for db in (8, 11):
-        t = torch.from_numpy(k.view(np.int64)).cuda().view(torch.uint64)
+        t = cuda(k)
         sk, _, idx = sorting.radix_sort(t, None, True, return_index=True, digit_bits=db)
         ek, _, eperm = oracle.radix_sort(k)
         assert np.array_equal(to_numpy(sk), ek)
