@@ -45,6 +45,10 @@ _SIGNATURES = {
     "jfx_segreduce_workspace_size": [c_int, c_u64, c_u64p],
     "jfx_segreduce": [c_int, c_vp, c_int, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                       c_u64p, c_vp, c_u64, c_vp],
+    "jfx_prof_enable": [c_int],
+    "jfx_prof_reset": [],
+    "jfx_prof_count": [ctypes.POINTER(c_int)],
+    "jfx_prof_get": [c_int, ctypes.c_char_p, c_u64, ctypes.POINTER(ctypes.c_double), c_u64p],
     "jfx_groupby_workspace_size": [c_int, c_int, c_u64, c_u64p],
     "jfx_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64p,
                     c_int, c_vp, c_u64, c_vp],
@@ -119,3 +123,30 @@ def device_error() -> int:
 
 def ptr(t: torch.Tensor | None) -> c_vp:
     return c_vp(0 if t is None else t.data_ptr())
+
+
+class KernelTimer:
+    """Per-kernel HIP-event timing inside libjfx (events recorded on the
+    stream each kernel is launched on). Use as a context manager around the
+    timed region, then read .stats() -> {name: (total_ms, launches)}."""
+
+    def __enter__(self):
+        call("jfx_prof_reset")
+        call("jfx_prof_enable", 1)
+        return self
+
+    def __exit__(self, *exc):
+        call("jfx_prof_enable", 0)
+
+    @staticmethod
+    def stats() -> dict:
+        n = c_int(0)
+        call("jfx_prof_count", ctypes.byref(n))
+        out = {}
+        buf = ctypes.create_string_buffer(256)
+        for i in range(n.value):
+            ms = ctypes.c_double(0.0)
+            cnt = ctypes.c_uint64(0)
+            call("jfx_prof_get", i, buf, 256, ctypes.byref(ms), ctypes.byref(cnt))
+            out[buf.value.decode()] = (ms.value, cnt.value)
+        return out

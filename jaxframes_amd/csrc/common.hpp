@@ -110,26 +110,79 @@ __device__ __forceinline__ void lookback_publish(uint32_t* desc, uint32_t tile, 
 // Decoupled lookback for one counter column: after lookback_publish, walks
 // back over earlier tiles, returns the exclusive prefix over tiles < tile and
 // publishes this tile's inclusive prefix. desc is laid out [tile][stride]
-// with this column at offset `col`.
+// with this column at offset `col`. Descriptors are fetched four tiles at a
+// time (independent loads, one round trip) and consumed in order.
 __device__ __forceinline__ uint32_t lookback_u32(uint32_t* desc, uint32_t tile, uint32_t stride,
                                                  uint32_t col, uint32_t count, uint32_t* err) {
   if (tile == 0) return 0u;
   uint32_t excl = 0;
   int64_t t = (int64_t)tile - 1;
   uint32_t spins = 0;
-  while (t >= 0) {
-    uint32_t v = ld_acq_word(desc + (size_t)t * stride + col);
-    uint32_t f = v & ~LB_VAL;
-    if (f == 0u) {
-      if (++spins > LB_SPIN_LIMIT) { atomicOr(err, DEVERR_LOOKBACK); break; }
+  bool done = false;
+  while (!done) {
+    uint32_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      v[q] = (t - q >= 0) ? ld_acq_word(desc + (size_t)(t - q) * stride + col) : LB_PFX;
+    int used = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (done || used < q) continue;
+      const uint32_t f = v[q] & ~LB_VAL;
+      if (f == 0u) continue;  // not published yet: refetch from here
+      excl += v[q] & LB_VAL;
+      used = q + 1;
+      if (f == LB_PFX) done = true;
+    }
+    if (done) break;
+    t -= used;
+    if (used == 0) {
+      if (++spins > LB_SPIN_LIMIT) {
+        atomicOr(err, DEVERR_LOOKBACK);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  st_word(desc + (size_t)tile * stride + col, LB_PFX | (excl + count));
+  return excl;
+}
+
+// Wave-parallel lookback for a single counter (desc[tile]): each round, lane
+// l reads tile-1-l; the nearest inclusive prefix ends the walk. Must be called
+// by all 64 lanes of one wave; every lane returns the exclusive prefix and
+// lane 0 publishes this tile's inclusive prefix.
+__device__ __forceinline__ uint32_t wave_lookback_u32(uint32_t* desc, uint32_t tile, uint32_t count,
+                                                      uint32_t* err) {
+  if (tile == 0) return 0u;
+  const uint32_t lane = __lane_id();
+  uint32_t excl = 0;
+  int64_t base = (int64_t)tile - 1;
+  uint32_t spins = 0;
+  while (true) {
+    const int64_t t = base - (int64_t)lane;
+    const uint32_t v = t >= 0 ? ld_acq_word(desc + t) : LB_PFX;
+    const uint32_t f = v & ~LB_VAL;
+    const uint64_t pfx = __ballot(f == LB_PFX);
+    const uint64_t zero = __ballot(f == 0u);
+    const int first = pfx ? __builtin_ctzll(pfx) : 64;
+    const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+    if (zero & need) {
+      if (++spins > LB_SPIN_LIMIT) {
+        if (lane == 0) atomicOr(err, DEVERR_LOOKBACK);
+        break;
+      }
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    excl += v & LB_VAL;
-    if (f == LB_PFX) break;
-    --t;
+    uint32_t x = ((int)lane <= first) ? (v & LB_VAL) : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    excl += x;
+    if (first < 64) break;
+    base -= 64;
   }
-  st_word(desc + (size_t)tile * stride + col, LB_PFX | (excl + count));
+  if (lane == 0) st_word(desc + tile, LB_PFX | (excl + count));
   return excl;
 }
 

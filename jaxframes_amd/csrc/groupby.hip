@@ -19,6 +19,7 @@
 #include "blockscan.hpp"
 #include "runtime.hpp"
 #include "segreduce.hpp"
+#include "sortctx.hpp"
 
 namespace jfx {
 
@@ -29,7 +30,7 @@ constexpr int SEG_W = SEG_BLOCK / 64;
 
 __host__ __device__ inline uint32_t seg_pad(uint32_t e) { return e + (e >> 5); }
 
-template <typename K, typename V, typename MT>
+template <typename K, typename V, typename MT, bool MM>
 struct SegArgs {
   const K* keys;
   int oc;
@@ -42,18 +43,31 @@ struct SegArgs {
   V* out_min;
   V* out_max;
   MT* out_mean;
-  AggState<V>* tile_lead;   // partial before the tile's first head (whole tile if none)
-  AggState<V>* tile_trail;  // partial from the tile's last head to its end
+  AggState<V, MM>* tile_lead;   // partial before the tile's first head (whole tile if none)
+  AggState<V, MM>* tile_trail;  // partial from the tile's last head to its end
   uint2* tile_info;         // {heads before the tile, heads in the tile}
   uint32_t* desc;
   uint32_t* tile_counter;
   uint32_t* err;
   uint32_t* ngroups;        // device scalar G
   uint32_t num_tiles;
+  const uint32_t* seg_start; // known-segment mode: G+1 starts (last = n)
+  uint32_t nseg;             // known-segment mode: G (<= 2048)
 };
 
-template <typename V, typename MT>
-__device__ __forceinline__ MT mean_of(const AggState<V>& st) {
+// First index i in s[0..m) with s[i] >= x (s strictly increasing).
+__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t* s, uint32_t m, uint64_t x) {
+  uint32_t lo = 0, hi = m;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)s[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <typename V, typename MT, bool MM>
+__device__ __forceinline__ MT mean_of(const AggState<V, MM>& st) {
   if constexpr (ValTraits<V>::is_float) {
     return (MT)st.sum() / (MT)(st.n ? st.n : 1u);
   } else {
@@ -61,18 +75,22 @@ __device__ __forceinline__ MT mean_of(const AggState<V>& st) {
   }
 }
 
-template <typename K, typename V, typename MT>
-__device__ __forceinline__ void finalize(const SegArgs<K, V, MT>& a, uint32_t g, const AggState<V>& st) {
+template <typename K, typename V, typename MT, bool MM>
+__device__ __forceinline__ void finalize(const SegArgs<K, V, MT, MM>& a, uint32_t g,
+                                         const AggState<V, MM>& st) {
   if (a.out_sum) a.out_sum[g] = st.sum();
   if (a.out_count) a.out_count[g] = (int64_t)st.n;
-  if (a.out_min) a.out_min[g] = st.mn;
-  if (a.out_max) a.out_max[g] = st.mx;
-  if (a.out_mean) a.out_mean[g] = mean_of<V, MT>(st);
+  if constexpr (MM) {
+    if (a.out_min) a.out_min[g] = st.mn;
+    if (a.out_max) a.out_max[g] = st.mx;
+  }
+  if (a.out_mean) a.out_mean[g] = mean_of<V, MT, MM>(st);
 }
 
 // (flag, state) segmented combine: earlier then later.
-template <typename V>
-__device__ __forceinline__ void seg_combine(uint32_t& fa, AggState<V>& a, uint32_t fb, const AggState<V>& b) {
+template <typename V, bool MM>
+__device__ __forceinline__ void seg_combine(uint32_t& fa, AggState<V, MM>& a, uint32_t fb,
+                                            const AggState<V, MM>& b) {
   if (fb) {
     a = b;
   } else {
@@ -83,17 +101,18 @@ __device__ __forceinline__ void seg_combine(uint32_t& fa, AggState<V>& a, uint32
 
 // Workgroup exclusive segmented scan of (f, st) in thread order. Returns the
 // exclusive value in (ef, est) and the workgroup total in (tf, tst).
-template <typename V, int W>
-__device__ __forceinline__ void block_seg_scan(uint32_t f, const AggState<V>& st, uint32_t& ef,
-                                               AggState<V>& est, uint32_t& tf, AggState<V>& tst,
-                                               AggState<V>* s_wst, uint32_t* s_wf) {
+template <typename V, bool MM, int W>
+__device__ __forceinline__ void block_seg_scan(uint32_t f, const AggState<V, MM>& st, uint32_t& ef,
+                                               AggState<V, MM>& est, uint32_t& tf,
+                                               AggState<V, MM>& tst, AggState<V, MM>* s_wst,
+                                               uint32_t* s_wf) {
   const uint32_t lane = lane_id();
   const int wid = threadIdx.x >> 6;
   uint32_t incf = f;
-  AggState<V> inc = st;
+  AggState<V, MM> inc = st;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
-    AggState<V> up = shfl_up_state(inc, off);
+    AggState<V, MM> up = shfl_up_state(inc, off);
     const uint32_t fu = __shfl_up(incf, off, 64);
     if (lane >= (uint32_t)off) {
       if (!incf) {
@@ -104,10 +123,10 @@ __device__ __forceinline__ void block_seg_scan(uint32_t f, const AggState<V>& st
     }
   }
   // exclusive within the wave
-  AggState<V> ex = shfl_up_state(inc, 1);
+  AggState<V, MM> ex = shfl_up_state(inc, 1);
   uint32_t exf = __shfl_up(incf, 1, 64);
   if (lane == 0) {
-    ex = AggState<V>::identity();
+    ex = AggState<V, MM>::identity();
     exf = 0u;
   }
   if (lane == 63) {
@@ -117,10 +136,10 @@ __device__ __forceinline__ void block_seg_scan(uint32_t f, const AggState<V>& st
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t cf = 0u;
-    AggState<V> c = AggState<V>::identity();
+    AggState<V, MM> c = AggState<V, MM>::identity();
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      const AggState<V> t = s_wst[w];
+      const AggState<V, MM> t = s_wst[w];
       const uint32_t tfw = s_wf[w];
       s_wst[w] = c;
       s_wf[w] = cf;
@@ -138,20 +157,28 @@ __device__ __forceinline__ void block_seg_scan(uint32_t f, const AggState<V>& st
   __syncthreads();
 }
 
-template <typename K, typename V, typename MT>
-__global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT> a) {
-  __shared__ K s_k[SEG_TILE + SEG_TILE / 32];
+// KNOWN: the groups are given as sorted row ranges (seg_start) instead of
+// being detected from the keys, so no keys are read and no lookback is needed.
+template <typename K, typename V, typename MT, bool MM, bool KNOWN>
+__global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT, MM> a) {
+  __shared__ K s_k[KNOWN ? 1 : SEG_TILE + SEG_TILE / 32];
+  __shared__ uint32_t s_starts[KNOWN ? 2049 : 1];
   __shared__ V s_v[SEG_TILE + SEG_TILE / 32];
   __shared__ uint32_t s_cnt[SEG_BLOCK];
   __shared__ uint32_t s_tmp[SEG_W + 1];
-  __shared__ AggState<V> s_wst[SEG_W + 1];
+  __shared__ AggState<V, MM> s_wst[SEG_W + 1];
   __shared__ uint32_t s_wf[SEG_W + 1];
   __shared__ uint32_t s_tile, s_excl;
 
   const int tid = threadIdx.x;
-  if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
+  // Tiles that look back take their index from an atomic counter (every
+  // earlier tile is then already running); known-segment tiles do not wait on
+  // each other and use the workgroup index.
+  if constexpr (!KNOWN) {
+    if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+    __syncthreads();
+  }
+  const uint32_t tile = KNOWN ? blockIdx.x : s_tile;
   const uint64_t tbase = (uint64_t)tile * SEG_TILE;
   const uint32_t tn = (uint32_t)((a.n - tbase) < (uint64_t)SEG_TILE ? (a.n - tbase) : (uint64_t)SEG_TILE);
 
@@ -160,45 +187,69 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT>
   for (int i = 0; i < SEG_IPT; ++i) {
     const uint32_t e = (uint32_t)i * SEG_BLOCK + tid;
     if (e < tn) {
-      s_k[seg_pad(e)] = a.keys[tbase + e];
+      if constexpr (!KNOWN) s_k[seg_pad(e)] = a.keys[tbase + e];
       s_v[seg_pad(e)] = a.perm ? a.vals[a.perm[tbase + e]] : a.vals[tbase + e];
     }
   }
   K prevk = K(0);
-  if (tid == 0 && tbase > 0) prevk = a.keys[tbase - 1];
+  if constexpr (KNOWN) {
+    for (uint32_t i = tid; i < a.nseg; i += SEG_BLOCK) s_starts[i] = a.seg_start[i];
+  } else {
+    if (tid == 0 && tbase > 0) prevk = a.keys[tbase - 1];
+  }
   __syncthreads();
 
   // blocked: thread t owns rows t*IPT .. t*IPT+IPT-1 of the tile
   const uint32_t e0 = (uint32_t)tid * SEG_IPT;
-  if (tid > 0 && e0 - 1 < tn) prevk = s_k[seg_pad(e0 - 1)];
   K kj[SEG_IPT];
   V vj[SEG_IPT];
   uint32_t hmask = 0u;
+  uint32_t T, toff, texcl;
+  if constexpr (KNOWN) {
+    texcl = lds_lower_bound(s_starts, a.nseg, tbase);
+    T = lds_lower_bound(s_starts, a.nseg, tbase + tn) - texcl;
+    uint32_t nxt = lds_lower_bound(s_starts, a.nseg, tbase + e0);
+    toff = nxt - texcl;
 #pragma unroll
-  for (int j = 0; j < SEG_IPT; ++j) {
-    const uint32_t e = e0 + j;
-    if (e < tn) {
-      kj[j] = s_k[seg_pad(e)];
-      vj[j] = s_v[seg_pad(e)];
-      const K pk = j ? kj[j - 1] : prevk;
-      const bool h = (tbase + e == 0) || !key_equal(kj[j], pk, a.oc);
-      hmask |= (uint32_t)h << j;
+    for (int j = 0; j < SEG_IPT; ++j) {
+      const uint32_t e = e0 + j;
+      if (e < tn) {
+        vj[j] = s_v[seg_pad(e)];
+        if (nxt < a.nseg && (uint64_t)s_starts[nxt] == tbase + e) {
+          hmask |= 1u << j;
+          ++nxt;
+        }
+      }
     }
+  } else {
+    if (tid > 0 && e0 - 1 < tn) prevk = s_k[seg_pad(e0 - 1)];
+#pragma unroll
+    for (int j = 0; j < SEG_IPT; ++j) {
+      const uint32_t e = e0 + j;
+      if (e < tn) {
+        kj[j] = s_k[seg_pad(e)];
+        vj[j] = s_v[seg_pad(e)];
+        const K pk = j ? kj[j - 1] : prevk;
+        const bool h = (tbase + e == 0) || !key_equal(kj[j], pk, a.oc);
+        hmask |= (uint32_t)h << j;
+      }
+    }
+    s_cnt[tid] = (uint32_t)__popc(hmask);
+    __syncthreads();
+    T = block_excl_scan_lds<SEG_BLOCK, SEG_BLOCK>(s_cnt, s_tmp);
+    toff = s_cnt[tid];
+    if (tid == 0) lookback_publish(a.desc, tile, 1u, 0u, T);
+    if (tid < 64) {
+      const uint32_t ex = wave_lookback_u32(a.desc, tile, T, a.err);
+      if (tid == 0) s_excl = ex;
+    }
+    __syncthreads();
+    texcl = s_excl;
   }
-  s_cnt[tid] = (uint32_t)__popc(hmask);
-  __syncthreads();
-  const uint32_t T = block_excl_scan_lds<SEG_BLOCK, SEG_BLOCK>(s_cnt, s_tmp);
-  const uint32_t toff = s_cnt[tid];
-  if (tid == 0) {
-    lookback_publish(a.desc, tile, 1u, 0u, T);
-    s_excl = lookback_u32(a.desc, tile, 1u, 0u, T, a.err);
-  }
-  __syncthreads();
-  const uint32_t texcl = s_excl;
 
   // sequential reduction of the thread's rows
-  AggState<V> cur = AggState<V>::identity();
-  AggState<V> lead = AggState<V>::identity();
+  AggState<V, MM> cur = AggState<V, MM>::identity();
+  AggState<V, MM> lead = AggState<V, MM>::identity();
   uint32_t seen = 0u;
   uint32_t gid = texcl + toff - 1u;  // segment open at the thread's start
 #pragma unroll
@@ -210,8 +261,10 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT>
         else lead = cur;
         seen = 1u;
         ++gid;
-        if (a.out_keys) a.out_keys[gid] = kj[j];
-        cur = AggState<V>::identity();
+        if constexpr (!KNOWN) {
+          if (a.out_keys) a.out_keys[gid] = kj[j];
+        }
+        cur = AggState<V, MM>::identity();
       }
       cur.add(vj[j]);
     }
@@ -219,10 +272,10 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT>
   if (!seen) lead = cur;
 
   uint32_t ef, tf;
-  AggState<V> est, tst;
-  block_seg_scan<V, SEG_W>(seen, seen ? cur : lead, ef, est, tf, tst, s_wst, s_wf);
+  AggState<V, MM> est, tst;
+  block_seg_scan<V, MM, SEG_W>(seen, seen ? cur : lead, ef, est, tf, tst, s_wst, s_wf);
   if (seen) {
-    AggState<V> x = est;
+    AggState<V, MM> x = est;
     x.merge(lead);
     if (ef) finalize(a, texcl + toff - 1u, x);  // group began inside this tile
     else a.tile_lead[tile] = x;                  // group began before this tile
@@ -234,37 +287,103 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT>
   }
 }
 
-template <typename K, typename V, typename MT>
-__global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT> a) {
-  __shared__ AggState<V> s_wst[16 + 1];
+// One workgroup; thread t owns a contiguous run of tiles. Pass 1 folds the
+// run into one (flag, state) element, a workgroup segmented scan gives each
+// thread the state of the group open at its first tile, pass 2 walks the run
+// again and finalises every group that ends at a tile's first head.
+template <typename K, typename V, typename MT, bool MM>
+__global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT, MM> a) {
+  __shared__ AggState<V, MM> s_wst[16 + 1];
   __shared__ uint32_t s_wf[16 + 1];
-  const int tid = threadIdx.x;
-  uint32_t cf = 0u;
-  AggState<V> carry = AggState<V>::identity();
-  for (uint32_t c0 = 0; c0 < a.num_tiles; c0 += 1024u) {
-    const uint32_t u = c0 + tid;
-    const bool valid = u < a.num_tiles;
-    uint2 info = valid ? a.tile_info[u] : make_uint2(0u, 0u);
-    const uint32_t hh = info.y > 0u ? 1u : 0u;
-    AggState<V> lead = valid ? a.tile_lead[u] : AggState<V>::identity();
-    AggState<V> el = hh ? a.tile_trail[u] : lead;
-    uint32_t ef, tf;
-    AggState<V> est, tst;
-    block_seg_scan<V, 16>(valid ? hh : 0u, el, ef, est, tf, tst, s_wst, s_wf);
-    uint32_t ff = cf;
-    AggState<V> full = carry;
-    seg_combine(ff, full, ef, est);
-    if (valid && hh && info.x > 0u) {
-      full.merge(lead);
-      finalize(a, info.x - 1u, full);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nt = a.num_tiles;
+  const uint32_t per = (nt + 1023u) / 1024u;
+  const uint32_t u0 = tid * per;
+  const uint32_t u1 = (u0 + per) < nt ? (u0 + per) : nt;
+  constexpr uint32_t B = 4;  // tiles fetched per round trip
+  uint32_t lf = 0u;
+  AggState<V, MM> lst = AggState<V, MM>::identity();
+  for (uint32_t u = u0; u < u1; u += B) {
+    uint32_t hh[B];
+    AggState<V, MM> el[B];
+#pragma unroll
+    for (uint32_t q = 0; q < B; ++q) {
+      const bool ok = u + q < u1;
+      hh[q] = ok && a.tile_info[u + q].y > 0u ? 1u : 0u;
+      el[q] = ok ? (hh[q] ? a.tile_trail[u + q] : a.tile_lead[u + q]) : AggState<V, MM>::identity();
     }
-    seg_combine(cf, carry, tf, tst);
+#pragma unroll
+    for (uint32_t q = 0; q < B; ++q)
+      if (u + q < u1) seg_combine(lf, lst, hh[q], el[q]);
+  }
+  uint32_t ef, tf;
+  AggState<V, MM> est, tst;
+  block_seg_scan<V, MM, 16>(lf, lst, ef, est, tf, tst, s_wst, s_wf);
+  uint32_t cf = ef;
+  AggState<V, MM> carry = est;
+  for (uint32_t u = u0; u < u1; u += B) {
+    uint2 info[B];
+    AggState<V, MM> lead[B], trail[B];
+#pragma unroll
+    for (uint32_t q = 0; q < B; ++q) {
+      const bool ok = u + q < u1;
+      info[q] = ok ? a.tile_info[u + q] : make_uint2(0u, 0u);
+      lead[q] = ok ? a.tile_lead[u + q] : AggState<V, MM>::identity();
+      trail[q] = (ok && info[q].y > 0u) ? a.tile_trail[u + q] : AggState<V, MM>::identity();
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < B; ++q) {
+      if (u + q >= u1) continue;
+      const uint32_t hq = info[q].y > 0u ? 1u : 0u;
+      if (hq && info[q].x > 0u) {
+        AggState<V, MM> full = carry;
+        full.merge(lead[q]);
+        finalize(a, info[q].x - 1u, full);
+      }
+      seg_combine(cf, carry, hq, hq ? trail[q] : lead[q]);
+    }
   }
   if (tid == 0) {
-    const uint2 last = a.tile_info[a.num_tiles - 1];
+    const uint2 last = a.tile_info[nt - 1];
     const uint32_t G = last.x + last.y;
-    if (G > 0u) finalize(a, G - 1u, carry);
+    if (G > 0u) finalize(a, G - 1u, tst);
     *a.ngroups = G;
+  }
+}
+
+// Known-segment groups of a single range-reduced counting-sort pass over
+// integer keys: digit d = ordered(key) - min is one group exactly when its
+// total is non-zero. Writes G, each group's first row and its key.
+template <typename K>
+__global__ void __launch_bounds__(1024)
+k_seg_segments(const uint32_t* total, uint32_t R, uint64_t mn, int oc, uint64_t n,
+               uint32_t* seg_start, K* out_keys, uint32_t* ngroups) {
+  __shared__ uint32_t s_base[2048];
+  __shared__ uint32_t s_gid[2048];
+  __shared__ uint32_t s_tmp[1024 / 64 + 1];
+  const int tid = threadIdx.x;
+  for (int d = tid; d < 2048; d += 1024) {
+    const uint32_t t = (uint32_t)d < R ? total[d] : 0u;
+    s_base[d] = t;
+    s_gid[d] = t ? 1u : 0u;
+  }
+  __syncthreads();
+  block_excl_scan_lds<2048, 1024>(s_base, s_tmp);
+  const uint32_t G = block_excl_scan_lds<2048, 1024>(s_gid, s_tmp);
+  for (int d = tid; d < (int)R; d += 1024) {
+    const uint32_t t = (d + 1 < 2048 ? s_gid[d + 1] : G) - s_gid[d];
+    if (t) {
+      seg_start[s_gid[d]] = s_base[d];
+      if (out_keys) {
+        K o = (K)(mn + (uint64_t)d);
+        if (oc == ORD_SIGNED) o ^= (K)1 << (sizeof(K) * 8 - 1);
+        out_keys[s_gid[d]] = o;
+      }
+    }
+  }
+  if (tid == 0) {
+    seg_start[G] = (uint32_t)n;
+    *ngroups = G;
   }
 }
 
@@ -277,7 +396,7 @@ struct SegWs {
   uint32_t* desc;
   uint32_t* counter;
   uint32_t* ngroups;
-  uint64_t bytes;
+  uint32_t* seg_start;
 };
 
 uint64_t seg_tiles(uint64_t n) { return (n + SEG_TILE - 1) / SEG_TILE; }
@@ -292,6 +411,7 @@ SegWs carve_seg(Carver& c, int vb, uint64_t n) {
   w.desc = c.take<uint32_t>(t);
   w.counter = c.take<uint32_t>(4);
   w.ngroups = c.take<uint32_t>(4);
+  w.seg_start = c.take<uint32_t>(2049);
   return w;
 }
 
@@ -301,76 +421,82 @@ __global__ void k_seg_zero(uint32_t* desc, uint64_t ntiles, uint32_t* counter) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
 }
 
-template <typename K, typename V, typename MT>
-int segreduce_typed(const SegWs& w, const void* keys, int oc, const void* vals, const uint32_t* perm,
-                    uint64_t n, void* out_keys, void* out_sum, int64_t* out_count, void* out_min,
-                    void* out_max, void* out_mean, hipStream_t s) {
-  SegArgs<K, V, MT> a{};
-  a.keys = static_cast<const K*>(keys);
-  a.oc = oc;
-  a.vals = static_cast<const V*>(vals);
-  a.perm = perm;
-  a.n = n;
-  a.out_keys = static_cast<K*>(out_keys);
-  a.out_sum = static_cast<V*>(out_sum);
-  a.out_count = out_count;
-  a.out_min = static_cast<V*>(out_min);
-  a.out_max = static_cast<V*>(out_max);
-  a.out_mean = static_cast<MT*>(out_mean);
-  a.tile_lead = static_cast<AggState<V>*>(w.lead);
-  a.tile_trail = static_cast<AggState<V>*>(w.trail);
+struct SegCall {
+  const void* keys;       // sorted keys (head mode) or null (known mode)
+  int oc;
+  const void* vals;
+  const uint32_t* perm;
+  uint64_t n;
+  void* out_keys;
+  void* out_sum;
+  int64_t* out_count;
+  void* out_min;
+  void* out_max;
+  void* out_mean;
+  bool known;
+  uint32_t nseg;          // known mode: G
+};
+
+template <typename K, typename V, typename MT, bool MM>
+int segreduce_mm(const SegWs& w, const SegCall& c, hipStream_t s) {
+  SegArgs<K, V, MT, MM> a{};
+  a.keys = static_cast<const K*>(c.keys);
+  a.oc = c.oc;
+  a.vals = static_cast<const V*>(c.vals);
+  a.perm = c.perm;
+  a.n = c.n;
+  a.out_keys = static_cast<K*>(c.out_keys);
+  a.out_sum = static_cast<V*>(c.out_sum);
+  a.out_count = c.out_count;
+  a.out_min = static_cast<V*>(c.out_min);
+  a.out_max = static_cast<V*>(c.out_max);
+  a.out_mean = static_cast<MT*>(c.out_mean);
+  a.tile_lead = static_cast<AggState<V, MM>*>(w.lead);
+  a.tile_trail = static_cast<AggState<V, MM>*>(w.trail);
   a.tile_info = w.info;
   a.desc = w.desc;
   a.tile_counter = w.counter;
   a.err = device_error_word();
   a.ngroups = w.ngroups;
-  a.num_tiles = (uint32_t)seg_tiles(n);
+  a.num_tiles = (uint32_t)seg_tiles(c.n);
+  a.seg_start = w.seg_start;
+  a.nseg = c.nseg;
   JFX_CHECK(a.err != nullptr, JFX_ERR_HIP, "cannot resolve the device error word");
-  k_seg_zero<<<64, 256, 0, s>>>(w.desc, a.num_tiles, w.counter);
+  JFX_LAUNCH("seg_zero", s, (k_seg_zero<<<64, 256, 0, s>>>(w.desc, a.num_tiles, w.counter)));
   JFX_HIP(hipGetLastError());
-  k_segreduce_tiles<K, V, MT><<<a.num_tiles, SEG_BLOCK, 0, s>>>(a);
+  if (c.known) {
+    JFX_LAUNCH("segreduce_tiles", s, (k_segreduce_tiles<K, V, MT, MM, true><<<a.num_tiles, SEG_BLOCK, 0, s>>>(a)));
+  } else {
+    JFX_LAUNCH("segreduce_tiles", s, (k_segreduce_tiles<K, V, MT, MM, false><<<a.num_tiles, SEG_BLOCK, 0, s>>>(a)));
+  }
   JFX_HIP(hipGetLastError());
-  k_segreduce_fix<K, V, MT><<<1, 1024, 0, s>>>(a);
+  JFX_LAUNCH("segreduce_fix", s, (k_segreduce_fix<K, V, MT, MM><<<1, 1024, 0, s>>>(a)));
   JFX_HIP(hipGetLastError());
   return JFX_OK;
 }
 
+template <typename K, typename V, typename MT>
+int segreduce_typed(const SegWs& w, const SegCall& c, hipStream_t s) {
+  if (c.out_min || c.out_max) return segreduce_mm<K, V, MT, true>(w, c, s);
+  return segreduce_mm<K, V, MT, false>(w, c, s);
+}
+
 template <typename K>
-int segreduce_key(const SegWs& w, int vt, const void* keys, int oc, const void* vals,
-                  const uint32_t* perm, uint64_t n, void* out_keys, void* out_sum,
-                  int64_t* out_count, void* out_min, void* out_max, void* out_mean, hipStream_t s) {
+int segreduce_key(const SegWs& w, int vt, const SegCall& c, hipStream_t s) {
   switch (vt) {
-    case JFX_F32:
-      return segreduce_typed<K, float, float>(w, keys, oc, vals, perm, n, out_keys, out_sum, out_count,
-                                              out_min, out_max, out_mean, s);
-    case JFX_F64:
-      return segreduce_typed<K, double, double>(w, keys, oc, vals, perm, n, out_keys, out_sum,
-                                                out_count, out_min, out_max, out_mean, s);
-    case JFX_I32:
-      return segreduce_typed<K, int32_t, float>(w, keys, oc, vals, perm, n, out_keys, out_sum,
-                                                out_count, out_min, out_max, out_mean, s);
-    case JFX_I64:
-      return segreduce_typed<K, int64_t, double>(w, keys, oc, vals, perm, n, out_keys, out_sum,
-                                                 out_count, out_min, out_max, out_mean, s);
-    case JFX_U32:
-      return segreduce_typed<K, uint32_t, float>(w, keys, oc, vals, perm, n, out_keys, out_sum,
-                                                 out_count, out_min, out_max, out_mean, s);
-    case JFX_U64:
-      return segreduce_typed<K, uint64_t, double>(w, keys, oc, vals, perm, n, out_keys, out_sum,
-                                                  out_count, out_min, out_max, out_mean, s);
+    case JFX_F32: return segreduce_typed<K, float, float>(w, c, s);
+    case JFX_F64: return segreduce_typed<K, double, double>(w, c, s);
+    case JFX_I32: return segreduce_typed<K, int32_t, float>(w, c, s);
+    case JFX_I64: return segreduce_typed<K, int64_t, double>(w, c, s);
+    case JFX_U32: return segreduce_typed<K, uint32_t, float>(w, c, s);
+    case JFX_U64: return segreduce_typed<K, uint64_t, double>(w, c, s);
   }
   return set_error(JFX_ERR_INVALID, "bad value type");
 }
 
-int segreduce_dispatch(const SegWs& w, int kt, const void* keys, int vt, const void* vals,
-                       const uint32_t* perm, uint64_t n, void* out_keys, void* out_sum,
-                       int64_t* out_count, void* out_min, void* out_max, void* out_mean, hipStream_t s) {
-  const int oc = order_code(kt);
-  if (key_bytes(kt) == 4)
-    return segreduce_key<uint32_t>(w, vt, keys, oc, vals, perm, n, out_keys, out_sum, out_count,
-                                   out_min, out_max, out_mean, s);
-  return segreduce_key<uint64_t>(w, vt, keys, oc, vals, perm, n, out_keys, out_sum, out_count,
-                                 out_min, out_max, out_mean, s);
+int segreduce_dispatch(const SegWs& w, int kt, int vt, const SegCall& c, hipStream_t s) {
+  if (key_bytes(kt) == 4) return segreduce_key<uint32_t>(w, vt, c, s);
+  return segreduce_key<uint64_t>(w, vt, c, s);
 }
 
 int read_ngroups(const SegWs& w, uint64_t* out, hipStream_t s) {
@@ -383,14 +509,18 @@ int read_ngroups(const SegWs& w, uint64_t* out, hipStream_t s) {
 }
 
 bool valid_vt(int vt) { return vt >= JFX_U32 && vt <= JFX_F64; }
+bool int_key(int kt) { return order_code(kt) != ORD_FLOAT; }
+
+template <typename K>
+int segments_launch(const SortCtx& c, const SegWs& w, void* out_keys, hipStream_t s) {
+  JFX_LAUNCH("seg_segments", s, (k_seg_segments<K><<<1, 1024, 0, s>>>(
+                                     c.total, c.range_radix(), c.mn, c.oc, c.n, w.seg_start,
+                                     static_cast<K*>(out_keys), w.ngroups)));
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
 
 }  // namespace
-
-int sort_dispatch(int kt, const void* keys_in, void* keys_out, int want_keys, int pay_bytes,
-                  const void* pay_in, void* pay_out, uint64_t n, int descending, int digit_bits,
-                  void* ws, uint64_t ws_bytes, hipStream_t s);
-uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n);
-int default_digit_bits(int kt, uint64_t n);
 
 }  // namespace jfx
 
@@ -418,12 +548,13 @@ int jfx_segreduce(int key_type, const void* sorted_keys, int val_type, const voi
   }
   JFX_CHECK(sorted_keys != nullptr && vals != nullptr && workspace != nullptr, JFX_ERR_INVALID,
             "null input or workspace");
-  jfx::Carver c(workspace, ws_bytes);
-  jfx::SegWs w = jfx::carve_seg(c, jfx::key_bytes(val_type), n);
-  JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "segreduce workspace too small");
+  jfx::Carver cv(workspace, ws_bytes);
+  jfx::SegWs w = jfx::carve_seg(cv, jfx::key_bytes(val_type), n);
+  JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "segreduce workspace too small");
   hipStream_t s = jfx::as_stream(stream);
-  int rc = jfx::segreduce_dispatch(w, key_type, sorted_keys, val_type, vals, perm, n, out_keys,
-                                   out_sum, out_count, out_min, out_max, out_mean, s);
+  jfx::SegCall c{sorted_keys, jfx::order_code(key_type), vals, perm, n, out_keys, out_sum,
+                 out_count, out_min, out_max, out_mean, false, 0u};
+  int rc = jfx::segreduce_dispatch(w, key_type, val_type, c, s);
   if (rc != JFX_OK) return rc;
   if (out_ngroups) return jfx::read_ngroups(w, out_ngroups, s);
   return JFX_OK;
@@ -457,21 +588,48 @@ int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, 
   }
   JFX_CHECK(keys != nullptr && vals != nullptr && workspace != nullptr, JFX_ERR_INVALID,
             "null input or workspace");
+  JFX_CHECK(digit_bits == 0 || digit_bits == 8 || digit_bits == 11, JFX_ERR_INVALID,
+            "digit_bits must be 0, 8 or 11");
   const int kb = jfx::key_bytes(key_type), vb = jfx::key_bytes(val_type);
-  jfx::Carver c(workspace, ws_bytes);
-  void* skeys = c.take<char>(n * kb);
-  void* svals = c.take<char>(n * vb);
-  jfx::SegWs w = jfx::carve_seg(c, vb, n);
+  jfx::Carver cv(workspace, ws_bytes);
+  void* skeys = cv.take<char>(n * kb);
+  void* svals = cv.take<char>(n * vb);
+  jfx::SegWs w = jfx::carve_seg(cv, vb, n);
   const uint64_t sws = jfx::sort_ws_bytes(key_type, vb, n);
-  void* sort_ws = c.take<char>(sws);
-  JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "groupby workspace too small");
+  void* sort_ws = cv.take<char>(sws);
+  JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "groupby workspace too small");
   hipStream_t s = jfx::as_stream(stream);
-  if (digit_bits == 0) digit_bits = jfx::default_digit_bits(key_type, n);
-  int rc = jfx::sort_dispatch(key_type, keys, skeys, 1, vb, vals, svals, n, 0, digit_bits, sort_ws,
-                              sws, s);
+  jfx::SortCtx ctx;
+  int rc = jfx::sort_prepare(ctx, key_type, keys, n, vb, digit_bits, sort_ws, sws, s);
   if (rc != JFX_OK) return rc;
-  rc = jfx::segreduce_dispatch(w, key_type, skeys, val_type, svals, nullptr, n, out_keys, out_sum,
-                               out_count, out_min, out_max, out_mean, s);
+  const int oc = jfx::order_code(key_type);
+  if (jfx::int_key(key_type) && ctx.single_pass()) {
+    // one counting-sort pass carries only the values; groups are the
+    // non-empty digits, so keys are neither written nor re-read
+    rc = jfx::sort_single_pass(ctx, keys, nullptr, vals, svals, 0, 0);
+    if (rc != JFX_OK) return rc;
+    rc = kb == 4 ? jfx::segments_launch<uint32_t>(ctx, w, out_keys, s)
+                 : jfx::segments_launch<uint64_t>(ctx, w, out_keys, s);
+    if (rc != JFX_OK) return rc;
+    uint32_t nseg = 0;
+    {
+      // G is needed on the host to bound the known-segment table
+      uint64_t g = 0;
+      rc = jfx::read_ngroups(w, &g, s);
+      if (rc != JFX_OK) return rc;
+      nseg = (uint32_t)g;
+    }
+    jfx::SegCall c{nullptr, oc, svals, nullptr, n, nullptr, out_sum, out_count, out_min, out_max,
+                   out_mean, true, nseg};
+    rc = jfx::segreduce_dispatch(w, key_type, val_type, c, s);
+    if (rc != JFX_OK) return rc;
+    return jfx::read_ngroups(w, out_ngroups, s);
+  }
+  rc = jfx::sort_execute(ctx, keys, skeys, 1, vals, svals, 0, 0);
+  if (rc != JFX_OK) return rc;
+  jfx::SegCall c{skeys, oc, svals, nullptr, n, out_keys, out_sum, out_count, out_min, out_max,
+                 out_mean, false, 0u};
+  rc = jfx::segreduce_dispatch(w, key_type, val_type, c, s);
   if (rc != JFX_OK) return rc;
   return jfx::read_ngroups(w, out_ngroups, s);
 }

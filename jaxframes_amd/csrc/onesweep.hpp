@@ -30,6 +30,30 @@ struct ScatterArgs {
   int iota;              // payload = row index (uint32)
 };
 
+// Wave64 multisplit match: the mask of lanes in `valid` whose D-bit digit
+// equals this lane's, using one ballot per digit bit. Per bit: a
+// sign-extended bit (0 or ~0), one ballot, and an XNOR-AND on each 32-bit
+// half of the mask (one v_bitop3 each on gfx950).
+template <int D>
+__device__ __forceinline__ uint64_t match_digit(uint32_t dg, uint64_t valid) {
+  uint32_t lo = (uint32_t)valid, hi = (uint32_t)(valid >> 32);
+#pragma unroll
+  for (int b = 0; b < D; ++b) {
+    const uint32_t x = (uint32_t)((int32_t)(dg << (31 - b)) >> 31);
+    const uint64_t bb = __ballot(x != 0u);
+    lo &= ~((uint32_t)bb ^ x);
+    hi &= ~((uint32_t)(bb >> 32) ^ x);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Bits needed to represent digits below r (at least 1).
+__host__ __device__ inline int digit_bits_for(uint32_t r) {
+  int b = 1;
+  while ((1u << b) < r) ++b;
+  return b;
+}
+
 // Radix digit: ((ordered(key) - sub) >> shift) masked by the caller.
 template <typename K>
 struct RadixDigit {
@@ -55,7 +79,10 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
   constexpr int XB = (HAS_PAY && sizeof(PT) > sizeof(K)) ? (int)sizeof(PT) : (int)sizeof(K);
   constexpr uint32_t MASK = RADIX - 1;
 
-  __shared__ uint32_t s_whist[W * RADIX];
+  // Per-wave digit counters; 16 bits suffice (a wave ranks at most 64*KPT
+  // keys) and halve the LDS footprint of the 11-bit radix.
+  using HT = typename std::conditional<(D > 8), uint16_t, uint32_t>::type;
+  __shared__ HT s_whist[W * RADIX];
   __shared__ uint32_t s_tstart[RADIX];
   __shared__ uint32_t s_delta[RADIX];
   __shared__ uint32_t s_tmp[W + 1];
@@ -69,7 +96,7 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
   const uint32_t R = a.radix_eff;
 
   if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
-  for (int i = tid; i < W * RADIX; i += BLOCK) s_whist[i] = 0u;
+  for (int i = tid; i < W * RADIX; i += BLOCK) s_whist[i] = HT(0);
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint64_t tbase = (uint64_t)tile * TILE;
@@ -99,26 +126,23 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
   for (int i = 0; i < KPT; ++i) dg[i] = dig(key[i]) & MASK;
 
   // Wave-level multisplit ranking: peers = lanes with the same digit.
-  uint32_t* wh = s_whist + wid * RADIX;
+  HT* wh = s_whist + wid * RADIX;
 #pragma unroll
   for (int i = 0; i < KPT; ++i) {
     const uint32_t li = wb + (uint32_t)i * 64u + lane;
     const bool valid = li < tn;
-    uint64_t peers = ballot64(valid);
-#pragma unroll
-    for (int b = 0; b < D; ++b) {
-      const bool bit = (dg[i] >> b) & 1u;
-      const uint64_t bb = ballot64(bit);
-      peers &= bit ? bb : ~bb;
-    }
+    const uint64_t peers = match_digit<D>(dg[i], ballot64(valid));
     if (valid) {
       const uint32_t below = mbcnt(peers);
       const uint32_t cnt = wh[dg[i]];
       slot[i] = cnt + below;
       lds_order();
-      if (below == 0u) wh[dg[i]] = cnt + (uint32_t)__popcll(peers);
+      if (below == 0u) wh[dg[i]] = (HT)(cnt + (uint32_t)__popcll(peers));
       lds_order();
     }
+    // keep each item's ballots next to their use: hoisting the ballots of
+    // all items ahead of the LDS chain exhausts the SGPRs
+    __builtin_amdgcn_sched_barrier(0);
   }
   __syncthreads();
 
@@ -128,7 +152,7 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
 #pragma unroll
     for (int w = 0; w < W; ++w) {
       const uint32_t c = s_whist[w * RADIX + d];
-      s_whist[w * RADIX + d] = sum;
+      s_whist[w * RADIX + d] = (HT)sum;
       sum += c;
     }
     s_tstart[d] = sum;

@@ -1,6 +1,7 @@
 // Host runtime: error strings, pinned staging, device error word, version.
 #include "runtime.hpp"
 
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -59,9 +60,110 @@ uint32_t* device_error_word() {
   return addr[dev];
 }
 
+namespace {
+struct ProfState {
+  std::mutex mu;
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  hipEvent_t cur_start = nullptr;
+  struct Acc {
+    double ms = 0.0;
+    uint64_t count = 0;
+  };
+  std::vector<std::pair<std::string, Acc>> acc;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+  Acc& slot(const std::string& n) {
+    for (auto& p : acc)
+      if (p.first == n) return p.second;
+    acc.emplace_back(n, Acc{});
+    return acc.back().second;
+  }
+  int collect() {
+    for (auto& p : pending) {
+      if (hipEventSynchronize(p.b) != hipSuccess) return JFX_ERR_HIP;
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) return JFX_ERR_HIP;
+      Acc& a = slot(p.name);
+      a.ms += ms;
+      a.count += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+    return JFX_OK;
+  }
+};
+ProfState g_prof;
+}  // namespace
+
+bool prof_on() { return g_prof.on; }
+
+void prof_begin(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  g_prof.cur_start = g_prof.get();
+  (void)hipEventRecord(g_prof.cur_start, s);
+}
+
+void prof_end(const char* name, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  hipEvent_t b = g_prof.get();
+  (void)hipEventRecord(b, s);
+  g_prof.pending.push_back({name, g_prof.cur_start, b});
+  g_prof.cur_start = nullptr;
+}
+
 }  // namespace jfx
 
 extern "C" {
+
+int jfx_prof_enable(int on) {
+  std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
+  jfx::g_prof.on = on != 0;
+  return JFX_OK;
+}
+
+int jfx_prof_reset(void) {
+  std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
+  int rc = jfx::g_prof.collect();
+  jfx::g_prof.acc.clear();
+  return rc;
+}
+
+int jfx_prof_count(int* out_n) {
+  JFX_CHECK(out_n != nullptr, JFX_ERR_INVALID, "out_n is null");
+  std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
+  int rc = jfx::g_prof.collect();
+  *out_n = (int)jfx::g_prof.acc.size();
+  return rc;
+}
+
+int jfx_prof_get(int i, char* name_buf, uint64_t buf_len, double* total_ms, uint64_t* count) {
+  std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
+  JFX_CHECK(i >= 0 && i < (int)jfx::g_prof.acc.size(), JFX_ERR_INVALID, "index out of range");
+  const auto& p = jfx::g_prof.acc[i];
+  if (name_buf && buf_len) {
+    size_t m = p.first.size() < buf_len - 1 ? p.first.size() : buf_len - 1;
+    memcpy(name_buf, p.first.data(), m);
+    name_buf[m] = 0;
+  }
+  if (total_ms) *total_ms = p.second.ms;
+  if (count) *count = p.second.count;
+  return JFX_OK;
+}
 
 const char* jfx_last_error(void) { return jfx::g_last_error.c_str(); }
 

@@ -14,10 +14,10 @@ namespace jfx {
 int set_error(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
-#define JFX_HIP(call)                                   \
-  do {                                                  \
-    hipError_t _e = (call);                             \
-    if (_e != hipSuccess) return ::jfx::hip_fail(_e, #call); \
+#define JFX_HIP(...)                                          \
+  do {                                                        \
+    hipError_t _e = (__VA_ARGS__);                            \
+    if (_e != hipSuccess) return ::jfx::hip_fail(_e, #__VA_ARGS__); \
   } while (0)
 
 #define JFX_CHECK(cond, code, msg)                      \
@@ -49,5 +49,22 @@ void* pinned_scratch(size_t bytes);
 uint32_t* device_error_word();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Per-kernel timing with HIP events on the launch stream (enabled by
+// jfx_prof_enable). prof_begin/prof_end bracket one launch; when profiling
+// is off they cost one branch.
+bool prof_on();
+void prof_begin(hipStream_t s);
+void prof_end(const char* name, hipStream_t s);
+
+// Launch helper: times the kernel launch statement given as the trailing
+// arguments under `name`.
+#define JFX_LAUNCH(name, stream, ...)                    \
+  do {                                                   \
+    const bool _jfx_prof = ::jfx::prof_on();             \
+    if (_jfx_prof) ::jfx::prof_begin(stream);            \
+    __VA_ARGS__;                                         \
+    if (_jfx_prof) ::jfx::prof_end(name, stream);        \
+  } while (0)
 
 }  // namespace jfx

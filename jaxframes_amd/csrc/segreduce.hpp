@@ -60,23 +60,21 @@ __device__ __forceinline__ V two_sum(V a, V b, V& err) {
   return s;
 }
 
-// NaN-propagating min / max (numpy minimum/maximum semantics).
+// NaN-propagating min / max (IEEE 754-2019 minimum/maximum: one
+// v_minimum3/v_maximum3 on gfx950 for float32). A NaN anywhere in a group
+// makes its min and max NaN, as numpy's minimum/maximum reductions do.
 template <typename V> __device__ __forceinline__ V vmin(V a, V b) {
-  if constexpr (ValTraits<V>::is_float) {
-    if (a != a) return a;
-    if (b != b) return b;
-  }
-  return b < a ? b : a;
+  if constexpr (ValTraits<V>::is_float) return __builtin_elementwise_minimum(a, b);
+  else return b < a ? b : a;
 }
 template <typename V> __device__ __forceinline__ V vmax(V a, V b) {
-  if constexpr (ValTraits<V>::is_float) {
-    if (a != a) return a;
-    if (b != b) return b;
-  }
-  return b > a ? b : a;
+  if constexpr (ValTraits<V>::is_float) return __builtin_elementwise_maximum(a, b);
+  else return b > a ? b : a;
 }
 
-template <typename V>
+// MM = false drops the min / max fields from every update and exchange when
+// no min or max output is requested.
+template <typename V, bool MM = true>
 struct AggState {
   V s;
   V c;
@@ -103,8 +101,10 @@ struct AggState {
       s = (V)((U)s + (U)x);
     }
     n += 1u;
-    mn = vmin(mn, x);
-    mx = vmax(mx, x);
+    if constexpr (MM) {
+      mn = vmin(mn, x);
+      mx = vmax(mx, x);
+    }
   }
   // this := this (earlier rows) then b (later rows)
   __device__ __forceinline__ void merge(const AggState& b) {
@@ -117,8 +117,10 @@ struct AggState {
       s = (V)((U)s + (U)b.s);
     }
     n += b.n;
-    mn = vmin(mn, b.mn);
-    mx = vmax(mx, b.mx);
+    if constexpr (MM) {
+      mn = vmin(mn, b.mn);
+      mx = vmax(mx, b.mx);
+    }
   }
   __device__ __forceinline__ V sum() const {
     if constexpr (ValTraits<V>::is_float) return s + c;
@@ -126,14 +128,20 @@ struct AggState {
   }
 };
 
-template <typename V>
-__device__ __forceinline__ AggState<V> shfl_up_state(const AggState<V>& x, int off) {
-  AggState<V> r;
+template <typename V, bool MM>
+__device__ __forceinline__ AggState<V, MM> shfl_up_state(const AggState<V, MM>& x, int off) {
+  AggState<V, MM> r;
   r.s = __shfl_up(x.s, off, 64);
-  r.c = __shfl_up(x.c, off, 64);
+  if constexpr (ValTraits<V>::is_float) r.c = __shfl_up(x.c, off, 64);
+  else r.c = V(0);
   r.n = __shfl_up(x.n, off, 64);
-  r.mn = __shfl_up(x.mn, off, 64);
-  r.mx = __shfl_up(x.mx, off, 64);
+  if constexpr (MM) {
+    r.mn = __shfl_up(x.mn, off, 64);
+    r.mx = __shfl_up(x.mx, off, 64);
+  } else {
+    r.mn = x.mn;
+    r.mx = x.mx;
+  }
   return r;
 }
 

@@ -1,41 +1,64 @@
-// Stable LSD radix sort (onesweep) for 32/64-bit keys with 0/4/8-byte payloads.
+// Stable LSD radix sort for 32/64-bit keys with 0/4/8-byte payloads.
 //
 // Restates ParallelRadixSort.sort (parallel_algorithms.py:463-581): the
-// reference runs one stable argsort per 8-bit digit; here each digit is one
-// scatter pass over HBM. Pipeline per call:
-//   k_sort_init     zero histograms / counters, seed min/max
-//   k_sort_upfront  one read of the keys: min/max of the ordered keys, the
-//                   digit histograms of every pass of the layout, and a
-//                   histogram of the low 11 bits
-//   k_sort_plan     per-pass skip flags (a digit shared by all keys) and
-//                   exclusive digit offsets; offsets for a single range-reduced
-//                   pass when max-min < 2048 (digit = key - min)
-//   host            reads the 5-word plan header (one stream sync)
-//   k_onesweep x P  one scatter pass per varying digit
+// reference runs one stable argsort per 8-bit digit over all digits; here
+// each varying digit is one scatter pass over HBM. Pipeline per call:
+//   k_sort_stats     one read of the keys, one workgroup per chunk: min, max,
+//                    AND and OR of the ordered keys, and each chunk's
+//                    histogram of the low 11 bits
+//   host             reads the four statistics (one stream sync)
+//   max-min < 2048:  one chunked pass with digit = ordered(key) - min, its
+//                    per-chunk counts being the chunk histograms rotated by min
+//   otherwise:       k_sort_hist (histograms of the varying layout digits),
+//                    k_sort_plan (digit offsets), then one pass per varying
+//                    digit: the first (digit 0) chunked, the rest onesweep
+//                    with decoupled lookback. Digits on which all keys agree
+//                    are skipped (the reference's skip test at :512 only
+//                    skips leading zero digits and never fires for signed keys).
+// Descending order is the reversed ascending order (:552-555), written by the
+// last pass at n-1-pos.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
+#include "chunked.hpp"
 #include "onesweep.hpp"
-#include "runtime.hpp"
+#include "sortctx.hpp"
 
 namespace jfx {
 
-constexpr int MAXP = 8;          // most layout passes (64-bit keys, 8-bit digits)
-constexpr int MAXR = 2048;       // largest radix
-constexpr int SORT_W = 8;        // waves per scatter workgroup
-constexpr int SORT_KPT = 16;     // keys per thread
-constexpr int SORT_TILE = SORT_W * 64 * SORT_KPT;
-constexpr int UPFRONT_BLOCK = 1024;
+constexpr int OS_W = 8;      // onesweep: waves per workgroup
+constexpr int OS_KPT = 16;   // onesweep: keys per thread
+constexpr int OS_TILE = OS_W * 64 * OS_KPT;
+constexpr int CH_W = 8;      // chunked scatter: waves per workgroup
+constexpr int STATS_BLOCK = 1024;
+constexpr int HIST_BLOCK = 1024;
 
-__global__ void k_sort_init(uint32_t* hist, uint32_t nhist, unsigned long long* minmax,
-                            uint32_t* counters, uint32_t ncounters) {
+// Chunks are whole multiples of this many keys; every scatter variant's tile
+// divides it.
+template <typename K> constexpr int ch_granule() { return sizeof(K) == 4 ? 8192 : 4096; }
+
+// Chunk-scatter variant: 0 = plain, 1 = software-pipelined loads (JFX_CHUNK_CFG
+// overrides the default for tuning runs).
+int chunk_cfg() {
+  static int cfg = [] {
+    const char* e = getenv("JFX_CHUNK_CFG");
+    return e ? atoi(e) : 0;
+  }();
+  return cfg;
+}
+
+__global__ void k_sort_init(unsigned long long* stats, uint32_t* counters, uint32_t ncounters,
+                            uint32_t* hist, uint32_t nhist) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nhist; i += stride) hist[i] = 0u;
   if (blockIdx.x == 0) {
     if (threadIdx.x < ncounters) counters[threadIdx.x] = 0u;
     if (threadIdx.x == 0) {
-      minmax[0] = ~0ull;
-      minmax[1] = 0ull;
+      stats[0] = ~0ull;  // min
+      stats[1] = 0ull;   // max
+      stats[2] = ~0ull;  // and
+      stats[3] = 0ull;   // or
     }
   }
 }
@@ -45,293 +68,479 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-template <typename K, int D>
-__device__ __forceinline__ void upfront_one(K raw, int oc, uint32_t* s_h, uint32_t* s_low, K& mn,
-                                            K& mx) {
-  constexpr int RADIX = 1 << D;
-  constexpr int NP = (int)(sizeof(K) * 8 + D - 1) / D;
-  const K o = ordered(raw, oc);
-  mn = o < mn ? o : mn;
-  mx = o > mx ? o : mx;
-  atomicAdd(&s_low[(uint32_t)o & 2047u], 1u);
-#pragma unroll
-  for (int p = 0; p < NP; ++p)
-    atomicAdd(&s_h[p * RADIX + (uint32_t)((o >> (p * D)) & (K)(RADIX - 1))], 1u);
-}
+template <typename K>
+struct Stats4 {
+  K mn, mx, an, orr;
+  __device__ __forceinline__ void init() {
+    mn = ~K(0);
+    mx = K(0);
+    an = ~K(0);
+    orr = K(0);
+  }
+  __device__ __forceinline__ void add(K o) {
+    mn = o < mn ? o : mn;
+    mx = o > mx ? o : mx;
+    an &= o;
+    orr |= o;
+  }
+};
 
-template <typename K, int D>
-__global__ void __launch_bounds__(UPFRONT_BLOCK)
-k_sort_upfront(const K* __restrict__ keys, uint64_t n, int oc, uint32_t* g_hist, uint32_t* g_low,
-               unsigned long long* g_minmax) {
-  constexpr int RADIX = 1 << D;
-  constexpr int NP = (int)(sizeof(K) * 8 + D - 1) / D;
-  __shared__ uint32_t s_h[NP * RADIX];
+// One workgroup per chunk: statistics and the chunk's low-11-bit histogram.
+template <typename K>
+__global__ void __launch_bounds__(STATS_BLOCK)
+k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uint32_t* counts,
+             unsigned long long* g_stats) {
   __shared__ uint32_t s_low[2048];
-  for (int i = threadIdx.x; i < NP * RADIX; i += blockDim.x) s_h[i] = 0u;
-  for (int i = threadIdx.x; i < 2048; i += blockDim.x) s_low[i] = 0u;
+  for (int i = threadIdx.x; i < 2048; i += STATS_BLOCK) s_low[i] = 0u;
   __syncthreads();
-  K mn = ~K(0), mx = K(0);
+  const uint64_t cbeg = (uint64_t)blockIdx.x * chunk;
+  const uint64_t cend = (cbeg + chunk) < n ? (cbeg + chunk) : n;
+  Stats4<K> st;
+  st.init();
   constexpr int VEC = 16 / sizeof(K);
-  const uint64_t nvec = ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0) ? n / VEC : 0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  const uint4* kv = reinterpret_cast<const uint4*>(keys);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvec; i += stride) {
+  const bool aligned = ((reinterpret_cast<uintptr_t>(keys + cbeg) & 15u) == 0);
+  const uint64_t nv = aligned ? (cend - cbeg) / VEC : 0;
+  const uint4* kv = reinterpret_cast<const uint4*>(keys + cbeg);
+  uint64_t i = threadIdx.x;
+  for (; i + 3 * STATS_BLOCK < nv; i += 4 * STATS_BLOCK) {
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = kv[i + u * STATS_BLOCK];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const K* e = reinterpret_cast<const K*>(&v[u]);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const K o = ordered(e[j], oc);
+        st.add(o);
+        atomicAdd(&s_low[(uint32_t)o & 2047u], 1u);
+      }
+    }
+  }
+  for (; i < nv; i += STATS_BLOCK) {
     uint4 v = kv[i];
     const K* e = reinterpret_cast<const K*>(&v);
 #pragma unroll
-    for (int j = 0; j < VEC; ++j) upfront_one<K, D>(e[j], oc, s_h, s_low, mn, mx);
+    for (int j = 0; j < VEC; ++j) {
+      const K o = ordered(e[j], oc);
+      st.add(o);
+      atomicAdd(&s_low[(uint32_t)o & 2047u], 1u);
+    }
   }
-  for (uint64_t i = nvec * VEC + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    upfront_one<K, D>(keys[i], oc, s_h, s_low, mn, mx);
-  // wave min/max
-  unsigned long long wmn = (unsigned long long)mn, wmx = (unsigned long long)mx;
+  for (uint64_t t = cbeg + nv * VEC + threadIdx.x; t < cend; t += STATS_BLOCK) {
+    const K o = ordered(keys[t], oc);
+    st.add(o);
+    atomicAdd(&s_low[(uint32_t)o & 2047u], 1u);
+  }
+  unsigned long long a0 = st.mn, a1 = st.mx, a2 = st.an, a3 = st.orr;
+  if (sizeof(K) == 4) a2 |= 0xFFFFFFFF00000000ull;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long a = __shfl_xor(wmn, off, 64);
-    unsigned long long b = __shfl_xor(wmx, off, 64);
-    wmn = a < wmn ? a : wmn;
-    wmx = b > wmx ? b : wmx;
+    const unsigned long long b0 = __shfl_xor(a0, off, 64), b1 = __shfl_xor(a1, off, 64);
+    const unsigned long long b2 = __shfl_xor(a2, off, 64), b3 = __shfl_xor(a3, off, 64);
+    a0 = b0 < a0 ? b0 : a0;
+    a1 = b1 > a1 ? b1 : a1;
+    a2 &= b2;
+    a3 |= b3;
   }
+  // Same-address global atomics serialise at the coherence point (~10 ns
+  // each), so the workgroup reduces in LDS and stores one partial per chunk.
+  __shared__ unsigned long long s_red[4][STATS_BLOCK / 64];
+  const int wid = threadIdx.x >> 6;
   if (lane_id() == 0) {
-    atomicMin(&g_minmax[0], wmn);
-    atomicMax(&g_minmax[1], wmx);
+    s_red[0][wid] = a0;
+    s_red[1][wid] = a1;
+    s_red[2][wid] = a2;
+    s_red[3][wid] = a3;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < NP * RADIX; i += blockDim.x) {
-    const uint32_t v = s_h[i];
-    if (v) atomicAdd(&g_hist[i], v);
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < STATS_BLOCK / 64; ++w) {
+      a0 = s_red[0][w] < a0 ? s_red[0][w] : a0;
+      a1 = s_red[1][w] > a1 ? s_red[1][w] : a1;
+      a2 &= s_red[2][w];
+      a3 |= s_red[3][w];
+    }
+    g_stats[(size_t)blockIdx.x * 4 + 0] = a0;
+    g_stats[(size_t)blockIdx.x * 4 + 1] = a1;
+    g_stats[(size_t)blockIdx.x * 4 + 2] = a2;
+    g_stats[(size_t)blockIdx.x * 4 + 3] = a3;
   }
-  for (int i = threadIdx.x; i < 2048; i += blockDim.x) {
-    const uint32_t v = s_low[i];
-    if (v) atomicAdd(&g_low[i], v);
+  for (int t = threadIdx.x; t < 2048; t += STATS_BLOCK) counts[(size_t)blockIdx.x * 2048 + t] = s_low[t];
+}
+
+// Reduce the per-chunk statistics (min, max, and, or) into out[0..3].
+__global__ void __launch_bounds__(256)
+k_stats_reduce(const unsigned long long* part, uint32_t nchunks, unsigned long long* out) {
+  __shared__ unsigned long long s[4][256];
+  unsigned long long a0 = ~0ull, a1 = 0ull, a2 = ~0ull, a3 = 0ull;
+  for (uint32_t b = threadIdx.x; b < nchunks; b += 256) {
+    const unsigned long long* p = part + (size_t)b * 4;
+    a0 = p[0] < a0 ? p[0] : a0;
+    a1 = p[1] > a1 ? p[1] : a1;
+    a2 &= p[2];
+    a3 |= p[3];
+  }
+  s[0][threadIdx.x] = a0;
+  s[1][threadIdx.x] = a1;
+  s[2][threadIdx.x] = a2;
+  s[3][threadIdx.x] = a3;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      const int o = threadIdx.x + off;
+      s[0][threadIdx.x] = s[0][o] < s[0][threadIdx.x] ? s[0][o] : s[0][threadIdx.x];
+      s[1][threadIdx.x] = s[1][o] > s[1][threadIdx.x] ? s[1][o] : s[1][threadIdx.x];
+      s[2][threadIdx.x] &= s[2][o];
+      s[3][threadIdx.x] |= s[3][o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s[0][0];
+    out[1] = s[1][0];
+    out[2] = s[2][0];
+    out[3] = s[3][0];
   }
 }
 
-// Plan header words: [0,1] min (lo,hi), [2,3] max (lo,hi), [4] skip mask.
+// Global histograms of the varying layout digits (bit p of vary set).
+template <typename K, int D>
+__global__ void __launch_bounds__(HIST_BLOCK)
+k_sort_hist(const K* __restrict__ keys, uint64_t n, int oc, uint32_t vary, uint32_t* g_hist) {
+  constexpr int RADIX = 1 << D;
+  constexpr int NP = (int)(sizeof(K) * 8 + D - 1) / D;
+  __shared__ uint32_t s_h[NP * RADIX];
+  for (int i = threadIdx.x; i < NP * RADIX; i += HIST_BLOCK) s_h[i] = 0u;
+  __syncthreads();
+  constexpr int VEC = 16 / sizeof(K);
+  const uint64_t nvec = ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0) ? n / VEC : 0;
+  const uint64_t stride = (uint64_t)gridDim.x * HIST_BLOCK;
+  const uint4* kv = reinterpret_cast<const uint4*>(keys);
+  auto one = [&](K raw) {
+    const K o = ordered(raw, oc);
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if ((vary >> p) & 1u) atomicAdd(&s_h[p * RADIX + (uint32_t)((o >> (p * D)) & (K)(RADIX - 1))], 1u);
+  };
+  uint64_t i = (uint64_t)blockIdx.x * HIST_BLOCK + threadIdx.x;
+  for (; i + 3 * stride < nvec; i += 4 * stride) {
+    uint4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = kv[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const K* e = reinterpret_cast<const K*>(&v[u]);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) one(e[j]);
+    }
+  }
+  for (; i < nvec; i += stride) {
+    uint4 v = kv[i];
+    const K* e = reinterpret_cast<const K*>(&v);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) one(e[j]);
+  }
+  for (uint64_t t = nvec * VEC + (uint64_t)blockIdx.x * HIST_BLOCK + threadIdx.x; t < n; t += stride)
+    one(keys[t]);
+  __syncthreads();
+  for (int t = threadIdx.x; t < NP * RADIX; t += HIST_BLOCK) {
+    const uint32_t v = s_h[t];
+    if (v) atomicAdd(&g_hist[t], v);
+  }
+}
+
+// Exclusive digit offsets of every varying layout pass.
 __global__ void __launch_bounds__(1024)
-k_sort_plan(int radix, int npass, uint64_t n, const uint32_t* g_hist, const uint32_t* g_low,
-            const unsigned long long* g_minmax, uint32_t* g_base, uint32_t* g_base11,
-            uint32_t* g_base8, uint32_t* hdr) {
+k_sort_plan(int radix, int npass, uint32_t vary, const uint32_t* g_hist, uint32_t* g_base) {
   __shared__ uint32_t s[2048];
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
-  __shared__ uint32_t s_skip;
   const int tid = threadIdx.x;
-  const unsigned long long mn = g_minmax[0];
-  if (tid == 0) s_skip = 0u;
   for (int p = 0; p < npass; ++p) {
-    for (int i = tid; i < 2048; i += 1024) {
-      const uint32_t c = i < radix ? g_hist[p * radix + i] : 0u;
-      s[i] = c;
-      if ((uint64_t)c == n) atomicOr(&s_skip, 1u << p);
-    }
+    if (!((vary >> p) & 1u)) continue;
+    for (int i = tid; i < 2048; i += 1024) s[i] = i < radix ? g_hist[p * radix + i] : 0u;
     __syncthreads();
     block_excl_scan_lds<2048, 1024>(s, s_tmp);
     for (int i = tid; i < radix; i += 1024) g_base[p * radix + i] = s[i];
     __syncthreads();
   }
-  // single range-reduced pass, 11-bit digit: rotate the low-11-bit histogram
-  for (int i = tid; i < 2048; i += 1024) s[(uint32_t)(i - (uint32_t)mn) & 2047u] = g_low[i];
-  __syncthreads();
-  block_excl_scan_lds<2048, 1024>(s, s_tmp);
-  for (int i = tid; i < 2048; i += 1024) g_base11[i] = s[i];
-  __syncthreads();
-  for (int i = tid; i < 2048; i += 1024) s[i] = 0u;
-  __syncthreads();
-  for (int i = tid; i < 2048; i += 1024) atomicAdd(&s[(uint32_t)(i - (uint32_t)mn) & 255u], g_low[i]);
-  __syncthreads();
-  block_excl_scan_lds<2048, 1024>(s, s_tmp);
-  for (int i = tid; i < 256; i += 1024) g_base8[i] = s[i];
-  if (tid == 0) {
-    hdr[0] = (uint32_t)mn;
-    hdr[1] = (uint32_t)(mn >> 32);
-    hdr[2] = (uint32_t)g_minmax[1];
-    hdr[3] = (uint32_t)(g_minmax[1] >> 32);
-    hdr[4] = s_skip;
-  }
 }
 
 namespace {
 
-struct SortWs {
-  void* alt_keys;
-  void* alt_pay;
-  uint32_t* hist;
-  uint32_t* low;
-  unsigned long long* minmax;
-  uint32_t* base;
-  uint32_t* base11;
-  uint32_t* base8;
-  uint32_t* counters;
-  uint32_t* hdr;
-  uint32_t* desc[2];
-  uint64_t bytes;
-};
-
-uint64_t num_tiles(uint64_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
-
-SortWs carve_sort_ws(void* ws, uint64_t cap, int kb, int pb, uint64_t n) {
-  Carver c(ws, cap);
-  SortWs w{};
-  w.alt_keys = c.take<char>(n * kb);
-  w.alt_pay = c.take<char>(n * (uint64_t)pb);
-  w.hist = c.take<uint32_t>(MAXP * MAXR);
-  w.low = c.take<uint32_t>(2048);
-  w.minmax = c.take<unsigned long long>(2);
-  w.base = c.take<uint32_t>(MAXP * MAXR);
-  w.base11 = c.take<uint32_t>(2048);
-  w.base8 = c.take<uint32_t>(256);
-  w.counters = c.take<uint32_t>(64);
-  w.hdr = c.take<uint32_t>(16);
-  w.desc[0] = c.take<uint32_t>(num_tiles(n) * MAXR);
-  w.desc[1] = c.take<uint32_t>(num_tiles(n) * MAXR);
-  w.bytes = c.used();
-  return w;
+void geometry(SortCtx& c) {
+  const uint64_t tile = c.kb == 4 ? (uint64_t)ch_granule<uint32_t>() : (uint64_t)ch_granule<uint64_t>();
+  const uint64_t tiles = (c.n + tile - 1) / tile;
+  const uint64_t nch = std::max<uint64_t>(1, std::min<uint64_t>(SORT_MAXCHUNKS, tiles));
+  const uint64_t tpc = (tiles + nch - 1) / nch;
+  c.chunk = tpc * tile;
+  c.nchunks = (uint32_t)((c.n + c.chunk - 1) / c.chunk);
 }
 
-template <typename K, typename PT, int D>
-hipError_t launch_pass(const ScatterArgs<K, PT>& a, RadixDigit<K> dig, hipStream_t s) {
-  const uint32_t tiles = (uint32_t)num_tiles(a.n);
-  k_onesweep<K, PT, D, SORT_W, SORT_KPT, RadixDigit<K>><<<tiles, SORT_W * 64, 0, s>>>(a, dig);
-  return hipGetLastError();
+uint64_t os_tiles(uint64_t n) { return (n + OS_TILE - 1) / OS_TILE; }
+
+void carve(SortCtx& c, Carver& cv) {
+  c.alt_keys = cv.take<char>(c.n * c.kb);
+  c.alt_pay = cv.take<char>(c.n * (uint64_t)c.pb);
+  c.counts = cv.take<uint32_t>((uint64_t)SORT_MAXCHUNKS * 2048);
+  c.colex = cv.take<uint32_t>((uint64_t)SORT_MAXCHUNKS * 2048);
+  c.total = cv.take<uint32_t>(2048);
+  c.hist = cv.take<uint32_t>(SORT_MAXP * SORT_MAXR);
+  c.base = cv.take<uint32_t>(SORT_MAXP * SORT_MAXR);
+  c.stats = cv.take<unsigned long long>(4);
+  c.stats_part = cv.take<unsigned long long>((uint64_t)SORT_MAXCHUNKS * 4);
+  c.counters = cv.take<uint32_t>(64);
+  c.hdr = cv.take<uint32_t>(16);
+  c.desc[0] = cv.take<uint32_t>(os_tiles(c.n) * SORT_MAXR);
+  c.desc[1] = cv.take<uint32_t>(os_tiles(c.n) * SORT_MAXR);
 }
 
 template <typename K, typename PT>
-int sort_impl(int kt, const K* keys_in, K* keys_out, int want_keys, const PT* pay_in, PT* pay_out,
-              int iota, uint64_t n, int descending, int digit_bits, void* ws, uint64_t ws_bytes,
-              hipStream_t s) {
-  const int kb = sizeof(K);
-  constexpr int pb = std::is_same<PT, NoPay>::value ? 0 : (int)sizeof(PT);
-  SortWs w = carve_sort_ws(ws, ws_bytes, kb, pb, n);
-  JFX_CHECK(w.bytes <= ws_bytes, JFX_ERR_WORKSPACE, "sort workspace too small");
-  const int oc = order_code(kt);
-  const int D = digit_bits;
-  const int radix = 1 << D;
-  const int npass = (kb * 8 + D - 1) / D;
-  uint32_t* err = device_error_word();
-  JFX_CHECK(err != nullptr, JFX_ERR_HIP, "cannot resolve the device error word");
-
-  // histograms: hist (MAXP*MAXR) and low (2048) are carved contiguously? zero both.
-  k_sort_init<<<64, 256, 0, s>>>(w.hist, MAXP * MAXR, w.minmax, w.counters, 64);
-  JFX_HIP(hipGetLastError());
-  k_fill_u32<<<8, 256, 0, s>>>(w.low, 2048, 0u);
-  JFX_HIP(hipGetLastError());
-  int dev = 0, cus = 256;
-  JFX_HIP(hipGetDevice(&dev));
-  JFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  uint64_t want_blocks = (n + UPFRONT_BLOCK * 16 - 1) / (UPFRONT_BLOCK * 16);
-  uint32_t ub = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)cus, want_blocks));
-  if (D == 8)
-    k_sort_upfront<K, 8><<<ub, UPFRONT_BLOCK, 0, s>>>(keys_in, n, oc, w.hist, w.low, w.minmax);
-  else
-    k_sort_upfront<K, 11><<<ub, UPFRONT_BLOCK, 0, s>>>(keys_in, n, oc, w.hist, w.low, w.minmax);
-  JFX_HIP(hipGetLastError());
-  k_sort_plan<<<1, 1024, 0, s>>>(radix, npass, n, w.hist, w.low, w.minmax, w.base, w.base11,
-                                 w.base8, w.hdr);
-  JFX_HIP(hipGetLastError());
-  uint32_t* h = static_cast<uint32_t*>(pinned_scratch(16 * sizeof(uint32_t)));
-  JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
-  JFX_HIP(hipMemcpyAsync(h, w.hdr, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  JFX_HIP(hipStreamSynchronize(s));
-  const uint64_t mn = (uint64_t)h[0] | ((uint64_t)h[1] << 32);
-  const uint64_t mx = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
-  const uint32_t skip = h[4];
-  const uint64_t range = mx - mn;
-
-  struct PassSpec {
-    int d;
-    uint32_t shift;
-    K sub;
-    const uint32_t* base;
-    uint32_t radix_eff;
-  };
-  std::vector<PassSpec> passes;
-  if (range < 256) {
-    passes.push_back({8, 0u, (K)mn, w.base8, (uint32_t)range + 1u});
-  } else if (range < 2048) {
-    passes.push_back({11, 0u, (K)mn, w.base11, (uint32_t)range + 1u});
-  } else {
-    for (int p = 0; p < npass; ++p)
-      if (!((skip >> p) & 1u)) passes.push_back({D, (uint32_t)(p * D), (K)0, w.base + p * radix, (uint32_t)radix});
+struct Passes {
+  static hipError_t onesweep(int D, const ScatterArgs<K, PT>& a, RadixDigit<K> dig, hipStream_t s) {
+    const uint32_t tiles = (uint32_t)os_tiles(a.n);
+    if (D == 8) {
+      JFX_LAUNCH("onesweep_d8", s, (k_onesweep<K, PT, 8, OS_W, OS_KPT, RadixDigit<K>><<<tiles, OS_W * 64, 0, s>>>(a, dig)));
+    } else {
+      JFX_LAUNCH("onesweep_d11", s, (k_onesweep<K, PT, 11, OS_W, OS_KPT, RadixDigit<K>><<<tiles, OS_W * 64, 0, s>>>(a, dig)));
+    }
+    return hipGetLastError();
   }
-  const int A = (int)passes.size();
-  const uint64_t tiles = num_tiles(n);
-  k_fill_u32<<<512, 256, 0, s>>>(w.desc[0], tiles * passes[0].radix_eff, 0u);
+  template <int D, int W, int KPT, bool PF, int WPE>
+  static void chunk_launch(const char* name, const ScatterArgs<K, PT>& a, const ChunkGeom& g,
+                           const uint32_t* colex, const uint32_t* total, RadixDigit<K> dig,
+                           hipStream_t s) {
+    JFX_LAUNCH(name, s, (k_chunk_scatter<K, PT, D, W, KPT, PF, WPE, RadixDigit<K>><<<g.nchunks, W * 64, 0, s>>>(a, g, colex, total, dig)));
+  }
+  template <int D>
+  static void chunk_cfgs(const char* name, const ScatterArgs<K, PT>& a, const ChunkGeom& g,
+                         const uint32_t* colex, const uint32_t* total, RadixDigit<K> dig,
+                         hipStream_t s) {
+    switch (chunk_cfg()) {
+      case 1: chunk_launch<D, 8, 8, false, 4>(name, a, g, colex, total, dig, s); break;
+      case 2: chunk_launch<D, 8, 8, true, 4>(name, a, g, colex, total, dig, s); break;
+      case 3: chunk_launch<D, 4, 16, false, 1>(name, a, g, colex, total, dig, s); break;
+      case 4: chunk_launch<D, 4, 16, true, 1>(name, a, g, colex, total, dig, s); break;
+      default:
+        if constexpr (sizeof(K) == 4) chunk_launch<D, 8, 16, false, 1>(name, a, g, colex, total, dig, s);
+        else chunk_launch<D, 8, 8, false, 4>(name, a, g, colex, total, dig, s);
+    }
+  }
+  static hipError_t chunked(int D, const ScatterArgs<K, PT>& a, const ChunkGeom& g,
+                            const uint32_t* colex, const uint32_t* total, RadixDigit<K> dig,
+                            hipStream_t s) {
+    if (D == 8) chunk_cfgs<8>("chunk_scatter_d8", a, g, colex, total, dig, s);
+    else if (D == 10) chunk_cfgs<10>("chunk_scatter_d10", a, g, colex, total, dig, s);
+    else chunk_cfgs<11>("chunk_scatter_d11", a, g, colex, total, dig, s);
+    return hipGetLastError();
+  }
+};
+
+template <typename K>
+int stats_launch(SortCtx& c, const void* keys) {
+  JFX_LAUNCH("sort_stats", c.s, (k_sort_stats<K><<<c.nchunks, STATS_BLOCK, 0, c.s>>>(
+                                   static_cast<const K*>(keys), c.n, c.chunk, c.oc, c.counts, c.stats_part)));
+  JFX_HIP(hipGetLastError());
+  JFX_LAUNCH("stats_reduce", c.s, (k_stats_reduce<<<1, 256, 0, c.s>>>(c.stats_part, c.nchunks, c.stats)));
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
+
+template <typename K, typename PT>
+int single_pass_typed(SortCtx& c, const void* keys_in, void* keys_out, const void* pay_in,
+                      void* pay_out, int iota, int descending) {
+  const uint32_t R = c.range_radix();
+  const int D = R <= 256 ? 8 : (R <= 1024 ? 10 : 11);
+  JFX_LAUNCH("chunk_colscan", c.s, (k_chunk_colscan<<<(R + 3) / 4, 256, 0, c.s>>>(
+                                        c.counts, 2048u, CM_ROTATE, (uint32_t)(c.mn & 2047u), R,
+                                        c.nchunks, c.colex, c.total)));
+  JFX_HIP(hipGetLastError());
+  ScatterArgs<K, PT> a{};
+  a.keys_in = static_cast<const K*>(keys_in);
+  a.keys_out = static_cast<K*>(keys_out);
+  a.pay_in = static_cast<const PT*>(pay_in);
+  a.pay_out = static_cast<PT*>(pay_out);
+  a.n = c.n;
+  a.err = c.err;
+  a.radix_eff = R;
+  a.write_keys = keys_out != nullptr;
+  a.reverse = descending;
+  a.iota = iota;
+  ChunkGeom g{c.n, c.chunk, c.nchunks};
+  RadixDigit<K> dig{c.oc, 0u, (K)c.mn};
+  JFX_HIP(Passes<K, PT>::chunked(D, a, g, c.colex, c.total, dig, c.s));
+  return JFX_OK;
+}
+
+template <typename K, typename PT>
+int multi_pass_typed(SortCtx& c, const void* keys_in, void* keys_out, int want_keys,
+                     const void* pay_in, void* pay_out, int iota, int descending) {
+  const int D = c.digit_bits;
+  const int radix = 1 << D;
+  const int npass = (c.kb * 8 + D - 1) / D;
+  const uint64_t dmask = (uint64_t)radix - 1;
+  const uint64_t diff = c.band ^ c.bor;
+  uint32_t vary = 0;
+  for (int p = 0; p < npass; ++p)
+    if ((diff >> (p * D)) & dmask) vary |= 1u << p;
+  // histograms of the varying digits and their offsets
+  const uint32_t ub = 512;
+  if (D == 8) {
+    JFX_LAUNCH("sort_hist", c.s, (k_sort_hist<K, 8><<<ub, HIST_BLOCK, 0, c.s>>>(
+                                     static_cast<const K*>(keys_in), c.n, c.oc, vary, c.hist)));
+  } else {
+    JFX_LAUNCH("sort_hist", c.s, (k_sort_hist<K, 11><<<ub, HIST_BLOCK, 0, c.s>>>(
+                                     static_cast<const K*>(keys_in), c.n, c.oc, vary, c.hist)));
+  }
+  JFX_HIP(hipGetLastError());
+  JFX_LAUNCH("sort_plan", c.s, (k_sort_plan<<<1, 1024, 0, c.s>>>(radix, npass, vary, c.hist, c.base)));
   JFX_HIP(hipGetLastError());
 
-  const K* src_k = keys_in;
-  const PT* src_p = pay_in;
+  std::vector<int> passes;
+  for (int p = 0; p < npass; ++p)
+    if ((vary >> p) & 1u) passes.push_back(p);
+  const int A = (int)passes.size();
+  const uint64_t tiles = os_tiles(c.n);
+  const bool first_chunked = passes[0] == 0;
+  if (first_chunked) {
+    JFX_LAUNCH("chunk_colscan", c.s, (k_chunk_colscan<<<(radix + 3) / 4, 256, 0, c.s>>>(
+                                          c.counts, 2048u, D == 8 ? CM_FOLD8 : CM_DIRECT, 0u,
+                                          (uint32_t)radix, c.nchunks, c.colex, c.total)));
+    JFX_HIP(hipGetLastError());
+  }
+  const int first_os = first_chunked ? 1 : 0;
+  if (first_os < A) {
+    JFX_LAUNCH("fill_desc", c.s, (k_fill_u32<<<512, 256, 0, c.s>>>(c.desc[first_os & 1], tiles * radix, 0u)));
+    JFX_HIP(hipGetLastError());
+  }
+  const K* src_k = static_cast<const K*>(keys_in);
+  const PT* src_p = static_cast<const PT*>(pay_in);
+  ChunkGeom g{c.n, c.chunk, c.nchunks};
   for (int a = 0; a < A; ++a) {
     const bool last = (a == A - 1);
-    // the final pass lands in the caller's buffers
     const bool to_out = ((A - 1 - a) % 2) == 0;
-    K* dst_k = to_out ? keys_out : static_cast<K*>(w.alt_keys);
-    PT* dst_p = to_out ? pay_out : static_cast<PT*>(w.alt_pay);
+    K* dst_k = to_out ? static_cast<K*>(keys_out) : static_cast<K*>(c.alt_keys);
+    PT* dst_p = to_out ? static_cast<PT*>(pay_out) : static_cast<PT*>(c.alt_pay);
     ScatterArgs<K, PT> args{};
     args.keys_in = src_k;
     args.keys_out = dst_k;
     args.pay_in = src_p;
     args.pay_out = dst_p;
-    args.n = n;
-    args.base = passes[a].base;
-    args.desc = w.desc[a & 1];
-    args.desc_next = last ? nullptr : w.desc[(a + 1) & 1];
-    args.tile_counter = w.counters + a;
-    args.err = err;
-    args.radix_eff = passes[a].radix_eff;
+    args.n = c.n;
+    args.base = c.base + passes[a] * radix;
+    args.desc = c.desc[a & 1];
+    args.desc_next = (last || a + 1 < first_os) ? nullptr : c.desc[(a + 1) & 1];
+    args.tile_counter = c.counters + a;
+    args.err = c.err;
+    args.radix_eff = (uint32_t)radix;
     args.write_keys = (!last || want_keys) ? 1 : 0;
     args.reverse = (last && descending) ? 1 : 0;
     args.iota = (a == 0 && iota) ? 1 : 0;
-    RadixDigit<K> dig{oc, passes[a].shift, passes[a].sub};
-    hipError_t e = passes[a].d == 8 ? launch_pass<K, PT, 8>(args, dig, s)
-                                    : launch_pass<K, PT, 11>(args, dig, s);
-    JFX_HIP(e);
+    RadixDigit<K> dig{c.oc, (uint32_t)(passes[a] * D), (K)0};
+    if (a == 0 && first_chunked) {
+      args.desc = nullptr;
+      JFX_HIP(Passes<K, PT>::chunked(D, args, g, c.colex, c.total, dig, c.s));
+    } else {
+      JFX_HIP(Passes<K, PT>::onesweep(D, args, dig, c.s));
+    }
     src_k = dst_k;
     src_p = dst_p;
   }
   return JFX_OK;
 }
 
-}  // namespace
-
-int sort_dispatch(int kt, const void* keys_in, void* keys_out, int want_keys, int pay_bytes,
-                  const void* pay_in, void* pay_out, uint64_t n, int descending, int digit_bits,
-                  void* ws, uint64_t ws_bytes, hipStream_t s) {
-  const int kb = key_bytes(kt);
-  const bool iota = (pay_bytes == 4 && pay_in == nullptr);
-  if (kb == 4) {
-    using K = uint32_t;
-    if (pay_bytes == 0)
-      return sort_impl<K, NoPay>(kt, (const K*)keys_in, (K*)keys_out, want_keys, nullptr, nullptr,
-                                 0, n, descending, digit_bits, ws, ws_bytes, s);
-    if (pay_bytes == 4)
-      return sort_impl<K, uint32_t>(kt, (const K*)keys_in, (K*)keys_out, want_keys,
-                                    (const uint32_t*)pay_in, (uint32_t*)pay_out, iota, n,
-                                    descending, digit_bits, ws, ws_bytes, s);
-    return sort_impl<K, uint64_t>(kt, (const K*)keys_in, (K*)keys_out, want_keys,
-                                  (const uint64_t*)pay_in, (uint64_t*)pay_out, 0, n, descending,
-                                  digit_bits, ws, ws_bytes, s);
+template <typename K>
+int execute_k(SortCtx& c, const void* ki, void* ko, int want_keys, const void* pi, void* po,
+              int iota, int desc) {
+  if (c.single_pass()) {
+    void* kout = want_keys ? ko : nullptr;
+    if (c.pb == 0) return single_pass_typed<K, NoPay>(c, ki, kout, nullptr, nullptr, 0, desc);
+    if (c.pb == 4) return single_pass_typed<K, uint32_t>(c, ki, kout, pi, po, iota, desc);
+    return single_pass_typed<K, uint64_t>(c, ki, kout, pi, po, 0, desc);
   }
-  using K = uint64_t;
-  if (pay_bytes == 0)
-    return sort_impl<K, NoPay>(kt, (const K*)keys_in, (K*)keys_out, want_keys, nullptr, nullptr, 0,
-                               n, descending, digit_bits, ws, ws_bytes, s);
-  if (pay_bytes == 4)
-    return sort_impl<K, uint32_t>(kt, (const K*)keys_in, (K*)keys_out, want_keys,
-                                  (const uint32_t*)pay_in, (uint32_t*)pay_out, iota, n, descending,
-                                  digit_bits, ws, ws_bytes, s);
-  return sort_impl<K, uint64_t>(kt, (const K*)keys_in, (K*)keys_out, want_keys,
-                                (const uint64_t*)pay_in, (uint64_t*)pay_out, 0, n, descending,
-                                digit_bits, ws, ws_bytes, s);
+  if (c.pb == 0) return multi_pass_typed<K, NoPay>(c, ki, ko, want_keys, nullptr, nullptr, 0, desc);
+  if (c.pb == 4) return multi_pass_typed<K, uint32_t>(c, ki, ko, want_keys, pi, po, iota, desc);
+  return multi_pass_typed<K, uint64_t>(c, ki, ko, want_keys, pi, po, 0, desc);
 }
 
+}  // namespace
+
 uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n) {
-  return carve_sort_ws(nullptr, 0, key_bytes(kt), pay_bytes, n).bytes;
+  SortCtx c;
+  c.kb = key_bytes(kt);
+  c.pb = pay_bytes;
+  c.n = n;
+  Carver cv(nullptr, 0);
+  carve(c, cv);
+  return cv.used();
 }
 
 int default_digit_bits(int kt, uint64_t n) {
   (void)kt;
   (void)n;
   return 8;
+}
+
+int sort_prepare(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_bytes,
+                 int digit_bits, void* ws, uint64_t ws_bytes, hipStream_t s) {
+  c.kt = kt;
+  c.kb = key_bytes(kt);
+  c.oc = order_code(kt);
+  c.pb = pay_bytes;
+  c.n = n;
+  c.s = s;
+  c.digit_bits = digit_bits ? digit_bits : default_digit_bits(kt, n);
+  JFX_CHECK(c.digit_bits == 8 || c.digit_bits == 11, JFX_ERR_INVALID, "digit_bits must be 8 or 11");
+  Carver cv(ws, ws_bytes);
+  carve(c, cv);
+  JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "sort workspace too small");
+  geometry(c);
+  c.err = device_error_word();
+  JFX_CHECK(c.err != nullptr, JFX_ERR_HIP, "cannot resolve the device error word");
+  JFX_LAUNCH("sort_init", s, (k_sort_init<<<64, 256, 0, s>>>(c.stats, c.counters, 64, c.hist,
+                                                             SORT_MAXP * SORT_MAXR)));
+  JFX_HIP(hipGetLastError());
+  int rc = c.kb == 4 ? stats_launch<uint32_t>(c, keys_in) : stats_launch<uint64_t>(c, keys_in);
+  if (rc != JFX_OK) return rc;
+  unsigned long long* h = static_cast<unsigned long long*>(pinned_scratch(4 * sizeof(unsigned long long)));
+  JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
+  JFX_HIP(hipMemcpyAsync(h, c.stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  JFX_HIP(hipStreamSynchronize(s));
+  c.mn = h[0];
+  c.mx = h[1];
+  c.band = h[2];
+  c.bor = h[3];
+  if (c.kb == 4) c.band &= 0xFFFFFFFFull;
+  return JFX_OK;
+}
+
+int sort_execute(SortCtx& c, const void* keys_in, void* keys_out, int want_keys, const void* pay_in,
+                 void* pay_out, int iota, int descending) {
+  return c.kb == 4 ? execute_k<uint32_t>(c, keys_in, keys_out, want_keys, pay_in, pay_out, iota, descending)
+                   : execute_k<uint64_t>(c, keys_in, keys_out, want_keys, pay_in, pay_out, iota, descending);
+}
+
+int sort_single_pass(SortCtx& c, const void* keys_in, void* keys_out, const void* pay_in,
+                     void* pay_out, int iota, int descending) {
+  JFX_CHECK(c.single_pass(), JFX_ERR_INVALID, "single pass needs max-min < 2048");
+  if (c.kb == 4) {
+    if (c.pb == 0) return single_pass_typed<uint32_t, NoPay>(c, keys_in, keys_out, nullptr, nullptr, 0, descending);
+    if (c.pb == 4) return single_pass_typed<uint32_t, uint32_t>(c, keys_in, keys_out, pay_in, pay_out, iota, descending);
+    return single_pass_typed<uint32_t, uint64_t>(c, keys_in, keys_out, pay_in, pay_out, 0, descending);
+  }
+  if (c.pb == 0) return single_pass_typed<uint64_t, NoPay>(c, keys_in, keys_out, nullptr, nullptr, 0, descending);
+  if (c.pb == 4) return single_pass_typed<uint64_t, uint32_t>(c, keys_in, keys_out, pay_in, pay_out, iota, descending);
+  return single_pass_typed<uint64_t, uint64_t>(c, keys_in, keys_out, pay_in, pay_out, 0, descending);
+}
+
+int sort_dispatch(int kt, const void* keys_in, void* keys_out, int want_keys, int pay_bytes,
+                  const void* pay_in, void* pay_out, uint64_t n, int descending, int digit_bits,
+                  void* ws, uint64_t ws_bytes, hipStream_t s) {
+  SortCtx c;
+  int rc = sort_prepare(c, kt, keys_in, n, pay_bytes, digit_bits, ws, ws_bytes, s);
+  if (rc != JFX_OK) return rc;
+  const int iota = (pay_bytes == 4 && pay_in == nullptr) ? 1 : 0;
+  return sort_execute(c, keys_in, keys_out, want_keys, pay_in, pay_out, iota, descending);
 }
 
 }  // namespace jfx
@@ -362,8 +571,8 @@ int jfx_sort(int key_type, const void* keys_in, void* keys_out, int want_keys, i
   JFX_CHECK(pay_in == nullptr || pay_in != pay_out, JFX_ERR_INVALID,
             "pay_out must not alias pay_in");
   JFX_CHECK(workspace != nullptr, JFX_ERR_INVALID, "null workspace");
-  if (digit_bits == 0) digit_bits = jfx::default_digit_bits(key_type, n);
-  JFX_CHECK(digit_bits == 8 || digit_bits == 11, JFX_ERR_INVALID, "digit_bits must be 0, 8 or 11");
+  JFX_CHECK(digit_bits == 0 || digit_bits == 8 || digit_bits == 11, JFX_ERR_INVALID,
+            "digit_bits must be 0, 8 or 11");
   return jfx::sort_dispatch(key_type, keys_in, keys_out, want_keys, pay_bytes, pay_in, pay_out, n,
                             descending, digit_bits, workspace, ws_bytes, jfx::as_stream(stream));
 }

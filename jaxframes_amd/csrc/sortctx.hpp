@@ -1,0 +1,70 @@
+// Internal sort pipeline shared by the sort, groupby and join entry points.
+#pragma once
+
+#include <stdint.h>
+
+#include "runtime.hpp"
+
+namespace jfx {
+
+constexpr int SORT_MAXP = 8;    // most layout passes (64-bit keys, 8-bit digits)
+constexpr int SORT_MAXR = 2048; // largest radix
+constexpr uint32_t SORT_MAXCHUNKS = 512;
+
+// State of one sort between the statistics pass and the scatter passes.
+struct SortCtx {
+  int kt = 0;            // key type code
+  int kb = 4;            // key bytes
+  int oc = 0;            // order code
+  int pb = 0;            // payload bytes (0, 4, 8)
+  int digit_bits = 8;    // layout digit width for multi-pass sorts
+  uint64_t n = 0;
+  hipStream_t s = nullptr;
+  // chunk geometry of the chunked passes
+  uint64_t chunk = 0;
+  uint32_t nchunks = 0;
+  // statistics of the ordered keys (read back from the device)
+  uint64_t mn = 0, mx = 0, band = 0, bor = 0;
+  // workspace pieces
+  void* alt_keys = nullptr;
+  void* alt_pay = nullptr;
+  uint32_t* counts = nullptr;   // [nchunks][2048] low-11-bit chunk histograms
+  uint32_t* colex = nullptr;    // [nchunks][R] chunk column prefixes
+  uint32_t* total = nullptr;    // [2048] digit totals
+  uint32_t* hist = nullptr;     // [MAXP][MAXR] layout histograms
+  uint32_t* base = nullptr;     // [MAXP][MAXR] layout digit offsets
+  unsigned long long* stats = nullptr;  // min, max, and, or
+  unsigned long long* stats_part = nullptr;  // [nchunks][4] per-chunk statistics
+  uint32_t* counters = nullptr;
+  uint32_t* hdr = nullptr;
+  uint32_t* desc[2] = {nullptr, nullptr};
+  uint32_t* err = nullptr;
+
+  bool single_pass() const { return (mx - mn) < 2048ull; }
+  uint32_t range_radix() const { return (uint32_t)(mx - mn) + 1u; }
+};
+
+uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n);
+int default_digit_bits(int kt, uint64_t n);
+
+// Carve the workspace and run the statistics pass; synchronises the stream
+// once to read min / max / AND / OR of the ordered keys into ctx.
+int sort_prepare(SortCtx& ctx, int kt, const void* keys_in, uint64_t n, int pay_bytes,
+                 int digit_bits, void* ws, uint64_t ws_bytes, hipStream_t s);
+
+// Run all scatter passes (single chunked pass when max-min < 2048, else the
+// layout passes with constant digits skipped).
+int sort_execute(SortCtx& ctx, const void* keys_in, void* keys_out, int want_keys,
+                 const void* pay_in, void* pay_out, int iota, int descending);
+
+// Single-pass only: digit = ordered(key) - min. Runs the column scan and
+// leaves digit totals in ctx.total, then scatters (keys written only when
+// keys_out is non-null).
+int sort_single_pass(SortCtx& ctx, const void* keys_in, void* keys_out, const void* pay_in,
+                     void* pay_out, int iota, int descending);
+
+int sort_dispatch(int kt, const void* keys_in, void* keys_out, int want_keys, int pay_bytes,
+                  const void* pay_in, void* pay_out, uint64_t n, int descending, int digit_bits,
+                  void* ws, uint64_t ws_bytes, hipStream_t s);
+
+}  // namespace jfx
