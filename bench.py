@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the sort-based groupby hot path on one MI355X.
+
+Workload (one step): groupby-sum of 100M rows, int32 keys uniform in
+[0, 1000), float32 values N(0, 1), synthetic and seeded, generated on the
+device before the timed region. A step is one full `groupby_aggregate(keys,
+{'v': vals}, {'v': 'sum'})` as `JaxFrame.groupby('k').agg({'v': 'sum'})` runs
+it: key statistics, then (the key range being below 2048) a counting sort of
+every tile by key inside LDS followed by the head-flag segmented reduction of
+the sorted tile, the in-order merge of the chunk partials, and the read-back
+of the group count.
+
+Prints one JSON line with rows/s, per-step p50/p99, the roofline of the
+dominant kernel (HIP-event durations measured inside the timed region on the
+launch stream), and a CPU baseline (the numpy restatement timed on a bounded
+sample on this host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def _kernel_bytes(name: str, n: int, kb: int, vb: int) -> float | None:
+    """Algorithmic HBM bytes of one launch of a groupby-path kernel."""
+    if name.startswith("tile_reduce"):
+        return n * (kb + vb)                 # read keys + values (sorted in LDS)
+    if name.startswith("chunk_scatter"):
+        return n * (kb + vb) + n * vb        # read keys + values, write values
+    if name == "segreduce_tiles":
+        return n * vb                        # read sorted values (known groups)
+    if name == "sort_stats":
+        return n * kb                        # read keys
+    return None
+
+
+def _traffic(kernel: str):
+    """Measured HBM bytes per launch from the committed rocprof PMC summary,
+    if one exists for this kernel and workload."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "groupby_sum_100M_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(n_sample: int, seed: int):
+    """The numpy restatement of the reference groupby (oracle) on a bounded
+    sample of the same distribution, single-threaded numpy."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, 1000, n_sample, dtype=np.int32)
+    v = rng.standard_normal(n_sample, dtype=np.float32)
+    t0 = time.perf_counter()
+    oracle.groupby_aggregate(k, {"v": v}, {"v": "sum"})
+    dt = time.perf_counter() - t0
+    return n_sample / dt, dt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=100_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=100_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(this build has no multi-GPU path)", file=sys.stderr)
+        return 2
+
+    import torch
+    from jaxframes_amd import _lib, grouping
+
+    _lib.require_gpu()
+    dev = torch.device("cuda", 0)
+    n = args.rows
+    g = torch.Generator(device=dev)
+    g.manual_seed(1234)
+    keys = torch.randint(0, 1000, (n,), dtype=torch.int32, device=dev, generator=g)
+    vals = torch.randn(n, dtype=torch.float32, device=dev, generator=g)
+
+    def step():
+        return grouping.groupby_aggregate(keys, {"v": vals}, {"v": "sum"})
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    with _lib.KernelTimer() as kt:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            uk, res = step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    kstats = kt.stats()
+    elapsed = t1 - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = n * args.steps / elapsed
+
+    # per-step latency distribution (each step synchronised on its own)
+    lat = []
+    for _ in range(max(5, min(args.steps, 20))):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    assert _lib.device_error() == 0
+    ngroups = int(uk.shape[0])
+
+    # roofline of the dominant kernel
+    dom = max(kstats.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dcount) = dom
+    avg_s = dms / 1e3 / dcount
+    dbytes = _kernel_bytes(dname, n, 4, 4)
+    roof = None
+    if dbytes is not None:
+        ach = dbytes / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dname, "achieved": round(ach, 1),
+                "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBPS, 4),
+                "traffic": _traffic(dname), "avg_launch_us": round(avg_s * 1e6, 1),
+                "alg_bytes_per_launch": dbytes}
+    kernels = {k: {"us_per_step": round(v[0] / args.steps * 1e3, 1),
+                   "launches_per_step": v[1] // args.steps} for k, v in kstats.items()}
+    step_bytes = sum((_kernel_bytes(k, n, 4, 4) or 0) * (v[1] // args.steps)
+                     for k, v in kstats.items())
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sample = min(args.cpu_sample, n)
+        rps, dt = cpu_baseline(sample, seed=1234)
+        cpu = {"value": round(rps, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"{sample} rows, same distribution, numpy restatement of "
+                         f"SortBasedGroupBy.groupby_aggregate ({dt:.1f} s)",
+               "host_cpu": _cpu_model(), "os_cpu_count": os.cpu_count()}
+
+    line = {
+        "metric": "groupby-sum rows/s + % HBM peak at 100M rows; batched vs plain; p50/p99 ms",
+        "value": round(value, 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "p50_ms": round(p50, 4),
+        "p99_ms": round(p99, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device-generated, seeded)",
+        "config": {"workload": "groupby-sum, 100M rows, int32 keys uniform [0,1000), "
+                               "float32 N(0,1) values",
+                   "rows": n, "groups": ngroups, "agg": "sum", "path": "sort-based"},
+        "step_alg_bytes": step_bytes,
+        "step_hbm_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / PEAK_HBM_GBPS, 4),
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

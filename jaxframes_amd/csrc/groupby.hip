@@ -20,6 +20,7 @@
 #include "runtime.hpp"
 #include "segreduce.hpp"
 #include "sortctx.hpp"
+#include "tilereduce.hpp"
 
 namespace jfx {
 
@@ -357,7 +358,7 @@ __global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT, MM> a)
 template <typename K>
 __global__ void __launch_bounds__(1024)
 k_seg_segments(const uint32_t* total, uint32_t R, uint64_t mn, int oc, uint64_t n,
-               uint32_t* seg_start, K* out_keys, uint32_t* ngroups) {
+               uint32_t* seg_start, K* out_keys, uint32_t* ngroups, uint32_t* dgid) {
   __shared__ uint32_t s_base[2048];
   __shared__ uint32_t s_gid[2048];
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
@@ -372,6 +373,7 @@ k_seg_segments(const uint32_t* total, uint32_t R, uint64_t mn, int oc, uint64_t 
   const uint32_t G = block_excl_scan_lds<2048, 1024>(s_gid, s_tmp);
   for (int d = tid; d < (int)R; d += 1024) {
     const uint32_t t = (d + 1 < 2048 ? s_gid[d + 1] : G) - s_gid[d];
+    if (dgid) dgid[d] = t ? s_gid[d] : 0xFFFFFFFFu;
     if (t) {
       seg_start[s_gid[d]] = s_base[d];
       if (out_keys) {
@@ -397,6 +399,8 @@ struct SegWs {
   uint32_t* counter;
   uint32_t* ngroups;
   uint32_t* seg_start;
+  uint32_t* dgid;   // [2048] group index of each digit (small-range path)
+  void* part;       // [chunks][2048] chunk accumulators (small-range path)
 };
 
 uint64_t seg_tiles(uint64_t n) { return (n + SEG_TILE - 1) / SEG_TILE; }
@@ -412,6 +416,8 @@ SegWs carve_seg(Carver& c, int vb, uint64_t n) {
   w.counter = c.take<uint32_t>(4);
   w.ngroups = c.take<uint32_t>(4);
   w.seg_start = c.take<uint32_t>(2049);
+  w.dgid = c.take<uint32_t>(2048);
+  w.part = c.take<char>((uint64_t)SORT_MAXCHUNKS * 2048 * st);
   return w;
 }
 
@@ -515,9 +521,59 @@ template <typename K>
 int segments_launch(const SortCtx& c, const SegWs& w, void* out_keys, hipStream_t s) {
   JFX_LAUNCH("seg_segments", s, (k_seg_segments<K><<<1, 1024, 0, s>>>(
                                      c.total, c.range_radix(), c.mn, c.oc, c.n, w.seg_start,
-                                     static_cast<K*>(out_keys), w.ngroups)));
+                                     static_cast<K*>(out_keys), w.ngroups, w.dgid)));
   JFX_HIP(hipGetLastError());
   return JFX_OK;
+}
+
+
+template <typename K, typename V, typename MT, bool MM>
+int tile_reduce_mm(const SortCtx& c, const SegWs& w, const void* keys, const void* vals,
+                   const SegCall& sc, hipStream_t s) {
+  TileRedArgs<K, V, MM> a{};
+  a.keys = static_cast<const K*>(keys);
+  a.oc = c.oc;
+  a.sub = (K)c.mn;
+  a.R = c.range_radix();
+  a.vals = static_cast<const V*>(vals);
+  a.n = c.n;
+  a.chunk = c.chunk;
+  a.part = static_cast<AggState<V, MM>*>(w.part);
+  if (a.R <= 256) {
+    JFX_LAUNCH("tile_reduce_d8", s, (k_tile_reduce<K, V, MM, 8, 8, 8><<<c.nchunks, 512, 0, s>>>(a)));
+  } else if (a.R <= 1024) {
+    JFX_LAUNCH("tile_reduce_d10", s, (k_tile_reduce<K, V, MM, 10, 8, 8><<<c.nchunks, 512, 0, s>>>(a)));
+  } else {
+    JFX_LAUNCH("tile_reduce_d11", s, (k_tile_reduce<K, V, MM, 11, 4, 8><<<c.nchunks, 256, 0, s>>>(a)));
+  }
+  JFX_HIP(hipGetLastError());
+  GroupOut<V, MT> o{static_cast<V*>(sc.out_sum), sc.out_count, static_cast<V*>(sc.out_min),
+                    static_cast<V*>(sc.out_max), static_cast<MT*>(sc.out_mean)};
+  JFX_LAUNCH("tile_combine", s, (k_tile_combine<V, MT, MM><<<(a.R + 3) / 4, 256, 0, s>>>(
+                                     a.part, c.nchunks, a.R, w.dgid, o)));
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
+
+template <typename K, typename V, typename MT>
+int tile_reduce_typed(const SortCtx& c, const SegWs& w, const void* keys, const void* vals,
+                      const SegCall& sc, hipStream_t s) {
+  if (sc.out_min || sc.out_max) return tile_reduce_mm<K, V, MT, true>(c, w, keys, vals, sc, s);
+  return tile_reduce_mm<K, V, MT, false>(c, w, keys, vals, sc, s);
+}
+
+template <typename K>
+int tile_reduce_key(const SortCtx& c, const SegWs& w, int vt, const void* keys, const void* vals,
+                    const SegCall& sc, hipStream_t s) {
+  switch (vt) {
+    case JFX_F32: return tile_reduce_typed<K, float, float>(c, w, keys, vals, sc, s);
+    case JFX_F64: return tile_reduce_typed<K, double, double>(c, w, keys, vals, sc, s);
+    case JFX_I32: return tile_reduce_typed<K, int32_t, float>(c, w, keys, vals, sc, s);
+    case JFX_I64: return tile_reduce_typed<K, int64_t, double>(c, w, keys, vals, sc, s);
+    case JFX_U32: return tile_reduce_typed<K, uint32_t, float>(c, w, keys, vals, sc, s);
+    case JFX_U64: return tile_reduce_typed<K, uint64_t, double>(c, w, keys, vals, sc, s);
+  }
+  return set_error(JFX_ERR_INVALID, "bad value type");
 }
 
 }  // namespace
@@ -604,24 +660,18 @@ int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, 
   if (rc != JFX_OK) return rc;
   const int oc = jfx::order_code(key_type);
   if (jfx::int_key(key_type) && ctx.single_pass()) {
-    // one counting-sort pass carries only the values; groups are the
-    // non-empty digits, so keys are neither written nor re-read
-    rc = jfx::sort_single_pass(ctx, keys, nullptr, vals, svals, 0, 0);
+    // Small key range: every distinct key is one digit of a single counting
+    // sort; each tile is sorted and reduced in LDS (tile_reduce), so neither
+    // sorted keys nor sorted values are written to HBM.
+    rc = jfx::sort_digit_totals(ctx);
     if (rc != JFX_OK) return rc;
     rc = kb == 4 ? jfx::segments_launch<uint32_t>(ctx, w, out_keys, s)
                  : jfx::segments_launch<uint64_t>(ctx, w, out_keys, s);
     if (rc != JFX_OK) return rc;
-    uint32_t nseg = 0;
-    {
-      // G is needed on the host to bound the known-segment table
-      uint64_t g = 0;
-      rc = jfx::read_ngroups(w, &g, s);
-      if (rc != JFX_OK) return rc;
-      nseg = (uint32_t)g;
-    }
-    jfx::SegCall c{nullptr, oc, svals, nullptr, n, nullptr, out_sum, out_count, out_min, out_max,
-                   out_mean, true, nseg};
-    rc = jfx::segreduce_dispatch(w, key_type, val_type, c, s);
+    jfx::SegCall c{nullptr, oc, vals, nullptr, n, nullptr, out_sum, out_count, out_min, out_max,
+                   out_mean, true, 0u};
+    rc = kb == 4 ? jfx::tile_reduce_key<uint32_t>(ctx, w, val_type, keys, vals, c, s)
+                 : jfx::tile_reduce_key<uint64_t>(ctx, w, val_type, keys, vals, c, s);
     if (rc != JFX_OK) return rc;
     return jfx::read_ngroups(w, out_ngroups, s);
   }

@@ -533,6 +533,16 @@ int sort_single_pass(SortCtx& c, const void* keys_in, void* keys_out, const void
   return single_pass_typed<uint64_t, uint64_t>(c, keys_in, keys_out, pay_in, pay_out, 0, descending);
 }
 
+int sort_digit_totals(SortCtx& c) {
+  JFX_CHECK(c.single_pass(), JFX_ERR_INVALID, "digit totals need max-min < 2048");
+  const uint32_t R = c.range_radix();
+  JFX_LAUNCH("chunk_colscan", c.s, (k_chunk_colscan<<<(R + 3) / 4, 256, 0, c.s>>>(
+                                        c.counts, 2048u, CM_ROTATE, (uint32_t)(c.mn & 2047u), R,
+                                        c.nchunks, c.colex, c.total)));
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
+
 int sort_dispatch(int kt, const void* keys_in, void* keys_out, int want_keys, int pay_bytes,
                   const void* pay_in, void* pay_out, uint64_t n, int descending, int digit_bits,
                   void* ws, uint64_t ws_bytes, hipStream_t s) {

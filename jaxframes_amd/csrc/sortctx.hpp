@@ -63,6 +63,10 @@ int sort_execute(SortCtx& ctx, const void* keys_in, void* keys_out, int want_key
 int sort_single_pass(SortCtx& ctx, const void* keys_in, void* keys_out, const void* pay_in,
                      void* pay_out, int iota, int descending);
 
+// Single-pass only: run the column scan so ctx.total holds the count of
+// every digit (ordered(key) - min), without scattering.
+int sort_digit_totals(SortCtx& ctx);
+
 int sort_dispatch(int kt, const void* keys_in, void* keys_out, int want_keys, int pay_bytes,
                   const void* pay_in, void* pay_out, uint64_t n, int descending, int digit_bits,
                   void* ws, uint64_t ws_bytes, hipStream_t s);
