@@ -123,6 +123,31 @@ int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, 
                 void* out_mean, uint64_t* out_ngroups, int digit_bits, void* workspace,
                 uint64_t ws_bytes, void* stream);
 
+/* ---- hashing and the local hash partition ----------------------------- */
+/* Row hash of 1..8 key columns (device pointers, one per column, n rows
+ * each). Replaces hash_multi_columns (parallel_algorithms.py:69-136): one key
+ * -> fmix32(uint32(key)); K keys -> seed 0x1337, per key i
+ * h ^= uint32(k_i)*(0x9E3779B1+i), h = rotl(h,13), h *= 0x85EBCA6B, then
+ * fmix32. uint32(key) is the low 32 bits of an integer, the bits of a
+ * float32, and the bits of a float64 rounded to float32. */
+int jfx_hash_rows(const void* const* keys, const int* key_types, int nkeys, uint64_t n,
+                  uint32_t* out_hashes, void* stream);
+
+int jfx_hash_partition_workspace_size(int key_type, uint64_t n, uint32_t nbuckets,
+                                      uint64_t* out_bytes);
+
+/* Local hash partition into nbuckets (1..2048): bucket = hash % nbuckets with
+ * hash = fmix32(uint32(key)), or hash = key when prehashed (key_type must be
+ * JFX_U32, e.g. jfx_hash_rows output). Rows keep their input order inside a
+ * bucket. Outputs (any of out_keys / out_rows / out_hashes may be NULL):
+ * partitioned keys, the uint32 source row of each output row, out_offsets
+ * (int64, nbuckets+1) and each input row's hash. Replaces the per-device step
+ * of collective_hash_repartition (hash at :214, _bucketize_local_column at
+ * :163-182) with a compact layout instead of the padded [P, local_n] one. */
+int jfx_hash_partition(int key_type, const void* keys, uint64_t n, uint32_t nbuckets,
+                       int prehashed, void* out_keys, uint32_t* out_rows, int64_t* out_offsets,
+                       uint32_t* out_hashes, void* workspace, uint64_t ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

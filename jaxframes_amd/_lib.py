@@ -49,6 +49,10 @@ _SIGNATURES = {
     "jfx_prof_reset": [],
     "jfx_prof_count": [ctypes.POINTER(c_int)],
     "jfx_prof_get": [c_int, ctypes.c_char_p, c_u64, ctypes.POINTER(ctypes.c_double), c_u64p],
+    "jfx_hash_rows": [ctypes.POINTER(c_vp), ctypes.POINTER(c_int), c_int, c_u64, c_vp, c_vp],
+    "jfx_hash_partition_workspace_size": [c_int, c_u64, ctypes.c_uint32, c_u64p],
+    "jfx_hash_partition": [c_int, c_vp, c_u64, ctypes.c_uint32, c_int, c_vp, c_vp, c_vp, c_vp,
+                           c_vp, c_u64, c_vp],
     "jfx_groupby_workspace_size": [c_int, c_int, c_u64, c_u64p],
     "jfx_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64p,
                     c_int, c_vp, c_u64, c_vp],

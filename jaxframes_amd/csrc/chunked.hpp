@@ -47,7 +47,7 @@ __device__ __forceinline__ uint32_t chunk_count(const uint32_t* counts, uint32_t
   return c;
 }
 
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 k_chunk_colscan(const uint32_t* counts, uint32_t cstride, int mode, uint32_t sub, uint32_t R,
                 uint32_t nchunks, uint32_t* colex, uint32_t* total) {
   const uint32_t d = blockIdx.x * 4u + (threadIdx.x >> 6);

@@ -1,0 +1,248 @@
+"""Pins the CPU oracle (numpy restatement) before anything is checked
+against it: golden vectors transcribed from the reference's tests, closed
+forms, brute force, pandas and published MurmurHash3 vectors. A plausible
+mistake in the oracle (dropped tie-break, wrong sign flip, transposed join
+order, wrong hash constant) fails at least one check here."""
+
+import json
+import os
+
+import numpy as np
+import pandas as pd
+import pytest
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
+
+
+def _golden():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+# ---- sort -----------------------------------------------------------------
+
+def test_sort_golden_vectors():
+    g = _golden()["sort"]
+    for case in g:
+        keys = np.asarray(case["keys"], dtype=case.get("dtype", "int64"))
+        vals = np.asarray(case["values"]) if "values" in case else None
+        sk, sv, _ = oracle.radix_sort(keys, vals, case.get("ascending", True))
+        if np.asarray(case["expected_keys"]).dtype.kind == "f":
+            np.testing.assert_allclose(sk, case["expected_keys"], rtol=1e-6)
+        else:
+            assert sk.tolist() == case["expected_keys"], case["source"]
+        if "expected_values" in case:
+            assert sv.tolist() == case["expected_values"], case["source"]
+
+
+def test_sort_seeded_reference_case():
+    # test_parallel_algorithms.py:29-41: np.random.seed(42); randint(0, 100, 1000)
+    data = np.random.RandomState(42).randint(0, 100, size=1000)
+    sk, _, _ = oracle.radix_sort(data)
+    assert np.array_equal(sk, np.sort(data))
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.uint32, np.uint64, np.float32, np.float64])
+def test_sort_closed_form_stable_argsort(dt):
+    rng = np.random.default_rng(1)
+    n = 20000
+    if np.dtype(dt).kind == "f":
+        k = rng.choice(np.array([np.nan, -0.0, 0.0, np.inf, -np.inf, 1.5, -1.5, 2.5], dtype=dt), n)
+        k[::7] = rng.standard_normal(len(k[::7])).astype(dt)
+    else:
+        info = np.iinfo(dt)
+        k = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+        k[::3] = k[0]  # ties
+    _, _, perm = oracle.radix_sort(k)
+    assert np.array_equal(perm, np.argsort(k, kind="stable"))
+    _, _, perm_d = oracle.radix_sort(k, ascending=False)
+    assert np.array_equal(perm_d, np.argsort(k, kind="stable")[::-1])
+
+
+def test_ordered_bits_preserves_order():
+    rng = np.random.default_rng(2)
+    for dt in (np.int32, np.int64, np.float32, np.float64):
+        k = (rng.standard_normal(5000) * 1e6).astype(dt)
+        ob = oracle.ordered_bits(k)
+        a, b = k[:-1], k[1:]
+        oa, obb = ob[:-1], ob[1:]
+        assert np.array_equal(a < b, oa < obb)
+        assert np.array_equal(a == b, oa == obb)
+    z = oracle.ordered_bits(np.array([-0.0, 0.0], dtype=np.float32))
+    assert z[0] == z[1]
+
+
+def test_lexsort_golden_and_closed_form():
+    for case in _golden()["lexsort"]:
+        keys = [np.asarray(k) for k in case["keys"]]
+        idx = oracle.multi_column_lexsort(keys, case["ascending"])
+        for col, exp in zip(case["columns"], case["expected"]):
+            assert keys[col][idx].tolist() == exp, case["source"]
+    rng = np.random.default_rng(3)
+    a, b = rng.integers(0, 5, 1000), rng.integers(0, 5, 1000)
+    assert np.array_equal(oracle.multi_column_lexsort([a, b]), np.lexsort([b, a]))
+
+
+# ---- groupby ----------------------------------------------------------------
+
+def test_groupby_golden_vectors():
+    for case in _golden()["groupby"]:
+        keys = np.asarray(case["keys"])
+        vals = {c: np.asarray(v) for c, v in case["values"].items()}
+        uk, res = oracle.groupby_aggregate(keys, vals, case["aggs"])
+        assert uk.tolist() == case["expected_keys"], case["source"]
+        for c, exp in case["expected"].items():
+            np.testing.assert_allclose(np.asarray(res[c], dtype=np.float64), exp, rtol=1e-6,
+                                       err_msg=case["source"])
+
+
+def test_groupby_vs_pandas():
+    rng = np.random.default_rng(4)
+    n = 100000
+    k = rng.integers(0, 1000, n).astype(np.int32)
+    v = rng.standard_normal(n).astype(np.float32)
+    w = rng.integers(-50, 50, n).astype(np.int64)
+    uk, res = oracle.groupby_aggregate(k, {"v": v, "w": w, "c": v}, {"v": "sum", "w": "max", "c": "count"})
+    df = pd.DataFrame({"k": k, "v": v.astype(np.float64), "w": w}).groupby("k").agg(
+        v=("v", "sum"), w=("w", "max"), c=("v", "count"))
+    assert np.array_equal(uk, df.index.values)
+    np.testing.assert_allclose(res["v"], df["v"].values, rtol=1e-9, atol=1e-9)
+    assert np.array_equal(res["w"], df["w"].values)
+    assert np.array_equal(res["c"], df["c"].values)
+
+
+def test_groupby_min_max_mean_vs_pandas():
+    rng = np.random.default_rng(5)
+    k = rng.integers(-20, 20, 5000)
+    v = rng.standard_normal(5000)
+    df = pd.DataFrame({"k": k, "v": v}).groupby("k")["v"]
+    for fn in ("min", "max", "mean"):
+        uk, res = oracle.groupby_aggregate(k, {"v": v}, {"v": fn})
+        np.testing.assert_allclose(res["v"], getattr(df, fn)().values, rtol=1e-12)
+
+
+def test_groupby_multi_column_vs_pandas():
+    rng = np.random.default_rng(6)
+    a, b = rng.integers(0, 5, 3000), rng.integers(0, 7, 3000)
+    v = rng.integers(0, 100, 3000)
+    ukd, res = oracle.groupby_aggregate_multi_column([a, b], ["a", "b"], {"v": v}, {"v": "sum"})
+    df = pd.DataFrame({"a": a, "b": b, "v": v}).groupby(["a", "b"])["v"].sum().reset_index()
+    assert ukd["a"].tolist() == df["a"].tolist() and ukd["b"].tolist() == df["b"].tolist()
+    assert res["v"].tolist() == df["v"].tolist()
+
+
+def test_groupby_empty_and_unknown_agg():
+    uk, res = oracle.groupby_aggregate(np.zeros(0, np.int32), {"v": np.zeros(0)}, {"v": "sum"})
+    assert uk.size == 0 and res["v"].size == 0
+    with pytest.raises(ValueError):
+        oracle.groupby_aggregate(np.array([1]), {"v": np.array([1.0])}, {"v": "median"})
+
+
+# ---- join -------------------------------------------------------------------
+
+def test_join_golden_vectors():
+    for case in _golden()["join"]:
+        lk, rk = np.asarray(case["left_keys"]), np.asarray(case["right_keys"])
+        lv = {c: np.asarray(v) for c, v in case["left_values"].items()}
+        rv = {c: np.asarray(v) for c, v in case["right_values"].items()}
+        keys, out, _, _ = oracle.sort_merge_join(lk, lv, rk, rv, case["how"])
+        assert keys.tolist() == case["expected_keys"], case["source"]
+        for c, exp in case["expected"].items():
+            got = np.asarray(out[c], dtype=np.float64)
+            np.testing.assert_array_equal(np.isnan(got), np.isnan(np.asarray(exp, dtype=np.float64)))
+            np.testing.assert_allclose(got[~np.isnan(got)], np.asarray(exp, dtype=np.float64)[~np.isnan(got)])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_inner_join_vs_bruteforce(seed):
+    rng = np.random.default_rng(seed)
+    lk = rng.integers(0, 12, 60).astype(np.int64)
+    rk = rng.integers(0, 12, 40).astype(np.int64)
+    a = oracle.inner_join_pairs(lk, rk)
+    b = oracle.inner_join_pairs_bruteforce(lk, rk)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_inner_join_count_vs_pandas():
+    rng = np.random.default_rng(7)
+    lk = rng.integers(0, 500, 20000)
+    rk = rng.integers(0, 500, 3000)
+    keys, li, ri = oracle.inner_join_pairs(lk, rk)
+    m = pd.merge(pd.DataFrame({"k": lk, "l": np.arange(lk.size)}),
+                 pd.DataFrame({"k": rk, "r": np.arange(rk.size)}), on="k")
+    assert keys.size == len(m)
+    got = sorted(zip(li.tolist(), ri.tolist()))
+    assert got == sorted(zip(m["l"].tolist(), m["r"].tolist()))
+
+
+# ---- hashing / partition ----------------------------------------------------
+
+def test_fmix32_published_murmur3_vectors():
+    # MurmurHash3_x86_32 known answers (published test vectors). The empty
+    # input with seed s reduces to fmix32(s), pinning fmix32 directly.
+    assert oracle.murmur3_x86_32(b"", 0) == 0
+    assert oracle.murmur3_x86_32(b"", 1) == 0x514E28B7
+    assert oracle.murmur3_x86_32(b"", 0xFFFFFFFF) == 0x81F16F39
+    assert oracle.murmur3_x86_32(b"\x00\x00\x00\x00", 0) == 0x2362F9DE
+    assert oracle.murmur3_x86_32(b"aaaa", 0x9747B28C) == 0x5A97808A
+    assert oracle.murmur3_x86_32(b"Hello, world!", 1234) == 0xFAF6CDB3
+    assert oracle.murmur3_x86_32(b"The quick brown fox jumps over the lazy dog", 0x9747B28C) == 0x2FA826CD
+    assert int(oracle.fmix32(np.array([1], np.uint32))[0]) == 0x514E28B7
+
+
+def test_hash_reference_properties():
+    # test_distributed_multi_column.py:118-179
+    h = oracle.hash_multi_columns([np.array([1, 1, 2, 2]), np.array([10, 20, 10, 20])])
+    assert len(np.unique(h)) == 4
+    h = oracle.hash_multi_columns([np.array([1, 2, 1, 2]), np.array([10, 20, 10, 20])])
+    assert h[0] == h[2] and h[1] == h[3] and h[0] != h[1]
+    i = np.arange(1000)
+    h = oracle.hash_multi_columns([i % 100, i % 50])
+    c = np.bincount(h % 8, minlength=8)
+    assert (c > 1000 / 16).all() and (c < 1000 / 4).all()
+
+
+def test_hash_single_key_low_bits():
+    k = np.array([5, 5 + (1 << 32), -1, (1 << 32) - 1], dtype=np.int64)
+    h = oracle.hash_multi_columns([k])
+    assert h[0] == h[1] and h[2] == h[3]
+
+
+@pytest.mark.parametrize("nb", [1, 3, 8, 256])
+def test_hash_partition_vs_padded_bucketize(nb):
+    rng = np.random.default_rng(nb)
+    keys = rng.integers(-10**9, 10**9, 3000).astype(np.int64)
+    perm, offs = oracle.hash_partition(keys, nb)
+    targets = (oracle.hash_multi_columns([keys]) % np.uint32(nb)).astype(np.int64)
+    padded = oracle.bucketize_padded(keys, targets, nb, -999999)
+    valid = oracle.bucketize_padded(np.ones(keys.size, bool), targets, nb, False)
+    compact = padded[valid]   # row-major: bucket order, then position
+    assert np.array_equal(compact, keys[perm])
+    assert np.array_equal(np.diff(offs), valid.sum(axis=1))
+    # brute-force: each bucket holds its rows in input order
+    for b in range(nb):
+        rows = perm[offs[b]:offs[b + 1]]
+        assert np.array_equal(rows, np.flatnonzero(targets == b))
+
+
+def test_bucketize_padded_bruteforce():
+    arr = np.array([10, 11, 12, 13, 14])
+    t = np.array([1, 0, 1, 1, 0])
+    out = oracle.bucketize_padded(arr, t, 2, -1)
+    assert out.tolist() == [[11, 14, -1, -1, -1], [10, 12, 13, -1, -1]]
+
+
+def test_compact_and_filter():
+    m = np.array([True, False, True, True, False])
+    assert oracle.compact_rows(m).tolist() == [0, 2, 3]
+    out = oracle.filter_mask_compact({"a": np.arange(5)}, m)
+    assert out["a"].tolist() == [0, 2, 3]
+
+
+def test_pinned_by_covers_public_functions():
+    for name in ("radix_sort", "groupby_aggregate", "inner_join_pairs", "fmix32",
+                 "hash_partition", "sort_merge_join", "multi_column_lexsort"):
+        assert name in oracle.PINNED_BY
