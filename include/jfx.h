@@ -148,6 +148,54 @@ int jfx_hash_partition(int key_type, const void* keys, uint64_t n, uint32_t nbuc
                        int prehashed, void* out_keys, uint32_t* out_rows, int64_t* out_offsets,
                        uint32_t* out_hashes, void* workspace, uint64_t ws_bytes, void* stream);
 
+/* ---- sort-merge join -------------------------------------------------- */
+/* Both sides are first sorted with jfx_sort carrying the uint32 row index
+ * (ParallelSortMergeJoin.join step 1, parallel_algorithms.py:969-973). The
+ * merge then emits every (left, right) pair with equal keys, in (key, left
+ * sorted position, right sorted position) order — the order of
+ * jnp.where(sorted_left[:, None] == sorted_right[None, :])
+ * (parallel_algorithms.py:987-993) — without the L x R mask. NaN keys match
+ * nothing (NaN != NaN); -0.0 matches +0.0. Each side must have < 2^32 rows. */
+int jfx_join_workspace_size(uint64_t n_left, uint64_t* out_bytes);
+
+/* Counts the pairs (per 2048-row left tile, scanned) and writes the total to
+ * *out_pairs; synchronises the stream once. */
+int jfx_join_count(int key_type, const void* sorted_left, uint64_t n_left, const void* sorted_right,
+                   uint64_t n_right, uint64_t* out_pairs, void* workspace, uint64_t ws_bytes,
+                   void* stream);
+
+/* Writes the pairs counted by the preceding jfx_join_count on the same
+ * workspace: out_left_rows / out_right_rows get left_rows[i] / right_rows[j]
+ * of each pair (source row numbers), out_keys (optional) the key. Output
+ * buffers hold *out_pairs entries. */
+int jfx_join_fill(int key_type, const void* sorted_left, const uint32_t* left_rows, uint64_t n_left,
+                  const void* sorted_right, const uint32_t* right_rows, uint64_t n_right,
+                  uint32_t* out_left_rows, uint32_t* out_right_rows, void* out_keys,
+                  void* workspace, uint64_t ws_bytes, void* stream);
+
+/* For each sorted probe key, build_rows[first index of an equal key in
+ * sorted_build], or 0xFFFFFFFF when none: the searchsorted + equality test
+ * of the left / right / outer branches (parallel_algorithms.py:1014-1047). */
+int jfx_join_first_match(int key_type, const void* sorted_probe, uint64_t n_probe,
+                         const void* sorted_build, const uint32_t* build_rows, uint64_t n_build,
+                         uint32_t* out_rows, void* stream);
+
+/* dst[i] = src[rows[i]], or fill_bits when rows[i] == 0xFFFFFFFF, for 4- or
+ * 8-byte elements: the payload gather of parallel_algorithms.py:1051-1073
+ * (jnp.where(idx >= 0, col[idx], nan)). */
+int jfx_gather(int elem_bytes, const void* src, const uint32_t* rows, uint64_t n, void* dst,
+               uint64_t fill_bits, void* stream);
+
+/* ---- per-kernel timing ------------------------------------------------ */
+/* No counterpart in the reference. When enabled, every kernel launch is
+ * bracketed by HIP events on its own stream; totals per kernel name are
+ * read back with jfx_prof_count / jfx_prof_get (which synchronise on the
+ * pending events). */
+int jfx_prof_enable(int on);
+int jfx_prof_reset(void);
+int jfx_prof_count(int* out_n);
+int jfx_prof_get(int i, char* name_buf, uint64_t buf_len, double* total_ms, uint64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

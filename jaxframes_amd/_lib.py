@@ -53,6 +53,12 @@ _SIGNATURES = {
     "jfx_hash_partition_workspace_size": [c_int, c_u64, ctypes.c_uint32, c_u64p],
     "jfx_hash_partition": [c_int, c_vp, c_u64, ctypes.c_uint32, c_int, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_u64, c_vp],
+    "jfx_join_workspace_size": [c_u64, c_u64p],
+    "jfx_join_count": [c_int, c_vp, c_u64, c_vp, c_u64, c_u64p, c_vp, c_u64, c_vp],
+    "jfx_join_fill": [c_int, c_vp, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_u64,
+                      c_vp],
+    "jfx_join_first_match": [c_int, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp],
+    "jfx_gather": [c_int, c_vp, c_vp, c_u64, c_vp, c_u64, c_vp],
     "jfx_groupby_workspace_size": [c_int, c_int, c_u64, c_u64p],
     "jfx_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64p,
                     c_int, c_vp, c_u64, c_vp],

@@ -22,13 +22,15 @@ def sort_workspace_bytes(kcode: int, pay_bytes: int, n: int) -> int:
 
 
 def radix_sort(keys, values=None, ascending: bool = True, *, want_keys: bool = True,
-               return_index: bool = False, digit_bits: int = 0):
+               return_index: bool = False, digit_bits: int = 0, index_dtype=torch.int64):
     """Stable sort of a key column with an optional payload column.
 
     values: None, or a column whose elements are 4 or 8 bytes (carried
     through the passes as payload), or any other column (permuted by a
     gather of the row index the sort carries).
     return_index: also return the int64 source-row index of each output row.
+    index_dtype: torch.int64 (default) or torch.int32 (the kernel's uint32
+    row index as stored, no conversion).
     Returns (sorted_keys or None, sorted_values or None[, index]).
     """
     _lib.require_gpu()
@@ -42,7 +44,7 @@ def radix_sort(keys, values=None, ascending: bool = True, *, want_keys: bool = T
         raise ValueError("keys and values must have the same length")
     need_index = return_index or (v is not None and v.element_size() not in (4, 8))
     if n == 0:
-        idx = torch.empty(0, dtype=torch.int64, device=dev)
+        idx = torch.empty(0, dtype=index_dtype, device=dev)
         res_k = k.to(orig_dtype) if want_keys else None
         res_v = None if v is None else v[:0]
         return (res_k, res_v, idx) if return_index else (res_k, res_v)
@@ -65,8 +67,8 @@ def radix_sort(keys, values=None, ascending: bool = True, *, want_keys: bool = T
     res_k = (out_k.to(orig_dtype) if out_k.dtype != orig_dtype else out_k) if want_keys else None
     idx = None
     if need_index:
-        idx = pay_out.view(torch.int32).to(torch.int64) if n >= (1 << 31) else pay_out.long()
-        res_v = None if v is None else v[idx]
+        idx = pay_out if index_dtype == torch.int32 else pay_out.long()
+        res_v = None if v is None else v[pay_out.long()]
     elif v is not None:
         res_v = pay_out.view(v.dtype)
     else:
