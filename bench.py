@@ -89,6 +89,9 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--cpu-sample", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device copies of 400 MB (float32[1e8]) so that PMC byte "
+                         "counters can be scaled against a transfer of known size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -100,6 +103,15 @@ def main() -> int:
 
     _lib.require_gpu()
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        a = torch.ones(100_000_000, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": a.numel() * 4,
+                                               "bytes_written": b.numel() * 4, "launches": 3}}))
+        return 0
     n = args.rows
     g = torch.Generator(device=dev)
     g.manual_seed(1234)
