@@ -62,7 +62,8 @@ __device__ __forceinline__ bool key_equal(uint64_t a, uint64_t b, int oc) {
 }
 
 // ---- wave64 helpers -------------------------------------------------------
-__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+// The i1 builtin: the predicate goes straight into a scalar lane mask.
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 // popcount(mask & lanes below me)
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {

@@ -14,6 +14,8 @@
 //                      trailing partial.
 //   k_segreduce_fix    one workgroup scans the per-tile partials and writes
 //                      the groups that cross tile boundaries, plus G.
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 #include "blockscan.hpp"
@@ -21,6 +23,7 @@
 #include "segreduce.hpp"
 #include "sortctx.hpp"
 #include "tilereduce.hpp"
+#include "wavetable.hpp"
 
 namespace jfx {
 
@@ -358,17 +361,28 @@ __global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT, MM> a)
 template <typename K>
 __global__ void __launch_bounds__(1024)
 k_seg_segments(const uint32_t* total, uint32_t R, uint64_t mn, int oc, uint64_t n,
-               uint32_t* seg_start, K* out_keys, uint32_t* ngroups, uint32_t* dgid) {
+               uint32_t* seg_start, K* out_keys, uint32_t* ngroups, uint32_t* dgid,
+               uint32_t* skew, uint32_t lanes_per_table) {
   __shared__ uint32_t s_base[2048];
   __shared__ uint32_t s_gid[2048];
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
+  __shared__ uint32_t s_max;
   const int tid = threadIdx.x;
+  if (tid == 0) s_max = 0u;
+  __syncthreads();
+  uint32_t mx = 0u;
   for (int d = tid; d < 2048; d += 1024) {
     const uint32_t t = (uint32_t)d < R ? total[d] : 0u;
     s_base[d] = t;
     s_gid[d] = t ? 1u : 0u;
+    mx = t > mx ? t : mx;
   }
+  atomicMax(&s_max, mx);
   __syncthreads();
+  // skew selector for the small-range kernels: 1 when the most frequent key
+  // is expected in more than 2.7 of the lanes sharing one wave table per
+  // slot (8/3 * 64 = 170.7: s_max * lanes * 3 > 8 n)
+  if (skew && tid == 0) *skew = (uint64_t)s_max * lanes_per_table * 3u > n * 8u ? 1u : 0u;
   block_excl_scan_lds<2048, 1024>(s_base, s_tmp);
   const uint32_t G = block_excl_scan_lds<2048, 1024>(s_gid, s_tmp);
   for (int d = tid; d < (int)R; d += 1024) {
@@ -521,11 +535,77 @@ template <typename K>
 int segments_launch(const SortCtx& c, const SegWs& w, void* out_keys, hipStream_t s) {
   JFX_LAUNCH("seg_segments", s, (k_seg_segments<K><<<1, 1024, 0, s>>>(
                                      c.total, c.range_radix(), c.mn, c.oc, c.n, w.seg_start,
-                                     static_cast<K*>(out_keys), w.ngroups, w.dgid)));
+                                     static_cast<K*>(out_keys), w.ngroups, w.dgid,
+                                     w.ngroups + 1,
+                                     64u / wt_lane_groups(c.range_radix(),
+                                                          c.range_radix() <= 1024 ? 1024u : 2048u))));
   JFX_HIP(hipGetLastError());
   return JFX_OK;
 }
 
+
+// Kernel for the small-range groupby. By default both kernels are launched
+// and each reads the skew selector written by k_seg_segments, returning at
+// once unless it is the chosen one: the wave-table kernel for ordinary key
+// frequencies, the tile-sorted kernel when one key is frequent enough to make
+// the lanes of a wave collide on it. JFX_SMALL_RANGE=wave or =tile forces one.
+enum SmallRangePath { SR_AUTO = 0, SR_WAVE_TABLE = 1, SR_TILE_SORT = 2 };
+inline SmallRangePath small_range_path() {
+  const char* e = getenv("JFX_SMALL_RANGE");
+  if (e && strcmp(e, "tile") == 0) return SR_TILE_SORT;
+  if (e && strcmp(e, "wave") == 0) return SR_WAVE_TABLE;
+  return SR_AUTO;
+}
+
+// Waves per block (1, 2 or 4: the sort chunk must split evenly) so that a
+// block's tables take at most 64 KB when one table fits.
+constexpr int wt_waves(int te, size_t esz) {
+  return 65536 / (te * esz) >= 4 ? 4 : 65536 / (te * esz) >= 2 ? 2 : 1;
+}
+
+template <typename K, typename V, bool MM, int TE>
+int wave_table_te(const SortCtx& c, const SegWs& w, const void* keys, const void* vals,
+                  const uint32_t* sel, hipStream_t s, uint32_t& nparts) {
+  constexpr int W = wt_waves(TE, sizeof(WAcc<V, MM>) + sizeof(uint32_t));
+  constexpr uint64_t STEP = 64u * WT_RPL;
+  WaveTabArgs<K, V, MM> a{};
+  a.keys = static_cast<const K*>(keys);
+  a.vals = static_cast<const V*>(vals);
+  // raw bits of the minimum key (the range fast path only takes integer keys)
+  a.sub = c.oc == ORD_SIGNED ? (K)((K)c.mn ^ ((K)1 << (sizeof(K) * 8 - 1))) : (K)c.mn;
+  a.R = c.range_radix();
+  a.lg = wt_lane_groups(a.R, TE);
+  a.rt = (uint32_t)TE / a.lg;
+  a.n = c.n;
+  // Three blocks per CU fit in LDS: up to 768 blocks, each wave a whole
+  // number of steps, within the partial buffer (SORT_MAXCHUNKS x 2048).
+  const uint64_t cap = std::min<uint64_t>(768, (uint64_t)SORT_MAXCHUNKS * 2048 / a.R);
+  const uint64_t want = (c.n + (uint64_t)W * STEP - 1) / ((uint64_t)W * STEP);
+  uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, want));
+  a.rows_per_wave = ((c.n + (uint64_t)grid * W - 1) / ((uint64_t)grid * W) + STEP - 1) / STEP * STEP;
+  grid = (uint32_t)((c.n + a.rows_per_wave * W - 1) / (a.rows_per_wave * W));
+  a.sel = sel;
+  a.vec = ((uintptr_t)keys % 16 == 0 && (uintptr_t)vals % 16 == 0) ? 1 : 0;
+  a.part = static_cast<AggState<V, MM>*>(w.part);
+  a.err = c.err;
+  const char* nm = TE == 1024 ? "wave_table_1k" : "wave_table_2k";
+  if (a.vec) {
+    JFX_LAUNCH(nm, s, (k_wave_table<K, V, MM, TE, W, true><<<grid, W * 64, 0, s>>>(a)));
+  } else {
+    JFX_LAUNCH(nm, s, (k_wave_table<K, V, MM, TE, W, false><<<grid, W * 64, 0, s>>>(a)));
+  }
+  JFX_HIP(hipGetLastError());
+  nparts = grid;
+  return JFX_OK;
+}
+
+// Sets nparts to the number of partials the kernel writes (its grid size).
+template <typename K, typename V, bool MM>
+int wave_table_launch(const SortCtx& c, const SegWs& w, const void* keys, const void* vals,
+                      const uint32_t* sel, hipStream_t s, uint32_t& nparts) {
+  if (c.range_radix() <= 1024) return wave_table_te<K, V, MM, 1024>(c, w, keys, vals, sel, s, nparts);
+  return wave_table_te<K, V, MM, 2048>(c, w, keys, vals, sel, s, nparts);
+}
 
 template <typename K, typename V, typename MT, bool MM>
 int tile_reduce_mm(const SortCtx& c, const SegWs& w, const void* keys, const void* vals,
@@ -539,7 +619,17 @@ int tile_reduce_mm(const SortCtx& c, const SegWs& w, const void* keys, const voi
   a.n = c.n;
   a.chunk = c.chunk;
   a.part = static_cast<AggState<V, MM>*>(w.part);
-  if (a.R <= 256) {
+  const SmallRangePath path = small_range_path();
+  const uint32_t* sel = path == SR_AUTO ? w.ngroups + 1 : nullptr;
+  uint32_t wave_parts = 0;
+  if (path != SR_TILE_SORT) {
+    const int rc = wave_table_launch<K, V, MM>(c, w, keys, vals, sel, s, wave_parts);
+    if (rc != JFX_OK) return rc;
+  }
+  a.sel = sel;
+  if (path == SR_WAVE_TABLE) {
+    // the wave-table kernel alone
+  } else if (a.R <= 256) {
     JFX_LAUNCH("tile_reduce_d8", s, (k_tile_reduce<K, V, MM, 8, 8, 8><<<c.nchunks, 512, 0, s>>>(a)));
   } else if (a.R <= 1024) {
     JFX_LAUNCH("tile_reduce_d10", s, (k_tile_reduce<K, V, MM, 10, 8, 8><<<c.nchunks, 512, 0, s>>>(a)));
@@ -550,7 +640,8 @@ int tile_reduce_mm(const SortCtx& c, const SegWs& w, const void* keys, const voi
   GroupOut<V, MT> o{static_cast<V*>(sc.out_sum), sc.out_count, static_cast<V*>(sc.out_min),
                     static_cast<V*>(sc.out_max), static_cast<MT*>(sc.out_mean)};
   JFX_LAUNCH("tile_combine", s, (k_tile_combine<V, MT, MM><<<(a.R + 3) / 4, 256, 0, s>>>(
-                                     a.part, c.nchunks, a.R, w.dgid, o)));
+                                     a.part, path == SR_TILE_SORT ? c.nchunks : wave_parts,
+                                     a.R, w.dgid, o, c.total, sel, c.nchunks)));
   JFX_HIP(hipGetLastError());
   return JFX_OK;
 }

@@ -30,6 +30,7 @@ struct TileRedArgs {
   uint64_t n;
   uint64_t chunk;            // rows per chunk (multiple of the tile)
   AggState<V, MM>* part;     // [nchunks][R] chunk accumulators
+  const uint32_t* sel;       // skew selector: run only when it is nonzero (null: always run)
 };
 
 template <typename V, bool MM, int W>
@@ -108,6 +109,7 @@ k_tile_reduce(TileRedArgs<K, V, MM> a) {
   __shared__ uint32_t s_wf[W + 1];
   __shared__ uint32_t s_tmp[W + 1];
 
+  if (a.sel && *a.sel == 0u) return;
   const int tid = threadIdx.x;
   const int wid = tid >> 6;
   const uint32_t lane = lane_id();
@@ -235,10 +237,14 @@ struct GroupOut {
 template <typename V, typename MT, bool MM>
 __global__ void __launch_bounds__(256)
 k_tile_combine(const AggState<V, MM>* part, uint32_t nchunks, uint32_t R, const uint32_t* dgid,
-               GroupOut<V, MT> out) {
+               GroupOut<V, MT> out, const uint32_t* total, const uint32_t* sel,
+               uint32_t nchunks_sel) {
   const uint32_t d = blockIdx.x * 4u + (threadIdx.x >> 6);
   const uint32_t lane = lane_id();
   if (d >= R) return;
+  // the partial count of the kernel that ran: nchunks when the selector is
+  // zero (or absent), nchunks_sel when it is nonzero
+  if (sel && *sel != 0u) nchunks = nchunks_sel;
   const uint32_t per = (nchunks + 63u) / 64u;
   const uint32_t c0 = lane * per;
   const uint32_t c1 = (c0 + per) < nchunks ? (c0 + per) : nchunks;
@@ -255,6 +261,9 @@ k_tile_combine(const AggState<V, MM>* part, uint32_t nchunks, uint32_t R, const 
     if ((lane & (2u * off - 1u)) == 0u) st.merge(o);
   }
   if (lane == 0) {
+    // row counts come from the digit histogram (exact; the wave-table kernel
+    // does not count rows)
+    if (total) st.n = total[d];
     const uint32_t g = dgid[d];
     if (g != 0xFFFFFFFFu) {
       if (out.sum) out.sum[g] = st.sum();
