@@ -5,10 +5,10 @@ Workload (one step): groupby-sum of 100M rows, int32 keys uniform in
 [0, 1000), float32 values N(0, 1), synthetic and seeded, generated on the
 device before the timed region. A step is one full `groupby_aggregate(keys,
 {'v': vals}, {'v': 'sum'})` as `JaxFrame.groupby('k').agg({'v': 'sum'})` runs
-it: key statistics, then (the key range being below 2048) a counting sort of
-every tile by key inside LDS followed by the head-flag segmented reduction of
-the sorted tile, the in-order merge of the chunk partials, and the read-back
-of the group count.
+it: the key statistics pass (min / max and the per-digit row counts), then
+(the key range being below 2048) the wave-private LDS table kernel that folds
+every row into its group's compensated sum, the in-order merge of the block
+partials, and the read-back of the group count.
 
 Prints one JSON line with rows/s, per-step p50/p99, the roofline of the
 dominant kernel (HIP-event durations measured inside the timed region on the
@@ -43,8 +43,8 @@ def _cpu_model() -> str:
 
 def _kernel_bytes(name: str, n: int, kb: int, vb: int) -> float | None:
     """Algorithmic HBM bytes of one launch of a groupby-path kernel."""
-    if name.startswith("tile_reduce"):
-        return n * (kb + vb)                 # read keys + values (sorted in LDS)
+    if name.startswith("wave_table") or name.startswith("tile_reduce"):
+        return n * (kb + vb)                 # read keys + values once (folded in LDS)
     if name.startswith("chunk_scatter"):
         return n * (kb + vb) + n * vb        # read keys + values, write values
     if name == "segreduce_tiles":
@@ -192,7 +192,8 @@ def main() -> int:
         "data": "synthetic (device-generated, seeded)",
         "config": {"workload": "groupby-sum, 100M rows, int32 keys uniform [0,1000), "
                                "float32 N(0,1) values",
-                   "rows": n, "groups": ngroups, "agg": "sum", "path": "sort-based"},
+                   "rows": n, "groups": ngroups, "agg": "sum",
+                   "path": "small key range: key statistics pass, then wave-private LDS tables"},
         "step_alg_bytes": step_bytes,
         "step_hbm_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / PEAK_HBM_GBPS, 4),
         "roofline": roof,

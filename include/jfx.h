@@ -186,6 +186,27 @@ int jfx_join_first_match(int key_type, const void* sorted_probe, uint64_t n_prob
 int jfx_gather(int elem_bytes, const void* src, const uint32_t* rows, uint64_t n, void* dst,
                uint64_t fill_bits, void* stream);
 
+/* ---- composite keys (multi-column sort / groupby / join) -------------- */
+/* Packs 1..8 integer key columns (JFX_U32/I32/U64/I64, n rows each) into one
+ * uint64 key per row whose order and equality are the lexicographic ones of
+ * the columns: field j = ordered(k_j) - sub[j] (descending[j]: sub[j] -
+ * ordered(k_j)), bits[j] wide, the last column least significant; ordered()
+ * flips the sign bit of signed columns and sub[j] is the ordered minimum
+ * (maximum when descending) of column j. Replaces the per-column stable
+ * argsorts of multi_column_lexsort (parallel_algorithms.py:25-66) and the
+ * OR of per-column head flags of groupby_aggregate_multi_column (:697-711)
+ * by one sort / groupby of the composite. sum(bits) <= 64; descending may be
+ * NULL (all ascending). */
+int jfx_pack_keys(const void* const* keys, const int* key_types, int ncols, const uint64_t* sub,
+                  const uint32_t* bits, const int* descending, uint64_t n, uint64_t* out,
+                  void* stream);
+
+/* Inverse of jfx_pack_keys with the same column description: writes every
+ * column of each packed key to outs[j] (a column of key_types[j]). */
+int jfx_unpack_keys(const uint64_t* packed, uint64_t n, const int* key_types, int ncols,
+                    const uint64_t* sub, const uint32_t* bits, const int* descending,
+                    void* const* outs, void* stream);
+
 /* ---- per-kernel timing ------------------------------------------------ */
 /* No counterpart in the reference. When enabled, every kernel launch is
  * bracketed by HIP events on its own stream; totals per kernel name are

@@ -173,14 +173,25 @@ def groupby_aggregate_multi_column(keys: list, key_names: list, values: dict, ag
     """Multi-key groupby (parallel_algorithms.py:662-731): rows are ordered by a
     stable lexicographic sort (last key first) and grouped where any key
     changes; returns ({key_name: unique}, {col: aggregated})."""
-    from .multikey import lexsort_perm, multikey_heads
+    from .multikey import composite_plan, lexsort_perm, multikey_heads, pack, unpack
     _check_aggs(agg_funcs)
+    _lib.require_gpu()
     ks = [to_device(k) for k in keys]
     dev = ks[0].device
     n = ks[0].shape[0]
     if n == 0:
         return ({nm: k[:0] for nm, k in zip(key_names, ks)},
                 {c: to_device(values[c], dev)[:0] for c in agg_funcs})
+    wide = [widen_key(k) for k in ks]
+    plan = composite_plan([w for w, _ in wide]) if len(ks) > 1 else None
+    if plan is not None:
+        # integer keys whose ranges fit 64 bits: one single-key groupby of the
+        # composite key, whose groups and order are those of the key tuples
+        ukeys, res = groupby_aggregate(pack([w for w, _ in wide], plan), values, agg_funcs)
+        cols = unpack(ukeys, plan)
+        uniq = {nm: (c.to(orig) if c.dtype != orig else c)
+                for nm, c, (_, orig) in zip(key_names, cols, wide)}
+        return uniq, res
     perm = lexsort_perm(ks)
     sorted_keys = [k[perm] for k in ks]
     group_id, starts = multikey_heads(sorted_keys)
