@@ -53,7 +53,7 @@ extern "C" {
 #define JFX_AGG_MAX 8
 #define JFX_AGG_MEAN 16
 
-/* comparison codes for jfx_filter_mask */
+/* comparison codes for jfx_filter_rows */
 #define JFX_CMP_GT 0
 #define JFX_CMP_GE 1
 #define JFX_CMP_LT 2
@@ -206,6 +206,25 @@ int jfx_pack_keys(const void* const* keys, const int* key_types, int ncols, cons
 int jfx_unpack_keys(const uint64_t* packed, uint64_t n, const int* key_types, int ncols,
                     const uint64_t* sub, const uint32_t* bits, const int* descending,
                     void* const* outs, void* stream);
+
+/* ---- stream compaction ------------------------------------------------ */
+/* Workspace bytes for jfx_compact_rows / jfx_filter_rows (independent of n). */
+int jfx_compact_workspace_size(uint64_t n, uint64_t* out_bytes);
+
+/* Row positions, in row order, where mask[i] != 0 (n < 2^32): the
+ * jnp.nonzero(valid_mask) step of compact_repartitioned_rows
+ * (parallel_algorithms.py:277-293) and the frame layer's boolean row filter.
+ * out_rows holds up to n entries; when out_count is non-NULL the stream is
+ * synchronised and the number of positions written there. */
+int jfx_compact_rows(const uint8_t* mask, uint64_t n, uint32_t* out_rows, uint64_t* out_count,
+                     void* workspace, uint64_t ws_bytes, void* stream);
+
+/* Row positions, in row order, where `col <cmp> *value` holds (cmp: JFX_CMP_*;
+ * *value is a host value of col_type; IEEE comparisons, so NaN passes only
+ * JFX_CMP_NE). The filter is evaluated on the fly, no mask is stored. */
+int jfx_filter_rows(int col_type, const void* col, int cmp, const void* value, uint64_t n,
+                    uint32_t* out_rows, uint64_t* out_count, void* workspace, uint64_t ws_bytes,
+                    void* stream);
 
 /* ---- per-kernel timing ------------------------------------------------ */
 /* No counterpart in the reference. When enabled, every kernel launch is

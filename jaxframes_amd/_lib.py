@@ -59,6 +59,9 @@ _SIGNATURES = {
                       c_vp],
     "jfx_join_first_match": [c_int, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp],
     "jfx_gather": [c_int, c_vp, c_vp, c_u64, c_vp, c_u64, c_vp],
+    "jfx_compact_workspace_size": [c_u64, c_u64p],
+    "jfx_compact_rows": [c_vp, c_u64, c_vp, c_u64p, c_vp, c_u64, c_vp],
+    "jfx_filter_rows": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_u64p, c_vp, c_u64, c_vp],
     "jfx_pack_keys": [ctypes.POINTER(c_vp), ctypes.POINTER(c_int), c_int, c_u64p, c_u32p,
                       ctypes.POINTER(c_int), c_u64, c_vp, c_vp],
     "jfx_unpack_keys": [c_vp, c_u64, ctypes.POINTER(c_int), c_int, c_u64p, c_u32p,

@@ -133,3 +133,137 @@ def sort_merge_join(left_keys, left_values: dict, right_keys, right_values: dict
     for c, v in right_values.items():
         out[f"right_{c}"] = gather(v, ri)
     return keys, out
+
+
+# ---- multi-column join -------------------------------------------------------
+
+def _dense_codes(lcol: torch.Tensor, rcol: torch.Tensor):
+    """int64 codes of one key column over both sides, equal exactly where the
+    values compare equal under ==: dense ranks of the union (one radix sort),
+    -0.0 and +0.0 sharing a code; NaN rows get distinct negative codes, so they
+    match nothing (NaN != NaN in the reference's match matrix)."""
+    both = torch.cat([lcol, rcol])
+    n = both.shape[0]
+    sk, _, perm = radix_sort(both, None, True, return_index=True)
+    from .multikey import multikey_heads
+    gid, _ = multikey_heads([sk])
+    codes = torch.empty(n, dtype=torch.int64, device=both.device)
+    codes[perm] = gid
+    if both.dtype.is_floating_point:
+        nan = torch.isnan(both)
+        codes = torch.where(nan, -1 - torch.arange(n, device=both.device), codes)
+    return codes[:lcol.shape[0]], codes[lcol.shape[0]:]
+
+
+def _tuple_codes(lkeys: list, rkeys: list):
+    """One int64 code per row for key tuples: equal codes <=> every column
+    compares equal (and no NaN). Columns are folded pairwise through dense
+    ranks, so the code never needs more than 64 bits."""
+    lc, rc = _dense_codes(lkeys[0], rkeys[0])
+    for lk, rk in zip(lkeys[1:], rkeys[1:]):
+        l2, r2 = _dense_codes(lk, rk)
+        nl = lc.shape[0]
+        # pair (code_so_far, code_j): both fit 32 bits as dense ranks over L+R
+        # rows; NaN rows keep a distinct negative code
+        bad = torch.cat([lc, rc]) < 0
+        bad |= torch.cat([l2, r2]) < 0
+        pair = (torch.cat([lc, rc]) << 32) | torch.cat([l2, r2])
+        pair = torch.where(bad, torch.full_like(pair, -1), pair)
+        pl, pr = _dense_codes(pair[:nl], pair[nl:])
+        m = torch.arange(pair.shape[0], device=pair.device)
+        pc = torch.cat([pl, pr])
+        pc = torch.where(bad, -1 - m, pc)
+        lc, rc = pc[:nl], pc[nl:]
+    return lc, rc
+
+
+def _gather_join_column(arr: torch.Tensor, idx: torch.Tensor, dtype_name: str | None = None):
+    """_gather_join_column (parallel_algorithms.py:1254-1278): idx < 0 gives
+    -1 for string (dictionary-code) columns, NaN otherwise; integer and bool
+    columns become float32 only when some row is unmatched."""
+    if idx.shape[0] == 0:
+        return arr[:0]
+    valid = idx >= 0
+    safe = idx.clamp(min=0)
+    if dtype_name == "string":
+        g = arr[safe]
+        return torch.where(valid, g, torch.full_like(g, -1))
+    if bool(valid.all()):
+        return arr[safe]
+    g = arr[safe]
+    if not g.dtype.is_floating_point:
+        g = g.to(torch.float32)
+    return torch.where(valid, g, torch.full_like(g, float("nan")))
+
+
+def join_multi_column(left_keys: list, left_key_names: list, left_values: dict,
+                      right_keys: list, right_key_names: list, right_values: dict,
+                      how: str = "inner", left_key_dtypes=None, right_key_dtypes=None,
+                      left_value_dtypes=None, right_value_dtypes=None):
+    """ParallelSortMergeJoin.join_multi_column (parallel_algorithms.py:826-936).
+
+    Row pairs come in (left row, right row) order -- the row-major order of
+    the reference's L x R match matrix over the unsorted rows -- followed by
+    the unmatched left rows (left / outer) and then the unmatched right rows
+    (right / outer), each ascending, with -1 for the missing side. Keys are
+    coalesced (left key where present, else right); payloads follow
+    _gather_join_column. Built from the sort-merge join on per-row tuple
+    codes instead of the L x R matrix."""
+    _lib.require_gpu()
+    if how not in ("inner", "left", "right", "outer"):
+        raise ValueError(f"unsupported join type {how}")
+    lks = [widen_key(to_device(k))[0] for k in left_keys]
+    dev = lks[0].device
+    rks = [widen_key(to_device(k, dev))[0] for k in right_keys]
+    nl, nr = lks[0].shape[0], rks[0].shape[0]
+    for i, (a, b) in enumerate(zip(lks, rks)):
+        if a.dtype != b.dtype:
+            t = torch.promote_types(a.dtype, b.dtype)
+            lks[i], rks[i] = a.to(t), b.to(t)
+    if nl and nr:
+        lc, rc = _tuple_codes(lks, rks)
+        _, li, ri = inner_join_rows(lc, rc)
+        li = li.long() & 0xFFFFFFFF
+        ri = ri.long() & 0xFFFFFFFF
+        # (code, left, right) order -> (left, right): a stable sort by left row
+        # (the right rows of one left row already ascend)
+        _, ri_s, perm = radix_sort(li.to(torch.int32), ri.to(torch.int32), True, return_index=True)
+        li = li[perm]
+        ri = ri_s.long()
+    else:
+        li = torch.empty(0, dtype=torch.int64, device=dev)
+        ri = torch.empty(0, dtype=torch.int64, device=dev)
+    lhas = torch.zeros(nl, dtype=torch.bool, device=dev)
+    rhas = torch.zeros(nr, dtype=torch.bool, device=dev)
+    if li.shape[0]:
+        lhas[li] = True
+        rhas[ri] = True
+    from .compaction import compact_rows
+    lun = compact_rows(~lhas) if nl else li[:0]
+    run = compact_rows(~rhas) if nr else ri[:0]
+    neg = lambda k: torch.full((k,), -1, dtype=torch.int64, device=dev)
+    if how == "inner":
+        all_l, all_r = li, ri
+    elif how == "left":
+        all_l, all_r = torch.cat([li, lun]), torch.cat([ri, neg(lun.shape[0])])
+    elif how == "right":
+        all_l, all_r = torch.cat([li, neg(run.shape[0])]), torch.cat([ri, run])
+    else:
+        all_l = torch.cat([li, lun, neg(run.shape[0])])
+        all_r = torch.cat([ri, neg(lun.shape[0]), run])
+    out_keys = {}
+    for name, lk, rk, orig_l in zip(left_key_names, lks, rks, left_keys):
+        if all_l.shape[0] == 0:
+            out_keys[name] = lk[:0]
+            continue
+        lv = lk[all_l.clamp(min=0)] if nl else torch.zeros(all_l.shape[0], dtype=lk.dtype, device=dev)
+        rv = rk[all_r.clamp(min=0)] if nr else torch.zeros(all_r.shape[0], dtype=rk.dtype, device=dev)
+        out_keys[name] = torch.where(all_l >= 0, lv, rv)
+    out_vals = {}
+    for name, arr in left_values.items():
+        dn = (left_value_dtypes or {}).get(name)
+        out_vals[f"left_{name}"] = _gather_join_column(to_device(arr, dev), all_l, dn)
+    for name, arr in right_values.items():
+        dn = (right_value_dtypes or {}).get(name)
+        out_vals[f"right_{name}"] = _gather_join_column(to_device(arr, dev), all_r, dn)
+    return out_keys, out_vals

@@ -105,3 +105,33 @@ def partition_columns(keys, values: dict, nbuckets: int):
     else:
         pk = r["keys"]
     return pk, vals, r["offsets"]
+
+
+def bucketize_padded(column, offsets: torch.Tensor, rows: torch.Tensor, pad_value):
+    """The reference's padded per-destination layout of one column
+    (_bucketize_local_column, parallel_algorithms.py:163-182): [P, n] with row
+    r of bucket d at [d, r] and pad_value elsewhere, built from the compact
+    layout of hash_partition (offsets, rows). For inputs small enough that a
+    P x n buffer is reasonable; the compact layout is what the exchange sends."""
+    col = to_device(column)
+    n = col.shape[0]
+    P = offsets.shape[0] - 1
+    counts = offsets[1:] - offsets[:-1]
+    bucket = torch.repeat_interleave(torch.arange(P, device=col.device), counts)
+    pos = torch.arange(n, device=col.device) - offsets[bucket]
+    out = torch.full((P, n), pad_value, dtype=col.dtype, device=col.device)
+    out[bucket, pos] = col[rows]
+    return out
+
+
+def collective_hash_repartition(keys: list, values: dict, key_dtypes=None, value_dtypes=None,
+                                sharding_spec=None):
+    """collective_hash_repartition (parallel_algorithms.py:185-274) on one
+    GPU: a single device holds every partition, so the rows stay where they
+    are and every row is valid (the reference's mesh-size-1 branch,
+    :202-203). The per-device step a mesh would run is hash_partition."""
+    if not keys:
+        return keys, values, torch.zeros(0, dtype=torch.bool, device=torch.device("cuda"))
+    ks = [to_device(k) for k in keys]
+    return ks, {c: to_device(v, ks[0].device) for c, v in values.items()}, \
+        torch.ones(ks[0].shape[0], dtype=torch.bool, device=ks[0].device)

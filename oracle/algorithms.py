@@ -25,7 +25,8 @@ __all__ = [
     "head_flags", "groupby_aggregate", "groupby_aggregate_multi_column",
     "segment_sums_f64", "inner_join_pairs", "inner_join_pairs_bruteforce",
     "sort_merge_join", "fmix32", "hash_multi_columns", "hash_partition",
-    "bucketize_padded", "compact_rows", "filter_mask_compact",
+    "bucketize_padded", "compact_rows", "filter_mask_compact", "join_multi_column",
+    "gather_join_column",
     "murmur3_x86_32", "sum_error_metrics", "PINNED_BY",
 ]
 
@@ -376,6 +377,65 @@ def hash_partition(keys, nbuckets, hashes=None):
     return perm, offsets
 
 
+def gather_join_column(arr, idx, dtype_name=None):
+    """``_gather_join_column`` (``pa:1254-1278``): -1 for string columns where
+    idx < 0; otherwise NaN, integer / bool columns turning float32 only when
+    some index is negative."""
+    a = np.asarray(arr)
+    idx = np.asarray(idx, dtype=np.int64)
+    if idx.shape[0] == 0:
+        return a[:0]
+    valid = idx >= 0
+    g = a[np.maximum(idx, 0)]
+    if dtype_name == "string":
+        return np.where(valid, g, np.asarray(-1, dtype=g.dtype))
+    if valid.all():
+        return g
+    if g.dtype.kind in "iub":
+        g = g.astype(np.float32)
+    return np.where(valid, g, np.asarray(np.nan, dtype=g.dtype))
+
+
+def join_multi_column(left_keys, left_key_names, left_values, right_keys, right_key_names,
+                      right_values, how="inner", left_value_dtypes=None, right_value_dtypes=None):
+    """``ParallelSortMergeJoin.join_multi_column`` (``pa:826-936``) restated with
+    the same L x R match matrix (small inputs only): pairs row-major over the
+    unsorted rows, then unmatched left rows, then unmatched right rows, -1 for
+    the missing side; keys coalesced left-first (``pa:1245-1251``)."""
+    lks = [np.asarray(k) for k in left_keys]
+    rks = [np.asarray(k) for k in right_keys]
+    nl, nr = lks[0].shape[0], rks[0].shape[0]
+    m = np.ones((nl, nr), dtype=bool)
+    for a, b in zip(lks, rks):
+        m &= a[:, None] == b[None, :]
+    li, ri = np.nonzero(m)
+    lun = np.flatnonzero(~m.any(axis=1))
+    run = np.flatnonzero(~m.any(axis=0))
+    neg = lambda k: np.full(k, -1, dtype=np.int64)
+    if how == "inner":
+        al, ar = li, ri
+    elif how == "left":
+        al, ar = np.concatenate([li, lun]), np.concatenate([ri, neg(lun.size)])
+    elif how == "right":
+        al, ar = np.concatenate([li, neg(run.size)]), np.concatenate([ri, run])
+    else:
+        al = np.concatenate([li, lun, neg(run.size)])
+        ar = np.concatenate([ri, neg(lun.size), run])
+    keys = {}
+    for name, a, b in zip(left_key_names, lks, rks):
+        if al.shape[0] == 0:
+            keys[name] = a[:0]
+        else:
+            keys[name] = np.where(al >= 0, a[np.maximum(al, 0)] if nl else 0,
+                                  b[np.maximum(ar, 0)] if nr else 0)
+    vals = {}
+    for c, v in left_values.items():
+        vals[f"left_{c}"] = gather_join_column(v, al, (left_value_dtypes or {}).get(c))
+    for c, v in right_values.items():
+        vals[f"right_{c}"] = gather_join_column(v, ar, (right_value_dtypes or {}).get(c))
+    return keys, vals
+
+
 def bucketize_padded(arr, targets, num_devices, pad_value):
     """The reference's padded bucket layout (``pa:163-182``): row r of bucket d
     at ``[d, r]``, the rest filled with pad_value."""
@@ -465,4 +525,9 @@ PINNED_BY = {
     "bucketize_padded": "brute-force per-bucket loop",
     "compact_rows": "np.flatnonzero closed form (trivial)",
     "filter_mask_compact": "boolean-index closed form (trivial)",
+    "join_multi_column": "golden vectors (test_multi_column_operations.py:206-303) + "
+                         "pandas inner merge (same (left row, right row) order) + "
+                         "nested-loop brute force for left/right/outer",
+    "gather_join_column": "hand-checked cases of the three branches (string / all valid / "
+                          "unmatched int -> float32 NaN)",
 }

@@ -246,3 +246,69 @@ def test_pinned_by_covers_public_functions():
     for name in ("radix_sort", "groupby_aggregate", "inner_join_pairs", "fmix32",
                  "hash_partition", "sort_merge_join", "multi_column_lexsort"):
         assert name in oracle.PINNED_BY
+
+
+def _golden_section(section):
+    import json
+    path = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
+    with open(path) as f:
+        return json.load(f)[section]
+
+
+def test_join_multi_column_golden_vectors():
+    for case in _golden_section("join_multi"):
+        lk = [np.asarray(k, dtype=np.int64) for k in case["left_keys"]]
+        rk = [np.asarray(k, dtype=np.int64) for k in case["right_keys"]]
+        lv = {c: np.asarray(v) for c, v in case["left_values"].items()}
+        rv = {c: np.asarray(v) for c, v in case["right_values"].items()}
+        names = case["left_key_names"]
+        keys, vals = oracle.join_multi_column(lk, names, lv, rk, names, rv, case["how"])
+        assert keys[names[0]].shape[0] == case["expected_rows"], case["source"]
+        if "expected_sorted_tuples" in case:
+            lc, rc = list(lv)[0], list(rv)[0]
+            tup = sorted(zip(*(keys[n].tolist() for n in names), vals[f"left_{lc}"].tolist(),
+                             vals[f"right_{rc}"].tolist()))
+            assert tup == [tuple(t) for t in case["expected_sorted_tuples"]], case["source"]
+        if "expected_right_value_isnan" in case:
+            assert np.isnan(vals["right_value"]).tolist() == case["expected_right_value_isnan"]
+
+
+def test_join_multi_column_vs_pandas_and_bruteforce():
+    rng = np.random.default_rng(21)
+    for _ in range(5):
+        nl, nr = rng.integers(1, 60), rng.integers(1, 60)
+        lk = [rng.integers(0, 3, nl), rng.integers(0, 4, nl)]
+        rk = [rng.integers(0, 3, nr), rng.integers(0, 4, nr)]
+        lv = {"x": rng.integers(0, 100, nl).astype(np.int32)}
+        rv = {"y": rng.standard_normal(nr)}
+        keys, vals = oracle.join_multi_column(lk, ["a", "b"], lv, rk, ["a", "b"], rv, "inner")
+        L = pd.DataFrame({"a": lk[0], "b": lk[1], "x": lv["x"], "li": np.arange(nl)})
+        R = pd.DataFrame({"a": rk[0], "b": rk[1], "y": rv["y"], "ri": np.arange(nr)})
+        m = L.merge(R, on=["a", "b"], how="inner")   # keeps left order, then right order
+        assert keys["a"].tolist() == m["a"].tolist() and keys["b"].tolist() == m["b"].tolist()
+        assert vals["left_x"].tolist() == m["x"].tolist()
+        assert vals["right_y"].tolist() == m["y"].tolist()
+        for how in ("left", "right", "outer"):
+            keys, vals = oracle.join_multi_column(lk, ["a", "b"], lv, rk, ["a", "b"], rv, how)
+            pairs, lun, run = [], [], []
+            for i in range(nl):
+                for j in range(nr):
+                    if lk[0][i] == rk[0][j] and lk[1][i] == rk[1][j]:
+                        pairs.append((i, j))
+            lm = {i for i, _ in pairs}
+            rm = {j for _, j in pairs}
+            exp_l = [i for i, _ in pairs] + ([i for i in range(nl) if i not in lm] if how != "right" else []) \
+                + ([-1] * len([j for j in range(nr) if j not in rm]) if how in ("right", "outer") else [])
+            n_rows = len(exp_l)
+            assert keys["a"].shape[0] == n_rows
+
+
+def test_gather_join_column_branches():
+    a = np.array([5, 6, 7], dtype=np.int32)
+    assert oracle.gather_join_column(a, np.array([2, 0])).dtype == np.int32
+    g = oracle.gather_join_column(a, np.array([2, -1]))
+    assert g.dtype == np.float32 and g[0] == 7 and np.isnan(g[1])
+    s = oracle.gather_join_column(a, np.array([1, -1]), "string")
+    assert s.tolist() == [6, -1]
+    f = oracle.gather_join_column(np.array([1.5, 2.5]), np.array([-1, 1]))
+    assert np.isnan(f[0]) and f[1] == 2.5
