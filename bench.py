@@ -81,6 +81,51 @@ def cpu_baseline(n_sample: int, seed: int):
     return n_sample / dt, dt
 
 
+def batched_vs_plain(total_rows: int, reps: int = 5, plain_frames: int = 256):
+    """Batched groupby of 4096 frames (row counts log-uniform in [1K, 4M]
+    scaled to total_rows, int32 keys with per-frame cardinality log-uniform
+    in [10, 1M], float32 N(0,1) values) as one HIP-graph replay, against the
+    plain path (one groupby_aggregate call per frame) on the first
+    plain_frames frames."""
+    import torch
+    from jaxframes_amd import batched, datagen
+    keys, vals, offs, cards = datagen.batched_frames(total_rows=total_rows)
+    n = int(keys.shape[0])
+    plan = batched.BatchedGroupBy(keys, vals, offs, key_range=(0, int(cards.max()) - 1),
+                                  aggs=("sum", "count"))
+    plan.run(graph=True)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        a = time.perf_counter()
+        plan.launch(graph=True)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+    dt = (time.perf_counter() - t0) / reps
+    groups = int(plan.result().keys.shape[0])
+    rows_plain = int(offs[plain_frames])
+    batched.plain_per_frame(keys, vals, offs, "sum", frames=range(4))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    batched.plain_per_frame(keys, vals, offs, "sum", frames=range(plain_frames))
+    torch.cuda.synchronize()
+    plain_rps = rows_plain / (time.perf_counter() - t1)
+    out = {"workload": f"4096 frames, {n} rows, int32 keys (per-frame cardinality log-uniform "
+                       f"[10, 1M]), float32 values; sum + count per frame",
+           "rows": n, "frames": int(offs.shape[0] - 1), "groups": groups,
+           "ms_per_batch": round(dt * 1e3, 3), "rows_per_s": round(n / dt, 1),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "plain_rows_per_s": round(plain_rps, 1),
+           "plain_sample": f"first {plain_frames} frames, {rows_plain} rows",
+           "speedup_vs_plain": round(n / dt / plain_rps, 2),
+           "hip_graph": True, "digit_bits": plan.digit_bits}
+    del plan, keys, vals
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +134,8 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--cpu-sample", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-rows", type=int, default=400_000_000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 400 MB (float32[1e8]) so that PMC byte "
                          "counters can be scaled against a transfer of known size")
@@ -166,6 +213,8 @@ def main() -> int:
     step_bytes = sum((_kernel_bytes(k, n, 4, 4) or 0) * (v[1] // args.steps)
                      for k, v in kstats.items())
 
+    batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows)
+
     cpu = None
     if not args.no_cpu_baseline:
         sample = min(args.cpu_sample, n)
@@ -197,6 +246,7 @@ def main() -> int:
         "step_alg_bytes": step_bytes,
         "step_hbm_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / PEAK_HBM_GBPS, 4),
         "roofline": roof,
+        "batched": batched_line,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -186,6 +186,28 @@ int jfx_join_first_match(int key_type, const void* sorted_probe, uint64_t n_prob
 int jfx_gather(int elem_bytes, const void* src, const uint32_t* rows, uint64_t n, void* dst,
                uint64_t fill_bits, void* stream);
 
+/* ---- batched groupby ---------------------------------------------------- */
+/* Many independent single-key groupbys of uneven row counts ("frames"),
+ * concatenated row-wise with frame_offsets (device int64[nframes+1], 0 .. n),
+ * in one device pipeline with no host synchronisation (capturable in a HIP
+ * graph): composite key (frame << key_bits) | (ordered(key) - ordered(key_min))
+ * per row, one stable radix sort of the composites over exactly
+ * ceil(log2 nframes) + key_bits bits, one head-flag segmented reduction.
+ * Each frame's groups equal SortBasedGroupBy.groupby_aggregate
+ * (parallel_algorithms.py:733-803) of that frame. Groups come out
+ * frame-major, keys ascending; out_keys receives the composite keys (uint32
+ * when the bit total is <= 32, else uint64); *out_ngroups (a DEVICE word) the
+ * total group count. Keys must lie in [key_min, key_min + 2^key_bits)
+ * (checked; a violation raises the device error word). Output buffers hold n
+ * groups. No counterpart in the reference, which runs one groupby per frame. */
+int jfx_batched_groupby_workspace_size(uint32_t total_bits, int val_type, uint64_t n,
+                                       uint64_t* out_bytes);
+int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
+                        const int64_t* frame_offsets, uint32_t nframes, uint64_t key_min,
+                        uint32_t key_bits, int digit_bits, void* out_keys, void* out_sum,
+                        int64_t* out_count, void* out_min, void* out_max, void* out_mean,
+                        uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes, void* stream);
+
 /* ---- composite keys (multi-column sort / groupby / join) -------------- */
 /* Packs 1..8 integer key columns (JFX_U32/I32/U64/I64, n rows each) into one
  * uint64 key per row whose order and equality are the lexicographic ones of

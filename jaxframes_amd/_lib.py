@@ -17,6 +17,9 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libjfx.so"
 JFX_U32, JFX_I32, JFX_F32, JFX_U64, JFX_I64, JFX_F64 = range(6)
 AGG_SUM, AGG_COUNT, AGG_MIN, AGG_MAX, AGG_MEAN = 1, 2, 4, 8, 16
 CMP_CODES = {">": 0, ">=": 1, "<": 2, "<=": 3, "==": 4, "!=": 5}
+# device error word bits (csrc/common.hpp)
+DEVERR_LOOKBACK = 1   # a bounded cross-workgroup wait ran out
+DEVERR_BOUNDS = 2     # a computed position or key fell outside its declared range
 
 _ERRNAMES = {1: "invalid argument", 2: "HIP error", 3: "unsupported", 4: "workspace too small",
              5: "device error"}
@@ -59,6 +62,10 @@ _SIGNATURES = {
                       c_vp],
     "jfx_join_first_match": [c_int, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp],
     "jfx_gather": [c_int, c_vp, c_vp, c_u64, c_vp, c_u64, c_vp],
+    "jfx_batched_groupby_workspace_size": [ctypes.c_uint32, c_int, c_u64, c_u64p],
+    "jfx_batched_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, ctypes.c_uint32, c_u64,
+                            ctypes.c_uint32, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                            c_u64, c_vp],
     "jfx_compact_workspace_size": [c_u64, c_u64p],
     "jfx_compact_rows": [c_vp, c_u64, c_vp, c_u64p, c_vp, c_u64, c_vp],
     "jfx_filter_rows": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_u64p, c_vp, c_u64, c_vp],

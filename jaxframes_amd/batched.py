@@ -1,0 +1,225 @@
+"""Batched groupby: thousands of independent groupbys in one device pipeline.
+
+Input: one concatenated table (keys, values) with frame offsets; frame f is
+rows offsets[f]:offsets[f+1]. Output: every frame's groups, frame-major, as
+SortBasedGroupBy.groupby_aggregate (parallel_algorithms.py:733-803) would give
+frame by frame, with a CSR offset array per frame.
+
+The device work is jfx_batched_groupby (composite (frame, key) keys, one radix
+sort, one segmented reduction) and has no host synchronisation, so a plan can
+be captured once in a HIP graph and replayed; the digit width of the sort is
+picked per size class by a small autotuner that times the candidates on the
+first call.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import threading
+import time
+
+import torch
+
+from . import _lib
+from ._tensors import key_code, to_device, widen_key
+from .grouping import _VAL_CODE, _alloc_out, _value_tensor
+
+AGGS = ("sum", "count", "min", "max", "mean")
+
+
+class Autotuner:
+    """Best digit width (8 or 11 bits per radix pass) per (row-count class,
+    composite bit width), measured on first use. Thread-safe; optionally
+    persisted as JSON (path from JFX_TUNE_FILE)."""
+
+    CANDIDATES = (8, 11)
+
+    def __init__(self, path: str | None = None):
+        self._lock = threading.Lock()
+        self.path = path if path is not None else os.environ.get("JFX_TUNE_FILE")
+        self.table: dict[str, int] = {}
+        if self.path and os.path.exists(self.path):
+            with open(self.path) as f:
+                self.table = {k: int(v) for k, v in json.load(f).items()}
+
+    @staticmethod
+    def key(n: int, total_bits: int) -> str:
+        return f"n2^{max(n, 1).bit_length()}:bits{total_bits}"
+
+    def lookup(self, n: int, total_bits: int):
+        with self._lock:
+            return self.table.get(self.key(n, total_bits))
+
+    def tune(self, n: int, total_bits: int, run, reps: int = 2) -> int:
+        """run(digit_bits) launches one batch on the current stream."""
+        k = self.key(n, total_bits)
+        with self._lock:
+            if k in self.table:
+                return self.table[k]
+        best, best_t = None, None
+        for d in self.CANDIDATES:
+            run(d)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run(d)
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / reps
+            if best_t is None or t < best_t:
+                best, best_t = d, t
+        with self._lock:
+            self.table[k] = best
+            if self.path:
+                with open(self.path, "w") as f:
+                    json.dump(self.table, f)
+        return best
+
+
+TUNER = Autotuner()
+
+
+class BatchResult:
+    """Groups of every frame: offsets (int64[F+1]) into keys and the
+    aggregate columns."""
+
+    def __init__(self, offsets, keys, values: dict):
+        self.offsets = offsets
+        self.keys = keys
+        self.values = values
+
+    def frame(self, f: int):
+        a, b = int(self.offsets[f]), int(self.offsets[f + 1])
+        return self.keys[a:b], {c: v[a:b] for c, v in self.values.items()}
+
+
+class BatchedGroupBy:
+    """A reusable plan for one table shape: frame offsets, key range, value
+    type and aggregations fixed at construction.
+
+    keys / values: the concatenated columns (device tensors). frame_offsets:
+    int64 offsets, length nframes + 1. key_range: (min, max) of the keys over
+    all frames if known, else measured once here. aggs: subset of AGGS.
+    """
+
+    def __init__(self, keys, values, frame_offsets, *, key_range=None, aggs=("sum",),
+                 digit_bits: int = 0, tuner: Autotuner | None = TUNER):
+        _lib.require_gpu()
+        for a in aggs:
+            if a not in AGGS:
+                raise ValueError(f"Unsupported aggregation function: {a}")
+        self.keys, self.korig = widen_key(to_device(keys))
+        self.dev = self.keys.device
+        self.vals, self.vorig = _value_tensor(to_device(values, self.dev))
+        self.offsets = to_device(torch.as_tensor(frame_offsets, dtype=torch.int64), self.dev)
+        self.n = self.keys.shape[0]
+        self.nframes = self.offsets.shape[0] - 1
+        if self.vals.shape[0] != self.n or int(self.offsets[-1]) != self.n or int(self.offsets[0]) != 0:
+            raise ValueError("frame offsets must run from 0 to the number of rows")
+        self.kc = key_code(self.keys)
+        if key_range is None:
+            if self.n == 0:
+                key_range = (0, 0)
+            else:
+                mn, mx = torch.aminmax(self.keys)
+                key_range = (int(mn), int(mx))
+        self.kmin, kmax = int(key_range[0]), int(key_range[1])
+        span = kmax - self.kmin
+        self.key_bits = max(1, span.bit_length())
+        self.frame_bits = max(0, (self.nframes - 1).bit_length())
+        self.total_bits = self.key_bits + self.frame_bits
+        if self.total_bits > 64:
+            raise ValueError("frame and key bits exceed 64")
+        self.comp_dtype = torch.int32 if self.total_bits <= 32 else torch.int64
+        self.vcode = _VAL_CODE[self.vals.dtype]
+        self.aggs = tuple(aggs)
+        cap = max(self.n, 1)
+        self.out_keys = torch.empty(cap, dtype=self.comp_dtype, device=self.dev)
+        self.outs = {a: _alloc_out(a, self.vals, cap, self.dev) for a in self.aggs}
+        self.ngroups = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        nb = ctypes.c_uint64(0)
+        _lib.call("jfx_batched_groupby_workspace_size", self.total_bits, self.vcode, self.n,
+                  ctypes.byref(nb))
+        self.ws = torch.empty(nb.value, dtype=torch.uint8, device=self.dev)
+        self.graph = None
+        if digit_bits:
+            self.digit_bits = digit_bits
+        elif tuner is not None:
+            self.digit_bits = tuner.tune(self.n, self.total_bits, self._launch)
+        else:
+            self.digit_bits = 8
+
+    def _launch(self, digit_bits: int | None = None, stream=None):
+        s = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        ptr = lambda a: _lib.ptr(self.outs.get(a))
+        _lib.call("jfx_batched_groupby", self.kc, _lib.ptr(self.keys), self.vcode, _lib.ptr(self.vals),
+                  self.n, _lib.ptr(self.offsets), self.nframes, self.kmin & ((1 << 64) - 1),
+                  self.key_bits, digit_bits or self.digit_bits, _lib.ptr(self.out_keys),
+                  ptr("sum"), ptr("count"), ptr("min"), ptr("max"), ptr("mean"),
+                  _lib.ptr(self.ngroups), _lib.ptr(self.ws), self.ws.numel(),
+                  ctypes.c_void_p(s.cuda_stream))
+
+    def capture(self):
+        """Record the pipeline in a HIP graph (replayed by run(graph=True))."""
+        side = torch.cuda.Stream(device=self.dev)
+        side.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(side):
+            self._launch(stream=side)
+        torch.cuda.current_stream(self.dev).wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._launch(stream=torch.cuda.current_stream(self.dev))
+        self.graph = g
+        return g
+
+    def launch(self, graph: bool = False):
+        """Enqueue one batch (graph replay or eager launches); no host sync."""
+        if graph:
+            if self.graph is None:
+                self.capture()
+            self.graph.replay()
+        else:
+            self._launch()
+
+    def result(self) -> BatchResult:
+        """Read the group count (one sync) and split the composite keys."""
+        G = int(self.ngroups.item())
+        comp = self.out_keys[:G]
+        c64 = (comp.long() & 0xFFFFFFFF) if self.comp_dtype == torch.int32 else comp
+        kmask = (1 << self.key_bits) - 1
+        if self.total_bits == 64:   # logical shift of the top field through the signed view
+            frame = (c64 >> self.key_bits) & ((1 << self.frame_bits) - 1)
+        else:
+            frame = c64 >> self.key_bits
+        keys = (c64 & kmask) + self.kmin
+        if self.korig != torch.int64:
+            keys = keys.to(self.korig)
+        counts = torch.bincount(frame, minlength=self.nframes)
+        offsets = torch.zeros(self.nframes + 1, dtype=torch.int64, device=self.dev)
+        offsets[1:] = torch.cumsum(counts, 0)
+        vals = {}
+        for a, o in self.outs.items():
+            v = o[:G]
+            if a in ("sum", "min", "max") and v.dtype != self.vorig:
+                v = v.to(self.vorig)
+            vals[a] = v
+        return BatchResult(offsets, keys, vals)
+
+    def run(self, graph: bool = False) -> BatchResult:
+        self.launch(graph)
+        return self.result()
+
+
+def plain_per_frame(keys, values, frame_offsets, agg: str = "sum", frames=None):
+    """The plain path the batched plan is measured against: one
+    groupby_aggregate call per frame (same kernels, one query at a time).
+    Returns [(unique keys, aggregate)] for the frames asked for."""
+    from .grouping import groupby_aggregate
+    offs = [int(x) for x in torch.as_tensor(frame_offsets).tolist()]
+    out = []
+    for f in (range(len(offs) - 1) if frames is None else frames):
+        a, b = offs[f], offs[f + 1]
+        uk, r = groupby_aggregate(keys[a:b], {"v": values[a:b]}, {"v": agg})
+        out.append((uk, r["v"]))
+    return out

@@ -24,6 +24,7 @@
 #include "sortctx.hpp"
 #include "tilereduce.hpp"
 #include "wavetable.hpp"
+#include "batched.hpp"
 
 namespace jfx {
 
@@ -773,6 +774,102 @@ int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, 
   rc = jfx::segreduce_dispatch(w, key_type, val_type, c, s);
   if (rc != JFX_OK) return rc;
   return jfx::read_ngroups(w, out_ngroups, s);
+}
+
+// ---- batched groupby ---------------------------------------------------------
+
+int jfx_batched_groupby_workspace_size(uint32_t total_bits, int val_type, uint64_t n,
+                                       uint64_t* out_bytes) {
+  JFX_CHECK(out_bytes != nullptr, JFX_ERR_INVALID, "out_bytes is null");
+  JFX_CHECK(jfx::valid_vt(val_type), JFX_ERR_INVALID, "bad value type");
+  JFX_CHECK(total_bits >= 1 && total_bits <= 64, JFX_ERR_INVALID, "total_bits must be 1..64");
+  const int cb = total_bits <= 32 ? 4 : 8, vb = jfx::key_bytes(val_type);
+  jfx::Carver c(nullptr, 0);
+  c.take<char>(n * cb);
+  c.take<char>(n * cb);
+  c.take<char>(n * vb);
+  jfx::carve_seg(c, vb, n);
+  c.take<char>(jfx::sort_ws_bytes(cb == 4 ? JFX_U32 : JFX_U64, vb, n));
+  *out_bytes = c.used();
+  return JFX_OK;
+}
+
+int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
+                        const int64_t* frame_offsets, uint32_t nframes, uint64_t key_min,
+                        uint32_t key_bits, int digit_bits, void* out_keys, void* out_sum,
+                        int64_t* out_count, void* out_min, void* out_max, void* out_mean,
+                        uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes, void* stream) {
+  JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64 && jfx::int_key(key_type), JFX_ERR_INVALID,
+            "batched keys must be integers");
+  JFX_CHECK(jfx::valid_vt(val_type), JFX_ERR_INVALID, "bad value type");
+  JFX_CHECK(n < (1ull << 30), JFX_ERR_UNSUPPORTED, "n must be below 2^30 rows");
+  JFX_CHECK(nframes >= 1, JFX_ERR_INVALID, "at least one frame");
+  uint32_t fbits = 0;
+  while ((1ull << fbits) < nframes) ++fbits;
+  const uint32_t total = fbits + key_bits;
+  JFX_CHECK(key_bits >= 1 && total <= 64, JFX_ERR_UNSUPPORTED,
+            "frame and key bits exceed 64 (narrow the key range)");
+  JFX_CHECK(out_ngroups != nullptr, JFX_ERR_INVALID, "out_ngroups is null");
+  JFX_CHECK(digit_bits == 0 || digit_bits == 8 || digit_bits == 11, JFX_ERR_INVALID,
+            "digit_bits must be 0, 8 or 11");
+  hipStream_t s = jfx::as_stream(stream);
+  if (n == 0) {
+    JFX_LAUNCH("copy_ngroups", s, (jfx::k_zero_u32<<<1, 64, 0, s>>>(out_ngroups)));
+    JFX_HIP(hipGetLastError());
+    return JFX_OK;
+  }
+  JFX_CHECK(keys && vals && frame_offsets && workspace, JFX_ERR_INVALID, "null input or workspace");
+  const int cb = total <= 32 ? 4 : 8, vb = jfx::key_bytes(val_type);
+  const int ckt = cb == 4 ? JFX_U32 : JFX_U64;
+  jfx::Carver cv(workspace, ws_bytes);
+  void* comp = cv.take<char>(n * cb);
+  void* scomp = cv.take<char>(n * cb);
+  void* svals = cv.take<char>(n * vb);
+  jfx::SegWs w = jfx::carve_seg(cv, vb, n);
+  const uint64_t sws = jfx::sort_ws_bytes(ckt, vb, n);
+  void* sort_ws = cv.take<char>(sws);
+  JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "batched groupby workspace too small");
+  uint32_t* err = jfx::device_error_word();
+  JFX_CHECK(err != nullptr, JFX_ERR_HIP, "cannot resolve the device error word");
+  const int oc = jfx::order_code(key_type);
+  // ordered image of the minimum (integer orders: flip the sign bit of signed keys)
+  uint64_t omin = jfx::key_bytes(key_type) == 4 ? (key_min & 0xFFFFFFFFull) : key_min;
+  if (oc == jfx::ORD_SIGNED) omin ^= jfx::key_bytes(key_type) == 4 ? 0x80000000ull : 0x8000000000000000ull;
+  const uint32_t blocks = (uint32_t)((n + jfx::BC_ROWS - 1) / jfx::BC_ROWS);
+  if (jfx::key_bytes(key_type) == 4) {
+    if (cb == 4) {
+      JFX_LAUNCH("batch_compose", s, (jfx::k_batch_compose<uint32_t, uint32_t><<<blocks, jfx::BC_BLOCK, 0, s>>>(
+                                         (const uint32_t*)keys, oc, omin, key_bits, frame_offsets,
+                                         nframes, n, (uint32_t*)comp, err)));
+    } else {
+      JFX_LAUNCH("batch_compose", s, (jfx::k_batch_compose<uint32_t, uint64_t><<<blocks, jfx::BC_BLOCK, 0, s>>>(
+                                         (const uint32_t*)keys, oc, omin, key_bits, frame_offsets,
+                                         nframes, n, (uint64_t*)comp, err)));
+    }
+  } else {
+    if (cb == 4) {
+      JFX_LAUNCH("batch_compose", s, (jfx::k_batch_compose<uint64_t, uint32_t><<<blocks, jfx::BC_BLOCK, 0, s>>>(
+                                         (const uint64_t*)keys, oc, omin, key_bits, frame_offsets,
+                                         nframes, n, (uint32_t*)comp, err)));
+    } else {
+      JFX_LAUNCH("batch_compose", s, (jfx::k_batch_compose<uint64_t, uint64_t><<<blocks, jfx::BC_BLOCK, 0, s>>>(
+                                         (const uint64_t*)keys, oc, omin, key_bits, frame_offsets,
+                                         nframes, n, (uint64_t*)comp, err)));
+    }
+  }
+  JFX_HIP(hipGetLastError());
+  jfx::SortCtx ctx;
+  int rc = jfx::sort_prepare_known(ctx, ckt, comp, n, vb, digit_bits, total, sort_ws, sws, s);
+  if (rc != JFX_OK) return rc;
+  rc = jfx::sort_execute(ctx, comp, scomp, 1, vals, svals, 0, 0);
+  if (rc != JFX_OK) return rc;
+  jfx::SegCall c{scomp, jfx::ORD_UNSIGNED, svals, nullptr, n, out_keys, out_sum, out_count,
+                 out_min, out_max, out_mean, false, 0u};
+  rc = jfx::segreduce_dispatch(w, ckt, val_type, c, s);
+  if (rc != JFX_OK) return rc;
+  JFX_LAUNCH("copy_ngroups", s, (jfx::k_copy_u32<<<1, 64, 0, s>>>(w.ngroups, out_ngroups)));
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
 }
 
 }  // extern "C"

@@ -217,11 +217,24 @@ k_sort_hist(const K* __restrict__ keys, uint64_t n, int oc, uint32_t vary, uint3
   const uint64_t nvec = ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0) ? n / VEC : 0;
   const uint64_t stride = (uint64_t)gridDim.x * HIST_BLOCK;
   const uint4* kv = reinterpret_cast<const uint4*>(keys);
+  const uint32_t lane = lane_id();
+  // A digit on which the whole wave agrees (high digits of clustered or
+  // frame-major keys) is added once with the wave's count instead of 64
+  // colliding LDS atomics.
   auto one = [&](K raw) {
     const K o = ordered(raw, oc);
+    const uint64_t act = ballot64(true);
 #pragma unroll
-    for (int p = 0; p < NP; ++p)
-      if ((vary >> p) & 1u) atomicAdd(&s_h[p * RADIX + (uint32_t)((o >> (p * D)) & (K)(RADIX - 1))], 1u);
+    for (int p = 0; p < NP; ++p) {
+      if (!((vary >> p) & 1u)) continue;
+      const uint32_t bin = (uint32_t)((o >> (p * D)) & (K)(RADIX - 1));
+      const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
+      if (ballot64(bin == b0) == act) {
+        if (lane == (uint32_t)__builtin_ctzll(act)) atomicAdd(&s_h[p * RADIX + b0], (uint32_t)__popcll(act));
+      } else {
+        atomicAdd(&s_h[p * RADIX + bin], 1u);
+      }
+    }
   };
   uint64_t i = (uint64_t)blockIdx.x * HIST_BLOCK + threadIdx.x;
   for (; i + 3 * stride < nvec; i += 4 * stride) {
@@ -317,13 +330,12 @@ struct Passes {
                          const uint32_t* colex, const uint32_t* total, RadixDigit<K> dig,
                          hipStream_t s) {
     switch (chunk_cfg()) {
-      case 1: chunk_launch<D, 8, 8, false, 4>(name, a, g, colex, total, dig, s); break;
+      case 1: chunk_launch<D, 8, 16, false, 1>(name, a, g, colex, total, dig, s); break;
       case 2: chunk_launch<D, 8, 8, true, 4>(name, a, g, colex, total, dig, s); break;
       case 3: chunk_launch<D, 4, 16, false, 1>(name, a, g, colex, total, dig, s); break;
       case 4: chunk_launch<D, 4, 16, true, 1>(name, a, g, colex, total, dig, s); break;
       default:
-        if constexpr (sizeof(K) == 4) chunk_launch<D, 8, 16, false, 1>(name, a, g, colex, total, dig, s);
-        else chunk_launch<D, 8, 8, false, 4>(name, a, g, colex, total, dig, s);
+        chunk_launch<D, 8, 8, false, 4>(name, a, g, colex, total, dig, s);
     }
   }
   static hipError_t chunked(int D, const ScatterArgs<K, PT>& a, const ChunkGeom& g,
@@ -511,6 +523,39 @@ int sort_prepare(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_by
   c.band = h[2];
   c.bor = h[3];
   if (c.kb == 4) c.band &= 0xFFFFFFFFull;
+  return JFX_OK;
+}
+
+int sort_prepare_known(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_bytes,
+                       int digit_bits, uint32_t key_bits, void* ws, uint64_t ws_bytes,
+                       hipStream_t s) {
+  JFX_CHECK(kt == JFX_U32 || kt == JFX_U64, JFX_ERR_INVALID, "known-range sorts take unsigned keys");
+  JFX_CHECK(key_bits >= 1 && key_bits <= (uint32_t)key_bytes(kt) * 8, JFX_ERR_INVALID, "bad key_bits");
+  c.kt = kt;
+  c.kb = key_bytes(kt);
+  c.oc = order_code(kt);
+  c.pb = pay_bytes;
+  c.n = n;
+  c.s = s;
+  c.digit_bits = digit_bits ? digit_bits : default_digit_bits(kt, n);
+  JFX_CHECK(c.digit_bits == 8 || c.digit_bits == 11, JFX_ERR_INVALID, "digit_bits must be 8 or 11");
+  Carver cv(ws, ws_bytes);
+  carve(c, cv);
+  JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "sort workspace too small");
+  geometry(c);
+  c.err = device_error_word();
+  JFX_CHECK(c.err != nullptr, JFX_ERR_HIP, "cannot resolve the device error word");
+  JFX_LAUNCH("sort_init", s, (k_sort_init<<<64, 256, 0, s>>>(c.stats, c.counters, 64, c.hist,
+                                                             SORT_MAXP * SORT_MAXR)));
+  JFX_HIP(hipGetLastError());
+  // the chunk histograms of the first pass still come from the stats kernel
+  int rc = c.kb == 4 ? stats_launch<uint32_t>(c, keys_in) : stats_launch<uint64_t>(c, keys_in);
+  if (rc != JFX_OK) return rc;
+  const uint64_t top = key_bits >= 64 ? ~0ull : ((1ull << key_bits) - 1ull);
+  c.mn = 0;
+  c.mx = top;
+  c.band = 0;
+  c.bor = top;
   return JFX_OK;
 }
 

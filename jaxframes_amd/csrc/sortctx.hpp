@@ -54,6 +54,11 @@ int sort_prepare(SortCtx& ctx, int kt, const void* keys_in, uint64_t n, int pay_
 
 // Run all scatter passes (single chunked pass when max-min < 2048, else the
 // layout passes with constant digits skipped).
+// Like sort_prepare for unsigned keys known to lie in [0, 2^key_bits): no
+// host read, every digit below key_bits is sorted (graph-capturable).
+int sort_prepare_known(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_bytes,
+                       int digit_bits, uint32_t key_bits, void* ws, uint64_t ws_bytes,
+                       hipStream_t s);
 int sort_execute(SortCtx& ctx, const void* keys_in, void* keys_out, int want_keys,
                  const void* pay_in, void* pay_out, int iota, int descending);
 
