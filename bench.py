@@ -210,8 +210,15 @@ def main() -> int:
                 "alg_bytes_per_launch": dbytes}
     kernels = {k: {"us_per_step": round(v[0] / args.steps * 1e3, 1),
                    "launches_per_step": v[1] // args.steps} for k, v in kstats.items()}
+    # both small-range kernels are launched and the unselected one returns at
+    # once: count the bytes of the one that ran (the longer)
+    ran = dict(kstats)
+    alts = [k for k in ran if k.startswith("wave_table") or k.startswith("tile_reduce")]
+    for k in alts:
+        if k != max(alts, key=lambda a: ran[a][0]):
+            ran.pop(k)
     step_bytes = sum((_kernel_bytes(k, n, 4, 4) or 0) * (v[1] // args.steps)
-                     for k, v in kstats.items())
+                     for k, v in ran.items())
 
     batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows)
 
