@@ -208,6 +208,24 @@ int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void
                         int64_t* out_count, void* out_min, void* out_max, void* out_mean,
                         uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes, void* stream);
 
+/* ---- indexed filter -> groupby (serving) ------------------------------- */
+/* Per group of a frame indexed by (key, value) -- values ascending inside each
+ * group by their order-preserving image, NaN last (nan_start[g] = first NaN
+ * row of group g; ninf_end[g] = end of its leading -inf rows; pinf_start[g] =
+ * its first +inf row) -- and prefix[] = float64 prefix sums of those values
+ * (NaN and +-inf as 0, n + 1 entries): count, sum and mean of the group's rows with
+ * value <cmp> *value (JFX_CMP_*; IEEE semantics, a NaN value passes only !=
+ * and makes its group's sum NaN), found by binary searches and prefix-sum
+ * differences; out_mask[g] = 1 where the count is nonzero. The result of
+ * filtering rows and then SortBasedGroupBy.groupby_aggregate
+ * (parallel_algorithms.py:733-803), restricted to the groups that keep rows.
+ * val_type: JFX_F32 or JFX_F64. No counterpart in the reference. */
+int jfx_served_groups(int val_type, const void* vals, const double* prefix, const uint32_t* starts,
+                      const uint32_t* nan_start, const uint32_t* ninf_end,
+                      const uint32_t* pinf_start, uint32_t ngroups, int cmp, const void* value,
+                      int64_t* out_count, void* out_sum, void* out_mean, uint8_t* out_mask,
+                      void* stream);
+
 /* ---- composite keys (multi-column sort / groupby / join) -------------- */
 /* Packs 1..8 integer key columns (JFX_U32/I32/U64/I64, n rows each) into one
  * uint64 key per row whose order and equality are the lexicographic ones of

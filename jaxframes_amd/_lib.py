@@ -66,6 +66,8 @@ _SIGNATURES = {
     "jfx_batched_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, ctypes.c_uint32, c_u64,
                             ctypes.c_uint32, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_u64, c_vp],
+    "jfx_served_groups": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint32, c_int, c_vp,
+                          c_vp, c_vp, c_vp, c_vp, c_vp],
     "jfx_compact_workspace_size": [c_u64, c_u64p],
     "jfx_compact_rows": [c_vp, c_u64, c_vp, c_u64p, c_vp, c_u64, c_vp],
     "jfx_filter_rows": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_u64p, c_vp, c_u64, c_vp],

@@ -18,8 +18,8 @@ import torch
 from . import _lib
 from ._tensors import to_device
 
-_OPS = {">": _lib.CMP_CODES[">"], ">=": _lib.CMP_CODES[">="], "<": _lib.CMP_CODES["<"],
-        "<=": _lib.CMP_CODES["<="], "==": _lib.CMP_CODES["=="], "!=": _lib.CMP_CODES["!="]}
+OPS = {">": _lib.CMP_CODES[">"], ">=": _lib.CMP_CODES[">="], "<": _lib.CMP_CODES["<"],
+       "<=": _lib.CMP_CODES["<="], "==": _lib.CMP_CODES["=="], "!=": _lib.CMP_CODES["!="]}
 _COL_CODE = {torch.uint32: (_lib.JFX_U32, ctypes.c_uint32), torch.int32: (_lib.JFX_I32, ctypes.c_int32),
              torch.float32: (_lib.JFX_F32, ctypes.c_float), torch.uint64: (_lib.JFX_U64, ctypes.c_uint64),
              torch.int64: (_lib.JFX_I64, ctypes.c_int64), torch.float64: (_lib.JFX_F64, ctypes.c_double)}
@@ -38,7 +38,7 @@ def _stream(dev):
 def filter_rows(column, op: str, value) -> torch.Tensor:
     """int64 positions (ascending) of the rows where `column op value`."""
     _lib.require_gpu()
-    if op not in _OPS:
+    if op not in OPS:
         raise ValueError(f"unsupported comparison {op!r}")
     col = to_device(column)
     if col.dtype not in _COL_CODE:
@@ -50,7 +50,7 @@ def filter_rows(column, op: str, value) -> torch.Tensor:
     cnt = ctypes.c_uint64(0)
     val = ctype(value)
     ws = _ws(dev)
-    _lib.call("jfx_filter_rows", code, _lib.ptr(col), _OPS[op], ctypes.byref(val), n, _lib.ptr(out),
+    _lib.call("jfx_filter_rows", code, _lib.ptr(col), OPS[op], ctypes.byref(val), n, _lib.ptr(out),
               ctypes.byref(cnt), _lib.ptr(ws), ws.numel(), _stream(dev))
     return out[:cnt.value].long()
 

@@ -34,20 +34,62 @@ from ._tensors import key_code, to_device, widen_key
 # ---- resident frames ---------------------------------------------------------------
 
 class Frame:
-    """A resident frame (int keys, value column) and its key order."""
+    """A resident frame (int keys, value column) and its indexes:
+
+    - key order (stable): sorted keys and the row permutation, reused by joins
+      (pairs then come in the reference's (key, left row, right row) order);
+    - (key, value) order for float values: the values ascending inside each
+      key group (NaN last), float64 prefix sums of them, group starts, the
+      first NaN row of each group and the group keys, which answer
+      filter -> groupby by binary search (jfx_served_groups).
+    """
 
     def __init__(self, name: str, keys, values):
+        from .multikey import multikey_heads
         from .sorting import radix_sort
         self.name = name
         self.keys, _ = widen_key(to_device(keys))
         self.values = to_device(values, self.keys.device)
         self.n = self.keys.shape[0]
+        dev = self.keys.device
         sk, _, perm = radix_sort(self.keys, None, True, return_index=True, index_dtype=torch.int32)
         self.sorted_keys = sk
         self.perm = perm                                  # uint32 row index (int32 storage)
         self.sorted_values = self.values[perm.long()] if self.n else self.values
-        self.nbytes = sum(t.numel() * t.element_size() for t in
-                          (self.keys, self.values, self.sorted_keys, self.perm, self.sorted_values))
+        # identity rows: a join fill then reports sorted positions, and the left
+        # payload comes from sorted_values in (nearly) sequential order
+        self.iota = torch.arange(self.n, dtype=torch.int32, device=dev)
+        parts = [self.keys, self.values, self.sorted_keys, self.perm, self.sorted_values, self.iota]
+        self.indexed = self.values.dtype in (torch.float32, torch.float64) and self.n > 0
+        if self.indexed:
+            # (key, value) order: stable by value, then stable by key
+            _, _, pv = radix_sort(self.values, None, True, want_keys=False, return_index=True)
+            _, _, p2 = radix_sort(self.keys[pv], None, True, want_keys=False, return_index=True)
+            kv = pv[p2]
+            self.kv_values = self.values[kv]
+            gid, starts = multikey_heads([sk])
+            G = starts.shape[0]
+            self.group_keys = sk[starts]
+            self.group_starts = torch.empty(G + 1, dtype=torch.int32, device=dev)
+            self.group_starts[:G] = starts.to(torch.int32)
+            self.group_starts[G] = self.n
+            nan = torch.isnan(self.kv_values)
+            nn = torch.bincount(gid[nan], minlength=G)
+            npos = torch.bincount(gid[torch.isposinf(self.kv_values)], minlength=G)
+            nneg = torch.bincount(gid[torch.isneginf(self.kv_values)], minlength=G)
+            ends = self.group_starts[1:].long()
+            self.group_nan_start = (ends - nn).to(torch.int32)
+            self.group_pinf_start = (ends - nn - npos).to(torch.int32)
+            self.group_ninf_end = (self.group_starts[:G].long() + nneg).to(torch.int32)
+            # finite values only: infinities and NaN are counted per group
+            fin = torch.where(torch.isfinite(self.kv_values), self.kv_values,
+                              torch.zeros_like(self.kv_values))
+            self.prefix = torch.zeros(self.n + 1, dtype=torch.float64, device=dev)
+            self.prefix[1:] = torch.cumsum(fin.double(), 0)
+            self.ngroups = G
+            parts += [self.kv_values, self.group_keys, self.group_starts, self.group_nan_start,
+                      self.group_pinf_start, self.group_ninf_end, self.prefix]
+        self.nbytes = sum(t.numel() * t.element_size() for t in parts)
 
 
 class FrameStore:
@@ -105,6 +147,9 @@ def footprint(store: FrameStore, q) -> int:
     fr = store[q.frame]
     kb, vb = fr.keys.element_size(), fr.values.element_size()
     if isinstance(q, FilterGroupBy):
+        if fr.indexed:
+            # per-group count / sum / mean / mask, positions, the gathered groups
+            return fr.ngroups * (8 + 2 * vb + 1 + 4 + kb + 8 + 2 * vb) + (1 << 20)
         # positions, gathered keys + values, group outputs, segreduce scratch
         return fr.n * (4 + kb + vb) + fr.n * (kb + 3 * 8) + fr.n * 16 + (1 << 22)
     nd = q.dim_keys.shape[0]
@@ -137,12 +182,36 @@ def _segreduce(sk, sv, aggs):
     return out_keys[:G].clone(), {fn: o[:G].clone() for fn, o in outs.items()}
 
 
+_INDEXED_AGGS = {"sum", "count", "mean"}
+
+
 def run_filter_groupby(store: FrameStore, q: FilterGroupBy):
-    """filter -> groupby on the frame's key order: the rows passing the filter
-    are already grouped, so a segmented reduction over them finishes the
-    query (no sort)."""
-    from .compaction import filter_rows, take
+    """filter -> groupby. On an indexed frame: one binary-search pass over the
+    groups (jfx_served_groups) and a compaction of the groups that keep rows.
+    Otherwise on the frame's key order: the rows passing the filter are
+    already grouped, so a segmented reduction over them finishes the query
+    (no sort)."""
+    from .compaction import OPS, compact_rows, filter_rows, take
     fr = store[q.frame]
+    if fr.indexed and set(q.aggs) <= _INDEXED_AGGS:
+        if q.op not in OPS:
+            raise ValueError(f"unsupported comparison {q.op!r}")
+        dev = fr.keys.device
+        G = fr.ngroups
+        vt = _lib.JFX_F32 if fr.kv_values.dtype == torch.float32 else _lib.JFX_F64
+        cnt = torch.empty(G, dtype=torch.int64, device=dev)
+        sm = torch.empty(G, dtype=fr.kv_values.dtype, device=dev)
+        mean = torch.empty(G, dtype=fr.kv_values.dtype, device=dev)
+        mask = torch.empty(G, dtype=torch.uint8, device=dev)
+        val = (ctypes.c_float if vt == _lib.JFX_F32 else ctypes.c_double)(q.threshold)
+        _lib.call("jfx_served_groups", vt, _lib.ptr(fr.kv_values), _lib.ptr(fr.prefix),
+                  _lib.ptr(fr.group_starts), _lib.ptr(fr.group_nan_start),
+                  _lib.ptr(fr.group_ninf_end), _lib.ptr(fr.group_pinf_start), G, OPS[q.op],
+                  ctypes.byref(val), _lib.ptr(cnt), _lib.ptr(sm), _lib.ptr(mean), _lib.ptr(mask),
+                  ctypes.c_void_p(_lib.stream_handle(dev)))
+        pos = compact_rows(mask)
+        full = {"sum": sm, "count": cnt, "mean": mean}
+        return take(fr.group_keys, pos), {a: take(full[a], pos) for a in q.aggs}
     pos = filter_rows(fr.sorted_values, q.op, q.threshold)
     sk = take(fr.sorted_keys, pos)
     sv = take(fr.sorted_values, pos)
@@ -176,10 +245,70 @@ def run_join(store: FrameStore, q: Join):
     ri = torch.empty(M, dtype=torch.int32, device=dev)
     keys = torch.empty(M, dtype=fr.keys.dtype, device=dev)
     if M:
-        _lib.call("jfx_join_fill", kc, _lib.ptr(fr.sorted_keys), _lib.ptr(fr.perm), L, _lib.ptr(sd),
+        # left rows as sorted positions (iota): pairs keep the reference's
+        # (key, left row, right row) order because the key order is stable
+        _lib.call("jfx_join_fill", kc, _lib.ptr(fr.sorted_keys), _lib.ptr(fr.iota), L, _lib.ptr(sd),
                   _lib.ptr(dperm), R, _lib.ptr(li), _lib.ptr(ri), _lib.ptr(keys), _lib.ptr(ws),
                   ws.numel(), stream)
-    return keys, {"left_value": gather(fr.values, li), "right_value": gather(to_device(q.dim_values, dev), ri)}
+    return keys, {"left_value": gather(fr.sorted_values, li),
+                  "right_value": gather(to_device(q.dim_values, dev), ri)}
+
+
+def execute_filter_batch(store: FrameStore, qs: list) -> list:
+    """Several indexed filter -> groupby queries as one batch: per-query
+    group passes into slices of shared buffers, ONE compaction of all the
+    groups that keep rows and one gather per output column, so the batch
+    costs a fixed number of host synchronisations however many queries it
+    holds. Results equal run_filter_groupby's query by query."""
+    from .compaction import OPS, compact_rows, take
+    frs = [store[q.frame] for q in qs]
+    kinds = {(fr.group_keys.dtype, fr.kv_values.dtype) for fr in frs}
+    if len(kinds) > 1:   # one sub-batch per (key type, value type)
+        out = [None] * len(qs)
+        for kind in kinds:
+            idx = [i for i, fr in enumerate(frs) if (fr.group_keys.dtype, fr.kv_values.dtype) == kind]
+            for i, r in zip(idx, execute_filter_batch(store, [qs[i] for i in idx])):
+                out[i] = r
+        return out
+    dev = frs[0].keys.device
+    vdt = frs[0].kv_values.dtype
+    Gs = [fr.ngroups for fr in frs]
+    offs = [0]
+    for G in Gs:
+        offs.append(offs[-1] + G)
+    tot = offs[-1]
+    cnt = torch.empty(tot, dtype=torch.int64, device=dev)
+    sm = torch.empty(tot, dtype=vdt, device=dev)
+    mean = torch.empty(tot, dtype=vdt, device=dev)
+    mask = torch.empty(tot, dtype=torch.uint8, device=dev)
+    vt = _lib.JFX_F32 if vdt == torch.float32 else _lib.JFX_F64
+    ctype = ctypes.c_float if vt == _lib.JFX_F32 else ctypes.c_double
+    stream = ctypes.c_void_p(_lib.stream_handle(dev))
+    for q, fr, o, G in zip(qs, frs, offs, Gs):
+        if q.op not in OPS:
+            raise ValueError(f"unsupported comparison {q.op!r}")
+        val = ctype(q.threshold)
+        _lib.call("jfx_served_groups", vt, _lib.ptr(fr.kv_values), _lib.ptr(fr.prefix),
+                  _lib.ptr(fr.group_starts), _lib.ptr(fr.group_nan_start),
+                  _lib.ptr(fr.group_ninf_end), _lib.ptr(fr.group_pinf_start), G, OPS[q.op],
+                  ctypes.byref(val), _lib.ptr(cnt[o:]), _lib.ptr(sm[o:]), _lib.ptr(mean[o:]),
+                  _lib.ptr(mask[o:]), stream)
+    pos = compact_rows(mask)
+    keys_all = torch.cat([fr.group_keys for fr in frs])
+    cuts = torch.searchsorted(pos, torch.tensor(offs, dtype=torch.int64, device=dev)).tolist()
+    k, c, s_, m = take(keys_all, pos), take(cnt, pos), take(sm, pos), take(mean, pos)
+    full = {"sum": s_, "count": c, "mean": m}
+    out = []
+    for i, q in enumerate(qs):
+        a, b = cuts[i], cuts[i + 1]
+        out.append((k[a:b], {agg: full[agg][a:b] for agg in q.aggs}))
+    return out
+
+
+def batchable(store: FrameStore, q) -> bool:
+    """Queries execute_filter_batch can take."""
+    return (isinstance(q, FilterGroupBy) and store[q.frame].indexed
+            and set(q.aggs) <= _INDEXED_AGGS)
 
 
 def execute(store: FrameStore, q):
@@ -222,11 +351,16 @@ class Server:
     """
 
     def __init__(self, store: FrameStore, slots: int = 4, hbm_budget_bytes: int | None = None,
-                 executor=execute, use_streams: bool = True, footprint_fn=footprint):
+                 executor=execute, use_streams: bool = True, footprint_fn=footprint,
+                 batch_executor=execute_filter_batch, batchable_fn=batchable, max_batch: int = 32):
         self.store = store
         self.slots = slots
         self.budget = int(hbm_budget_bytes if hbm_budget_bytes is not None else store.budget)
         self.executor = executor
+        self.batch_executor = batch_executor
+        self.batchable_fn = batchable_fn
+        self.max_batch = max_batch
+        self.batches = []   # sizes of the batches run
         self.footprint_fn = footprint_fn
         self.cv = threading.Condition()
         self.pending: deque[Request] = deque()
@@ -271,6 +405,27 @@ class Server:
         self.admitted_order.append(head.rid)
         return head
 
+    def _take_batch(self, head: Request) -> list:
+        # called with the lock held after head was admitted: the batchable
+        # requests queued right behind it join its batch while they fit
+        batch = [head]
+        if self.batch_executor is None or not self.batchable_fn(self.store, head.query):
+            return batch
+        while (self.pending and len(batch) < self.max_batch
+               and self.batchable_fn(self.store, self.pending[0].query)
+               and self.reserved + self.pending[0].footprint <= self.budget):
+            r = self.pending.popleft()
+            self.reserved += r.footprint
+            self.peak_reserved = max(self.peak_reserved, self.reserved)
+            self.admitted_order.append(r.rid)
+            batch.append(r)
+        return batch
+
+    def _run(self, batch: list):
+        if len(batch) == 1:
+            return [self.executor(self.store, batch[0].query)]
+        return self.batch_executor(self.store, [r.query for r in batch])
+
     def _worker(self, i: int, use_streams: bool):
         stream = torch.cuda.Stream() if use_streams else None
         while True:
@@ -281,20 +436,26 @@ class Server:
                         return
                     self.cv.wait()
                     req = self._admit()
+                batch = self._take_batch(req)
+                self.batches.append(len(batch))
             try:
                 if stream is not None:
                     with torch.cuda.stream(stream):
-                        res = self.executor(self.store, req.query)
+                        results = self._run(batch)
                     stream.synchronize()
                 else:
-                    res = self.executor(self.store, req.query)
-                req.future.set_result((res, time.perf_counter() - req.arrival))
-            except BaseException as e:  # delivered to the caller through the future
-                req.future.set_exception(e)
+                    results = self._run(batch)
+                done = time.perf_counter()
+                for r, res in zip(batch, results):
+                    r.future.set_result((res, done - r.arrival))
+            except BaseException as e:  # delivered to the callers through the futures
+                for r in batch:
+                    if not r.future.done():
+                        r.future.set_exception(e)
             finally:
                 with self.cv:
                     self.running -= 1
-                    self.reserved -= req.footprint
+                    self.reserved -= sum(r.footprint for r in batch)
                     self.cv.notify_all()
 
     def close(self):

@@ -111,3 +111,68 @@ def test_budget_admission_on_gpu(store):
     assert srv.peak_running == 1
     for r in res[1:]:
         _same(res[0], r, exact_sums=True)
+
+
+@pytest.mark.parametrize("op", [">", ">=", "<", "<=", "==", "!="])
+def test_indexed_filter_ops_nan_and_signed_zero(op):
+    rng = np.random.default_rng(7)
+    n = 50000
+    k = rng.integers(0, 40, n).astype(np.int64)
+    pool = np.array([np.nan, -0.0, 0.0, 1.5, -2.0, np.inf, 0.25], dtype=np.float32)
+    v = np.where(rng.random(n) < 0.5, rng.choice(pool, n), rng.standard_normal(n).astype(np.float32))
+    v = v.astype(np.float32)
+    st = serving.FrameStore(1 << 30)
+    st.load("z", cuda(k), cuda(v))
+    assert st["z"].indexed
+    cmp = {">": np.greater, ">=": np.greater_equal, "<": np.less, "<=": np.less_equal,
+           "==": np.equal, "!=": np.not_equal}[op]
+    for t in (0.0, -0.0, 1.5, 0.25, float("nan")):
+        keys, res = serving.execute(st, serving.FilterGroupBy("z", op, t))
+        m = cmp(v, np.float32(t))
+        ek, ec = oracle.groupby_aggregate(k[m], {"v": v[m]}, {"v": "count"})
+        _, es = oracle.groupby_aggregate(k[m], {"v": v[m]}, {"v": "sum"})
+        assert np.array_equal(keys.cpu().numpy(), ek), (op, t)
+        assert np.array_equal(res["count"].cpu().numpy(), ec["v"]), (op, t)
+        g = res["sum"].cpu().numpy().astype(np.float64)
+        e = es["v"]
+        assert np.array_equal(np.isnan(g), np.isnan(e)), (op, t)
+        assert np.array_equal(g[np.isinf(e)], e[np.isinf(e)]), (op, t)   # +-inf sums exact
+        fin = np.isfinite(e)
+        assert np.all(np.isfinite(g[fin])), (op, t)
+        _, ea = oracle.groupby_aggregate(k[m], {"v": np.abs(v[m]).astype(np.float64)}, {"v": "sum"})
+        assert np.all(np.abs(g[fin] - e[fin]) <= SUM_NORM_TOL * np.maximum(ea["v"][fin], 1e-30)), (op, t)
+
+
+def test_batched_filter_queries_equal_one_at_a_time(store):
+    rng = np.random.default_rng(9)
+    st = serving.FrameStore(1 << 30)
+    st.frames.update(store.frames)
+    k = rng.integers(0, 500, 100_000).astype(np.int64)
+    st.load("d64", cuda(k), cuda(rng.standard_normal(k.size)))          # float64 values
+    ops = [">", ">=", "<", "<=", "==", "!="]
+    qs = [serving.FilterGroupBy(f"f{i % 3}" if i % 4 else "d64", ops[i % 6],
+                                float(rng.uniform(-2, 2)) if i % 5 else 10.0)   # 10.0: nothing passes >
+          for i in range(23)]
+    batch = serving.execute_filter_batch(st, qs)
+    for q, b in zip(qs, batch):
+        _same(serving.execute(st, q), b, exact_sums=True)
+
+
+def test_server_batches_queued_filter_queries(store):
+    qs = [serving.FilterGroupBy(f"f{i % 3}", ">", 0.1 * (i % 7)) for i in range(30)]
+    alone = [serving.execute(store, q) for q in qs]
+    srv = serving.Server(store, slots=1)
+    with srv.cv:                         # queue everything before the worker can admit
+        futs = [None] * len(qs)
+        for i, q in enumerate(qs):
+            srv._rid += 1
+            r = serving.Request(q, 0.0, rid=srv._rid, footprint=serving.footprint(store, q))
+            srv.pending.append(r)
+            futs[i] = r.future
+        srv.cv.notify_all()
+    res = [f.result()[0] for f in futs]
+    srv.close()
+    assert max(srv.batches) > 1
+    assert srv.admitted_order == sorted(srv.admitted_order)
+    for a, r in zip(alone, res):
+        _same(a, r, exact_sums=True)
