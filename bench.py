@@ -10,6 +10,10 @@ it: the key statistics pass (min / max and the per-digit row counts), then
 every row into its group's compensated sum, the in-order merge of the block
 partials, and the read-back of the group count.
 
+Beside the step: the batched groupby (4096 frames in one HIP-graph replay)
+and the served stream (request loop over resident frames, Poisson arrivals)
+against their plain per-query paths, and the 2e8-key hash partition.
+
 Prints one JSON line with rows/s, per-step p50/p99, the roofline of the
 dominant kernel (HIP-event durations measured inside the timed region on the
 launch stream), and a CPU baseline (the numpy restatement timed on a bounded
@@ -126,6 +130,71 @@ def batched_vs_plain(total_rows: int, reps: int = 5, plain_frames: int = 256):
     return out
 
 
+def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
+                    budget_bytes: int = 64 << 30):
+    """The served stream: 8 resident frames (1M..50M rows, int64 keys of
+    cardinality 10..1M, float32 values), 70% filter -> groupby(sum, count,
+    mean) and 30% inner joins with 1e5..1e6-row dimension tables, Poisson
+    arrivals at each rate in qps_levels through the request loop (slots
+    workers, HBM budget), against the plain path (the same queries one at a
+    time with no resident index and no batching). Also the hash partition of
+    2e8 uniform int64 keys into 256 buckets."""
+    import torch
+    from jaxframes_amd import datagen, partition, serving
+    frames = datagen.served_frames()
+    store = serving.FrameStore(budget_bytes)
+    for name, k, v, _ in frames:
+        store.load(name, k, v)
+    qs = datagen.served_queries(frames, nq)
+    rows = sum(store[q.frame].n for q in qs)
+    warm = qs[:8]
+    for q in warm:
+        serving.run_plain(store, q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for q in qs:
+        serving.run_plain(store, q)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    levels = []
+    for qps in qps_levels:
+        srv = serving.Server(store, slots=slots)
+        srv.warmup(warm)
+        arr = serving.poisson_arrivals(len(qs), qps, seed=3)
+        t0 = time.perf_counter()
+        res = serving.replay(srv, qs, arr)
+        wall = time.perf_counter() - t0
+        srv.close()
+        lat = np.array([l for _, l in res]) * 1e3
+        levels.append({"offered_qps": qps, "wall_s": round(wall, 4),
+                       "rows_per_s": round(rows / wall, 1), "qps": round(len(qs) / wall, 1),
+                       "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                       "peak_running": srv.peak_running,
+                       "mean_batch": round(float(np.mean(srv.batches)), 2),
+                       "speedup_vs_plain": round(plain_s / wall, 2)})
+    pk = datagen.partition_keys(200_000_000)
+    partition.hash_partition(pk, 256, return_keys=True, return_rows=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        partition.hash_partition(pk, 256, return_keys=True, return_rows=False)
+    torch.cuda.synchronize()
+    pdt = (time.perf_counter() - t0) / 5
+    del pk, store
+    torch.cuda.empty_cache()
+    return {"workload": "8 resident frames (rows log-uniform [1e6, 5e7], int64 keys, "
+                        "cardinality log-uniform [10, 1e6], float32 values); 70% filter(v > t) "
+                        "-> groupby(k).agg(sum, count, mean), 30% inner join with a "
+                        "1e5..1e6-row dimension table; Poisson arrivals",
+            "queries": len(qs), "rows": rows, "slots": slots, "hbm_budget_bytes": budget_bytes,
+            "plain_rows_per_s": round(rows / plain_s, 1),
+            "plain_qps": round(len(qs) / plain_s, 1), "levels": levels,
+            "hash_partition": {"workload": "2e8 uniform int64 keys -> 256 buckets (fmix32)",
+                               "ms": round(pdt * 1e3, 3),
+                               "keys_per_s": round(2e8 / pdt, 1)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +205,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-rows", type=int, default=400_000_000)
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 400 MB (float32[1e8]) so that PMC byte "
                          "counters can be scaled against a transfer of known size")
@@ -221,6 +291,7 @@ def main() -> int:
                      for k, v in ran.items())
 
     batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows)
+    served_line = None if args.no_served else served_vs_plain()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -254,6 +325,7 @@ def main() -> int:
         "step_hbm_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / PEAK_HBM_GBPS, 4),
         "roofline": roof,
         "batched": batched_line,
+        "served": served_line,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

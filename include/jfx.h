@@ -173,6 +173,22 @@ int jfx_join_fill(int key_type, const void* sorted_left, const uint32_t* left_ro
                   uint32_t* out_left_rows, uint32_t* out_right_rows, void* out_keys,
                   void* workspace, uint64_t ws_bytes, void* stream);
 
+/* jfx_join_fill with one payload column per side written by the fill: the
+ * value gather of parallel_algorithms.py:1052-1073 fused into the pair
+ * enumeration. out_left_values[k] = left_values[left_rows[i]] when
+ * left_by_row, else left_values[i] (i the pair's sorted-left position: a
+ * payload already in key order is read nearly sequentially); the right side
+ * likewise. Payload elements are 4 or 8 bytes. Every output is optional;
+ * left_rows / right_rows may be null when no output needs source rows (a
+ * null left_rows reports sorted positions in out_left_rows). */
+int jfx_join_fill_values(int key_type, const void* sorted_left, const uint32_t* left_rows,
+                         uint64_t n_left, const void* sorted_right, const uint32_t* right_rows,
+                         uint64_t n_right, int left_elem_bytes, const void* left_values,
+                         int left_by_row, int right_elem_bytes, const void* right_values,
+                         int right_by_row, uint32_t* out_left_rows, uint32_t* out_right_rows,
+                         void* out_keys, void* out_left_values, void* out_right_values,
+                         void* workspace, uint64_t ws_bytes, void* stream);
+
 /* For each sorted probe key, build_rows[first index of an equal key in
  * sorted_build], or 0xFFFFFFFF when none: the searchsorted + equality test
  * of the left / right / outer branches (parallel_algorithms.py:1014-1047). */
@@ -225,6 +241,38 @@ int jfx_served_groups(int val_type, const void* vals, const double* prefix, cons
                       const uint32_t* pinf_start, uint32_t ngroups, int cmp, const void* value,
                       int64_t* out_count, void* out_sum, void* out_mean, uint8_t* out_mask,
                       void* stream);
+
+/* One indexed filter -> groupby query of a batch: the index columns of
+ * jfx_served_groups, the frame's group keys (in group order) and the
+ * comparison. threshold is rounded to the value type. */
+typedef struct jfx_served_query {
+  const void* values;
+  const double* prefix;
+  const uint32_t* group_starts;
+  const uint32_t* nan_start;
+  const uint32_t* ninf_end;
+  const uint32_t* pinf_start;
+  const void* group_keys;
+  uint32_t ngroups;
+  int32_t cmp;
+  double threshold;
+} jfx_served_query;
+
+int jfx_served_batch_workspace_size(uint32_t nqueries, uint64_t total_groups, uint64_t* out_bytes);
+
+/* nqueries (<= 1024) queries of jfx_served_groups in one launch, with the
+ * groups that keep rows compacted: query i's result is entries
+ * [out_starts[i], out_starts[i+1]) of out_keys (key_bytes 4 or 8, copied
+ * from group_keys), out_count, out_sum and out_mean (val_type JFX_F32 or
+ * JFX_F64 for every query), groups ascending. Outputs hold sum(ngroups)
+ * entries; out_starts is host memory of nqueries + 1 entries, written after
+ * the call synchronises the stream once. Equal to jfx_served_groups followed
+ * by a stable compaction of each query's groups. No counterpart in the
+ * reference. */
+int jfx_served_batch(int val_type, int key_bytes, const jfx_served_query* queries,
+                     uint32_t nqueries, void* out_keys, int64_t* out_count, void* out_sum,
+                     void* out_mean, uint64_t* out_starts, void* workspace, uint64_t ws_bytes,
+                     void* stream);
 
 /* ---- composite keys (multi-column sort / groupby / join) -------------- */
 /* Packs 1..8 integer key columns (JFX_U32/I32/U64/I64, n rows each) into one

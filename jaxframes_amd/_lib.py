@@ -60,12 +60,17 @@ _SIGNATURES = {
     "jfx_join_count": [c_int, c_vp, c_u64, c_vp, c_u64, c_u64p, c_vp, c_u64, c_vp],
     "jfx_join_fill": [c_int, c_vp, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_u64,
                       c_vp],
+    "jfx_join_fill_values": [c_int, c_vp, c_vp, c_u64, c_vp, c_vp, c_u64, c_int, c_vp, c_int, c_int,
+                             c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_vp],
     "jfx_join_first_match": [c_int, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp],
     "jfx_gather": [c_int, c_vp, c_vp, c_u64, c_vp, c_u64, c_vp],
     "jfx_batched_groupby_workspace_size": [ctypes.c_uint32, c_int, c_u64, c_u64p],
     "jfx_batched_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, ctypes.c_uint32, c_u64,
                             ctypes.c_uint32, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_u64, c_vp],
+    "jfx_served_batch_workspace_size": [ctypes.c_uint32, c_u64, c_u64p],
+    "jfx_served_batch": [c_int, c_int, c_vp, ctypes.c_uint32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                         c_u64, c_vp],
     "jfx_served_groups": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint32, c_int, c_vp,
                           c_vp, c_vp, c_vp, c_vp, c_vp],
     "jfx_compact_workspace_size": [c_u64, c_u64p],

@@ -173,11 +173,24 @@ __global__ void __launch_bounds__(1024) k_join_scan(uint64_t* c, uint32_t n, uin
   if (tid == 0) *total = s_w[16];
 }
 
-template <typename K>
+// One payload column per side written by the fill itself (the value gather
+// of parallel_algorithms.py:1052-1073 fused into pair enumeration): src is
+// indexed by the side's source row (by_row) or by its sorted position.
+template <typename LT, typename RT>
+struct FillPay {
+  const LT* lv;
+  const RT* rv;
+  LT* out_lv;
+  RT* out_rv;
+  int l_by_row;
+  int r_by_row;
+};
+
+template <typename K, typename LT, typename RT>
 __global__ void __launch_bounds__(JB)
 k_join_fill(const K* sl, const uint32_t* lperm, uint64_t L, const K* sr, const uint32_t* rperm,
             int oc, const uint64_t* wlo, const uint64_t* whi, const uint64_t* tile_off,
-            uint32_t* out_l, uint32_t* out_r, K* out_key) {
+            uint32_t* out_l, uint32_t* out_r, K* out_key, FillPay<LT, RT> pay) {
   __shared__ K s_w[JWMAX];
   __shared__ uint32_t s_incl[JT];   // inclusive scan of the tile's counts
   __shared__ uint32_t s_lo[JT];
@@ -209,9 +222,14 @@ k_join_fill(const K* sl, const uint32_t* lperm, uint64_t L, const K* sr, const u
     const uint32_t r = a;
     const uint64_t i = tb + r;
     const uint64_t rr = (uint64_t)s_lo[r] + (k - s_incl[r]);
-    out_l[base + k] = lperm[i];
-    out_r[base + k] = rperm[rr];
+    // left rows: through lperm, or the sorted position when lperm is null
+    const uint32_t lrow = lperm ? lperm[i] : (uint32_t)i;
+    const uint32_t rrow = rperm ? rperm[rr] : (uint32_t)rr;
+    if (out_l) out_l[base + k] = lrow;
+    if (out_r) out_r[base + k] = rrow;
     if (out_key) out_key[base + k] = sl[i];
+    if (pay.out_lv) pay.out_lv[base + k] = pay.lv[pay.l_by_row ? lrow : i];
+    if (pay.out_rv) pay.out_rv[base + k] = pay.rv[pay.r_by_row ? rrow : rr];
   }
 }
 
@@ -286,16 +304,49 @@ int join_count_typed(int oc, const void* sl, uint64_t L, const void* sr, uint64_
   return JFX_OK;
 }
 
-template <typename K>
+template <typename K, typename LT = uint32_t, typename RT = uint32_t>
 int join_fill_typed(int oc, const void* sl, const uint32_t* lperm, uint64_t L, const void* sr,
                     const uint32_t* rperm, const JoinWs& w, uint32_t* out_l, uint32_t* out_r,
-                    void* out_key, hipStream_t s) {
+                    void* out_key, hipStream_t s, FillPay<LT, RT> pay = {}) {
   const uint32_t nt = (uint32_t)jtiles(L);
-  JFX_LAUNCH("join_fill", s, (k_join_fill<K><<<nt, JB, 0, s>>>((const K*)sl, lperm, L, (const K*)sr,
-                                                                rperm, oc, w.wlo, w.whi, w.tcnt,
-                                                                out_l, out_r, (K*)out_key)));
+  JFX_LAUNCH("join_fill", s, (k_join_fill<K, LT, RT><<<nt, JB, 0, s>>>(
+                                 (const K*)sl, lperm, L, (const K*)sr, rperm, oc, w.wlo, w.whi,
+                                 w.tcnt, out_l, out_r, (K*)out_key, pay)));
   JFX_HIP(hipGetLastError());
   return JFX_OK;
+}
+
+template <typename K, typename LT, typename RT>
+int join_fill_pay3(int oc, const void* sl, const uint32_t* lperm, uint64_t L, const void* sr,
+                   const uint32_t* rperm, const JoinWs& w, uint32_t* out_l, uint32_t* out_r,
+                   void* out_key, const void* lv, int l_by_row, void* out_lv, const void* rv,
+                   int r_by_row, void* out_rv, hipStream_t s) {
+  FillPay<LT, RT> pay{(const LT*)lv, (const RT*)rv, (LT*)out_lv, (RT*)out_rv, l_by_row, r_by_row};
+  return join_fill_typed<K, LT, RT>(oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key, s, pay);
+}
+
+template <typename K, typename LT>
+int join_fill_pay2(int rb, int oc, const void* sl, const uint32_t* lperm, uint64_t L,
+                   const void* sr, const uint32_t* rperm, const JoinWs& w, uint32_t* out_l,
+                   uint32_t* out_r, void* out_key, const void* lv, int l_by_row, void* out_lv,
+                   const void* rv, int r_by_row, void* out_rv, hipStream_t s) {
+  if (rb == 8)
+    return join_fill_pay3<K, LT, uint64_t>(oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key,
+                                           lv, l_by_row, out_lv, rv, r_by_row, out_rv, s);
+  return join_fill_pay3<K, LT, uint32_t>(oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key, lv,
+                                         l_by_row, out_lv, rv, r_by_row, out_rv, s);
+}
+
+template <typename K>
+int join_fill_pay(int lb, int rb, int oc, const void* sl, const uint32_t* lperm, uint64_t L,
+                  const void* sr, const uint32_t* rperm, const JoinWs& w, uint32_t* out_l,
+                  uint32_t* out_r, void* out_key, const void* lv, int l_by_row, void* out_lv,
+                  const void* rv, int r_by_row, void* out_rv, hipStream_t s) {
+  if (lb == 8)
+    return join_fill_pay2<K, uint64_t>(rb, oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key,
+                                       lv, l_by_row, out_lv, rv, r_by_row, out_rv, s);
+  return join_fill_pay2<K, uint32_t>(rb, oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key, lv,
+                                     l_by_row, out_lv, rv, r_by_row, out_rv, s);
 }
 
 }  // namespace
@@ -354,6 +405,44 @@ int jfx_join_fill(int key_type, const void* sorted_left, const uint32_t* left_ro
                                           right_rows, w, out_left_rows, out_right_rows, out_keys, s);
   return jfx::join_fill_typed<uint64_t>(oc, sorted_left, left_rows, n_left, sorted_right,
                                         right_rows, w, out_left_rows, out_right_rows, out_keys, s);
+}
+
+int jfx_join_fill_values(int key_type, const void* sorted_left, const uint32_t* left_rows,
+                         uint64_t n_left, const void* sorted_right, const uint32_t* right_rows,
+                         uint64_t n_right, int left_elem_bytes, const void* left_values,
+                         int left_by_row, int right_elem_bytes, const void* right_values,
+                         int right_by_row, uint32_t* out_left_rows, uint32_t* out_right_rows,
+                         void* out_keys, void* out_left_values, void* out_right_values,
+                         void* workspace, uint64_t ws_bytes, void* stream) {
+  JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64, JFX_ERR_INVALID, "bad key type");
+  JFX_CHECK((left_elem_bytes == 4 || left_elem_bytes == 8) &&
+                (right_elem_bytes == 4 || right_elem_bytes == 8),
+            JFX_ERR_UNSUPPORTED, "payload elements must be 4 or 8 bytes");
+  if (n_left == 0 || n_right == 0) return JFX_OK;
+  JFX_CHECK(sorted_left && sorted_right && workspace, JFX_ERR_INVALID, "null argument");
+  JFX_CHECK(!out_left_values || left_values, JFX_ERR_INVALID, "left payload output without input");
+  JFX_CHECK(!out_right_values || right_values, JFX_ERR_INVALID,
+            "right payload output without input");
+  JFX_CHECK(!(out_left_rows || (out_left_values && left_by_row)) || left_rows, JFX_ERR_INVALID,
+            "left rows needed for row outputs or row-indexed payloads");
+  JFX_CHECK(!(out_right_rows || (out_right_values && right_by_row)) || right_rows, JFX_ERR_INVALID,
+            "right rows needed for row outputs or row-indexed payloads");
+  jfx::Carver c(workspace, ws_bytes);
+  jfx::JoinWs w = jfx::carve_join(c, n_left);
+  JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "join workspace too small");
+  const int oc = jfx::order_code(key_type);
+  hipStream_t s = jfx::as_stream(stream);
+  if (jfx::key_bytes(key_type) == 4)
+    return jfx::join_fill_pay<uint32_t>(left_elem_bytes, right_elem_bytes, oc, sorted_left,
+                                        left_rows, n_left, sorted_right, right_rows, w,
+                                        out_left_rows, out_right_rows, out_keys, left_values,
+                                        left_by_row, out_left_values, right_values, right_by_row,
+                                        out_right_values, s);
+  return jfx::join_fill_pay<uint64_t>(left_elem_bytes, right_elem_bytes, oc, sorted_left, left_rows,
+                                      n_left, sorted_right, right_rows, w, out_left_rows,
+                                      out_right_rows, out_keys, left_values, left_by_row,
+                                      out_left_values, right_values, right_by_row,
+                                      out_right_values, s);
 }
 
 int jfx_join_first_match(int key_type, const void* sorted_probe, uint64_t n_probe,

@@ -64,59 +64,197 @@ __device__ __forceinline__ uint32_t sv_bound(const V* v, uint32_t lo, uint32_t h
   return lo;
 }
 
+// Evaluation of one group: passing-row count and the float64 sum of the
+// passing values (NaN / +-inf rules as stated above).
+struct SvEval {
+  uint64_t cnt;
+  double s;
+};
+
 template <typename V>
-__global__ void __launch_bounds__(256) k_served_groups(ServedArgs<V> a) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.G) return;
-  const uint32_t lo = a.starts[g], hi = a.starts[g + 1], hn = a.nan_start[g];
-  const bool tnan = a.thr != a.thr;
-  const auto x = ord_val(a.thr);
+__device__ __forceinline__ SvEval served_eval(const V* vals, const double* prefix,
+                                              const uint32_t* starts, const uint32_t* nan_start,
+                                              const uint32_t* ninf_end, const uint32_t* pinf_start,
+                                              uint32_t g, int cmp, V thr) {
+  const uint32_t lo = starts[g], hi = starts[g + 1], hn = nan_start[g];
+  const bool tnan = thr != thr;
+  const auto x = ord_val(thr);
   // up to two passing runs [r0, r1) and [r2, r3) among the non-NaN rows
   uint32_t r0 = lo, r1 = lo, r2 = hn, r3 = hn;
   bool nan_pass = false;
   if (!tnan) {
-    switch (a.cmp) {
-      case JFX_CMP_GT: r0 = sv_bound(a.vals, lo, hn, x, true); r1 = hn; break;
-      case JFX_CMP_GE: r0 = sv_bound(a.vals, lo, hn, x, false); r1 = hn; break;
-      case JFX_CMP_LT: r0 = lo; r1 = sv_bound(a.vals, lo, hn, x, false); break;
-      case JFX_CMP_LE: r0 = lo; r1 = sv_bound(a.vals, lo, hn, x, true); break;
+    switch (cmp) {
+      case JFX_CMP_GT: r0 = sv_bound(vals, lo, hn, x, true); r1 = hn; break;
+      case JFX_CMP_GE: r0 = sv_bound(vals, lo, hn, x, false); r1 = hn; break;
+      case JFX_CMP_LT: r0 = lo; r1 = sv_bound(vals, lo, hn, x, false); break;
+      case JFX_CMP_LE: r0 = lo; r1 = sv_bound(vals, lo, hn, x, true); break;
       case JFX_CMP_EQ:
-        r0 = sv_bound(a.vals, lo, hn, x, false);
-        r1 = sv_bound(a.vals, r0, hn, x, true);
+        r0 = sv_bound(vals, lo, hn, x, false);
+        r1 = sv_bound(vals, r0, hn, x, true);
         break;
       default:  // NE: everything but the equal run, NaN rows included
         r0 = lo;
-        r1 = sv_bound(a.vals, lo, hn, x, false);
-        r2 = sv_bound(a.vals, r1, hn, x, true);
+        r1 = sv_bound(vals, lo, hn, x, false);
+        r2 = sv_bound(vals, r1, hn, x, true);
         r3 = hn;
         nan_pass = true;
         break;
     }
-  } else if (a.cmp == JFX_CMP_NE) {  // t is NaN: every row passes !=
+  } else if (cmp == JFX_CMP_NE) {  // t is NaN: every row passes !=
     r0 = lo;
     r1 = hn;
     nan_pass = true;
   }
   const uint32_t nn = nan_pass ? hi - hn : 0u;
-  const uint64_t cnt = (uint64_t)(r1 - r0) + (uint64_t)(r3 - r2) + nn;
+  SvEval e;
+  e.cnt = (uint64_t)(r1 - r0) + (uint64_t)(r3 - r2) + nn;
   // finite rows through the prefix sums; -inf rows lead and +inf rows end the
   // group's non-NaN run, so the passing runs' overlaps with them count them
-  double s = (a.prefix[r1] - a.prefix[r0]) + (a.prefix[r3] - a.prefix[r2]);
-  const uint32_t ne = a.ninf_end[g], ps = a.pinf_start[g];
+  double s = (prefix[r1] - prefix[r0]) + (prefix[r3] - prefix[r2]);
+  const uint32_t ne = ninf_end[g], ps = pinf_start[g];
   auto overlap = [](uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
-    const uint32_t b = x0 > y0 ? x0 : y0, e = x1 < y1 ? x1 : y1;
-    return e > b ? e - b : 0u;
+    const uint32_t b = x0 > y0 ? x0 : y0, en = x1 < y1 ? x1 : y1;
+    return en > b ? en - b : 0u;
   };
   const uint32_t nneg = overlap(r0, r1, lo, ne) + overlap(r2, r3, lo, ne);
   const uint32_t npos = overlap(r0, r1, ps, hn) + overlap(r2, r3, ps, hn);
   if (npos) s += __builtin_inf();
   if (nneg) s -= __builtin_inf();
   if (nn) s = __builtin_nan("");
-  a.out_count[g] = (int64_t)cnt;
-  if (a.out_sum) a.out_sum[g] = (V)s;
-  if (a.out_mean) a.out_mean[g] = cnt ? (V)(s / (double)cnt) : (V)0;
-  a.out_mask[g] = cnt ? 1 : 0;
+  e.s = s;
+  return e;
 }
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_served_groups(ServedArgs<V> a) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.G) return;
+  const SvEval e = served_eval(a.vals, a.prefix, a.starts, a.nan_start, a.ninf_end, a.pinf_start,
+                               g, a.cmp, a.thr);
+  a.out_count[g] = (int64_t)e.cnt;
+  if (a.out_sum) a.out_sum[g] = (V)e.s;
+  if (a.out_mean) a.out_mean[g] = e.cnt ? (V)(e.s / (double)e.cnt) : (V)0;
+  a.out_mask[g] = e.cnt ? 1 : 0;
+}
+
+// ---- a batch of queries in one launch --------------------------------------
+// The groups of all queries form one index space (query-major); a workgroup
+// takes the next tile of SVB groups by ticket, evaluates them, and writes the
+// groups that keep rows compacted in that order, its output position coming
+// from a decoupled lookback over the tiles' pass counts. The first group of
+// each query records where the query's output starts.
+constexpr int SVB = 256;
+constexpr uint32_t SV_QMAX = 1024;
+
+struct SvDev {
+  jfx_served_query q;
+  uint64_t goff;  // first index of the query's groups in the batch
+};
+
+template <typename V, typename KT>
+__global__ void __launch_bounds__(SVB)
+k_served_batch(const SvDev* qs, uint32_t Q, uint64_t tot, uint32_t* desc, uint32_t* counter,
+               uint32_t* err, KT* out_keys, int64_t* out_count, V* out_sum, V* out_mean,
+               uint64_t* out_start) {
+  __shared__ uint64_t s_off[SV_QMAX];
+  __shared__ uint32_t s_wcnt[SVB / 64];
+  __shared__ uint32_t s_tile, s_excl;
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) s_tile = atomicAdd(counter, 1u);
+  for (uint32_t i = tid; i < Q; i += SVB) s_off[i] = qs[i].goff;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint64_t g = (uint64_t)tile * SVB + tid;
+  bool pass = false;
+  SvEval e{0, 0.0};
+  uint32_t q = 0;
+  uint32_t gl = 0;
+  if (g < tot) {
+    uint32_t lo = 0, hi = Q;  // last query whose first group is <= g
+    while (lo + 1 < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_off[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    q = lo;
+    gl = (uint32_t)(g - s_off[q]);
+    const jfx_served_query& qq = qs[q].q;
+    e = served_eval(static_cast<const V*>(qq.values), qq.prefix, qq.group_starts, qq.nan_start,
+                    qq.ninf_end, qq.pinf_start, gl, qq.cmp, (V)qq.threshold);
+    pass = e.cnt != 0;
+  }
+  const uint64_t m = ballot64(pass);
+  const uint32_t below = mbcnt(m);
+  const uint32_t wave = tid >> 6;
+  if ((tid & 63) == 0) s_wcnt[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  uint32_t woff = 0, T = 0;
+#pragma unroll
+  for (int w = 0; w < SVB / 64; ++w) {
+    if ((uint32_t)w < wave) woff += s_wcnt[w];
+    T += s_wcnt[w];
+  }
+  if (tid == 0) lookback_publish(desc, tile, 1u, 0u, T);
+  if (tid < 64) {
+    const uint32_t ex = wave_lookback_u32(desc, tile, T, err);
+    if (tid == 0) s_excl = ex;
+  }
+  __syncthreads();
+  if (g >= tot) return;
+  const uint64_t pos = (uint64_t)s_excl + woff + below;
+  if (gl == 0) {  // this query's output starts here, as do those of empty queries before it
+    out_start[q] = pos;
+    for (uint32_t p = q; p > 0 && s_off[p - 1] == s_off[q]; --p) out_start[p - 1] = pos;
+  }
+  if (g == tot - 1)  // the end of the output, and the starts of trailing empty queries
+    for (uint32_t p = q + 1; p <= Q; ++p) out_start[p] = pos + (pass ? 1u : 0u);
+  if (pass) {
+    out_keys[pos] = static_cast<const KT*>(qs[q].q.group_keys)[gl];
+    out_count[pos] = (int64_t)e.cnt;
+    out_sum[pos] = (V)e.s;
+    out_mean[pos] = (V)(e.s / (double)e.cnt);
+  }
+}
+
+__global__ void k_served_zero(uint32_t* w, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) w[i] = 0u;
+}
+
+namespace {
+
+struct SvWs {
+  SvDev* table;
+  uint64_t* start;
+  uint32_t* desc;    // one descriptor per tile, then the ticket counter
+};
+
+SvWs carve_served(Carver& c, uint32_t Q, uint64_t tot) {
+  SvWs w{};
+  w.table = c.take<SvDev>(Q);
+  w.start = c.take<uint64_t>((uint64_t)Q + 1);
+  w.desc = c.take<uint32_t>((tot + SVB - 1) / SVB + 1);
+  return w;
+}
+
+template <typename V, typename KT>
+int served_batch_typed(const SvWs& w, uint32_t Q, uint64_t tot, void* ok, int64_t* oc, void* os,
+                       void* om, hipStream_t s) {
+  const uint64_t nt = (tot + SVB - 1) / SVB;
+  uint32_t* err = device_error_word();
+  JFX_CHECK(err != nullptr, JFX_ERR_HIP, "cannot resolve the device error word");
+  JFX_LAUNCH("served_zero", s, (k_served_zero<<<(uint32_t)std::min<uint64_t>(64, (nt + 256) / 256),
+                                                 256, 0, s>>>(w.desc, nt + 1)));
+  JFX_HIP(hipGetLastError());
+  JFX_LAUNCH("served_batch", s, (k_served_batch<V, KT><<<(uint32_t)nt, SVB, 0, s>>>(
+                                    w.table, Q, tot, w.desc, w.desc + nt, err,
+                                    static_cast<KT*>(ok), oc, static_cast<V*>(os),
+                                    static_cast<V*>(om), w.start)));
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
+
+}  // namespace
 
 }  // namespace jfx
 
@@ -150,6 +288,76 @@ int jfx_served_groups(int val_type, const void* vals, const double* prefix, cons
     JFX_LAUNCH("served_groups", s, (jfx::k_served_groups<double><<<blocks, 256, 0, s>>>(a)));
   }
   JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
+
+int jfx_served_batch_workspace_size(uint32_t nqueries, uint64_t total_groups,
+                                    uint64_t* out_bytes) {
+  JFX_CHECK(out_bytes != nullptr, JFX_ERR_INVALID, "out_bytes is null");
+  jfx::Carver c(nullptr, 0);
+  jfx::carve_served(c, nqueries, total_groups);
+  *out_bytes = c.used();
+  return JFX_OK;
+}
+
+int jfx_served_batch(int val_type, int key_bytes, const jfx_served_query* queries,
+                     uint32_t nqueries, void* out_keys, int64_t* out_count, void* out_sum,
+                     void* out_mean, uint64_t* out_starts, void* workspace, uint64_t ws_bytes,
+                     void* stream) {
+  JFX_CHECK(val_type == JFX_F32 || val_type == JFX_F64, JFX_ERR_UNSUPPORTED,
+            "indexed filters take float32 / float64 values");
+  JFX_CHECK(key_bytes == 4 || key_bytes == 8, JFX_ERR_UNSUPPORTED, "group keys must be 4 or 8 bytes");
+  JFX_CHECK(out_starts != nullptr, JFX_ERR_INVALID, "out_starts is null");
+  JFX_CHECK(nqueries <= jfx::SV_QMAX, JFX_ERR_UNSUPPORTED, "at most 1024 queries per batch");
+  if (nqueries == 0) {
+    out_starts[0] = 0;
+    return JFX_OK;
+  }
+  JFX_CHECK(queries != nullptr, JFX_ERR_INVALID, "queries is null");
+  uint64_t tot = 0;
+  for (uint32_t i = 0; i < nqueries; ++i) {
+    const jfx_served_query& q = queries[i];
+    JFX_CHECK(q.cmp >= JFX_CMP_GT && q.cmp <= JFX_CMP_NE, JFX_ERR_INVALID, "bad comparison code");
+    JFX_CHECK(q.ngroups == 0 || (q.values && q.prefix && q.group_starts && q.nan_start &&
+                                 q.ninf_end && q.pinf_start && q.group_keys),
+              JFX_ERR_INVALID, "null query column");
+    tot += q.ngroups;
+  }
+  JFX_CHECK(tot < (1ull << 30), JFX_ERR_UNSUPPORTED, "a batch holds fewer than 2^30 groups");
+  if (tot == 0) {
+    for (uint32_t i = 0; i <= nqueries; ++i) out_starts[i] = 0;
+    return JFX_OK;
+  }
+  JFX_CHECK(out_keys && out_count && out_sum && out_mean && workspace, JFX_ERR_INVALID,
+            "null output or workspace");
+  jfx::Carver c(workspace, ws_bytes);
+  jfx::SvWs w = jfx::carve_served(c, nqueries, tot);
+  JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "served batch workspace too small");
+  hipStream_t s = jfx::as_stream(stream);
+  // query table and the read-back of the output starts share one pinned block
+  const size_t tb = sizeof(jfx::SvDev) * nqueries, sb = sizeof(uint64_t) * (nqueries + 1);
+  char* h = static_cast<char*>(jfx::pinned_scratch(tb + sb));
+  JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
+  jfx::SvDev* ht = reinterpret_cast<jfx::SvDev*>(h);
+  uint64_t off = 0;
+  for (uint32_t i = 0; i < nqueries; ++i) {
+    ht[i].q = queries[i];
+    ht[i].goff = off;
+    off += queries[i].ngroups;
+  }
+  JFX_HIP(hipMemcpyAsync(w.table, ht, tb, hipMemcpyHostToDevice, s));
+  int rc;
+  if (val_type == JFX_F32)
+    rc = key_bytes == 4 ? jfx::served_batch_typed<float, uint32_t>(w, nqueries, tot, out_keys, out_count, out_sum, out_mean, s)
+                        : jfx::served_batch_typed<float, uint64_t>(w, nqueries, tot, out_keys, out_count, out_sum, out_mean, s);
+  else
+    rc = key_bytes == 4 ? jfx::served_batch_typed<double, uint32_t>(w, nqueries, tot, out_keys, out_count, out_sum, out_mean, s)
+                        : jfx::served_batch_typed<double, uint64_t>(w, nqueries, tot, out_keys, out_count, out_sum, out_mean, s);
+  if (rc != JFX_OK) return rc;
+  uint64_t* hs = reinterpret_cast<uint64_t*>(h + tb);
+  JFX_HIP(hipMemcpyAsync(hs, w.start, sb, hipMemcpyDeviceToHost, s));
+  JFX_HIP(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i <= nqueries; ++i) out_starts[i] = hs[i];
   return JFX_OK;
 }
 

@@ -110,3 +110,48 @@ def batched_frames(nframes: int = 4096, min_rows: int = 1000, max_rows: int = 4_
     keys = (torch.rand(n, dtype=torch.float64, device=device, generator=gd) * per_row_card).to(torch.int32)
     vals = torch.randn(n, dtype=torch.float32, device=device, generator=gd)
     return keys, vals, offsets, cards
+
+
+def served_frames(nframes: int = 8, min_rows: int = 10**6, max_rows: int = 5 * 10**7,
+                  min_card: int = 10, max_card: int = 10**6, seed: int = 1234, device="cuda"):
+    """Resident frames of the served stream: [(name, int64 keys, float32
+    values)], rows and key cardinality log-uniform."""
+    g = torch.Generator()
+    g.manual_seed(seed)
+    u = torch.rand(nframes, generator=g, dtype=torch.float64)
+    rows = torch.floor(torch.exp(math.log(min_rows) + u * (math.log(max_rows) - math.log(min_rows))))
+    uc = torch.rand(nframes, generator=g, dtype=torch.float64)
+    cards = torch.floor(torch.exp(math.log(min_card) + uc * (math.log(max_card) - math.log(min_card))))
+    gd = _gen(seed + 1, device)
+    out = []
+    for i in range(nframes):
+        n, c = int(rows[i]), int(cards[i])
+        k = torch.randint(0, c, (n,), dtype=torch.int64, device=device, generator=gd)
+        v = torch.randn(n, dtype=torch.float32, device=device, generator=gd)
+        out.append((f"f{i}", k, v, c))
+    return out
+
+
+def served_queries(frames, nq: int, join_frac: float = 0.3, min_dim: int = 10**5,
+                   max_dim: int = 10**6, seed: int = 1234, device="cuda"):
+    """The served query mix over frames from served_frames: filter(v > t,
+    t ~ U(-1, 1)) -> groupby(k).agg(sum, count, mean), and (join_frac of
+    them) inner joins with a dimension table of log-uniform size whose
+    distinct int64 keys cover part of the frame's key range."""
+    from .serving import FilterGroupBy, Join
+    g = torch.Generator()
+    g.manual_seed(seed)
+    gd = _gen(seed + 2, device)
+    qs = []
+    for _ in range(nq):
+        f = int(torch.randint(0, len(frames), (1,), generator=g))
+        name, _, _, card = frames[f]
+        if float(torch.rand(1, generator=g, dtype=torch.float64)) >= join_frac:
+            t = float(torch.rand(1, generator=g, dtype=torch.float64)) * 2 - 1
+            qs.append(FilterGroupBy(name, ">", t))
+        else:
+            u = float(torch.rand(1, generator=g, dtype=torch.float64))
+            nd = int(math.floor(math.exp(math.log(min_dim) + u * (math.log(max_dim) - math.log(min_dim)))))
+            dk = torch.randperm(max(card, nd), device=device, generator=gd)[:nd].to(torch.int64)
+            qs.append(Join(name, dk, torch.randn(nd, dtype=torch.float32, device=device, generator=gd)))
+    return qs

@@ -80,14 +80,74 @@ def _first_match(sorted_probe, sorted_build, build_perm, kc):
     return out
 
 
+def payload_source(col, dev) -> torch.Tensor:
+    """A payload column as the join emits it: floats as they are (half types
+    widened to float32), integers promoted to float64 (8-byte) or float32 —
+    the promotion jnp.where(valid, col[idx], nan) applies in the reference."""
+    col = to_device(col, dev)
+    if col.dtype.is_floating_point:
+        return col if col.element_size() in (4, 8) else col.float()
+    return col.to(torch.float64 if col.element_size() == 8 else torch.float32)
+
+
+def fill_values(kc, sl, lperm, L, sr, rperm, R, M, lsrc, l_by_row, rsrc, r_by_row, ws, *,
+                keys_dtype, want_rows=(False, False)):
+    """The M pairs counted on ws: keys, one payload per side written by the
+    fill (jfx_join_fill_values), and the row numbers when asked for."""
+    dev = sl.device
+    keys = torch.empty(M, dtype=keys_dtype, device=dev)
+    lv = torch.empty(M, dtype=lsrc.dtype, device=dev) if lsrc is not None else None
+    rv = torch.empty(M, dtype=rsrc.dtype, device=dev) if rsrc is not None else None
+    li = torch.empty(M, dtype=torch.int32, device=dev) if want_rows[0] else None
+    ri = torch.empty(M, dtype=torch.int32, device=dev) if want_rows[1] else None
+    if M:
+        _lib.call("jfx_join_fill_values", kc, _lib.ptr(sl), _lib.ptr(lperm), L, _lib.ptr(sr),
+                  _lib.ptr(rperm), R, lsrc.element_size() if lsrc is not None else 4,
+                  _lib.ptr(lsrc), int(l_by_row), rsrc.element_size() if rsrc is not None else 4,
+                  _lib.ptr(rsrc), int(r_by_row), _lib.ptr(li), _lib.ptr(ri), _lib.ptr(keys),
+                  _lib.ptr(lv), _lib.ptr(rv), _lib.ptr(ws), ws.numel(), _stream(dev))
+    return keys, lv, rv, li, ri
+
+
+def _inner_join_fused(left_keys, left_values: dict, right_keys, right_values: dict):
+    """Inner join whose first payload column on each side is written by the
+    fill; further columns are gathered by row."""
+    lk, sl, lperm, lorig = _sorted_side(left_keys)
+    rk, sr, rperm, _ = _sorted_side(right_keys)
+    if lk.dtype != rk.dtype:
+        raise TypeError(f"join keys have different dtypes: {lk.dtype} vs {rk.dtype}")
+    dev = lk.device
+    L, R = lk.shape[0], rk.shape[0]
+    kc = key_code(lk)
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_join_workspace_size", L, ctypes.byref(nb))
+    ws = _lib.workspace(nb.value, dev)
+    m = ctypes.c_uint64(0)
+    _lib.call("jfx_join_count", kc, _lib.ptr(sl), L, _lib.ptr(sr), R, ctypes.byref(m),
+              _lib.ptr(ws), ws.numel(), _stream(dev))
+    M = m.value
+    if M >= (1 << 32):
+        raise _lib.JfxError(f"inner join produces {M} pairs; at most 2^32-1 are supported")
+    lcols, rcols = list(left_values.items()), list(right_values.items())
+    lsrc = payload_source(lcols[0][1], dev) if lcols else None
+    rsrc = payload_source(rcols[0][1], dev) if rcols else None
+    keys, lv, rv, li, ri = fill_values(kc, sl, lperm, L, sr, rperm, R, M, lsrc, True, rsrc, True,
+                                       ws, keys_dtype=lk.dtype,
+                                       want_rows=(len(lcols) > 1, len(rcols) > 1))
+    if keys.dtype != lorig:
+        keys = keys.to(lorig)
+    out = {}
+    for j, (c, v) in enumerate(lcols):
+        out[f"left_{c}"] = lv if j == 0 else gather(v, li)
+    for j, (c, v) in enumerate(rcols):
+        out[f"right_{c}"] = rv if j == 0 else gather(v, ri)
+    return keys, out
+
+
 def gather(col: torch.Tensor, rows: torch.Tensor, fill_nan: bool = True) -> torch.Tensor:
     """col[rows] with rows as uint32 (int32 storage); rows == 0xFFFFFFFF give
     NaN. Integer columns come back as floats (the reference's promotion)."""
-    col = to_device(col, rows.device)
-    if col.dtype.is_floating_point:
-        src = col if col.element_size() in (4, 8) else col.float()
-    else:
-        src = col.to(torch.float64 if col.element_size() == 8 else torch.float32)
+    src = payload_source(col, rows.device)
     eb = src.element_size()
     out = torch.empty(rows.shape[0], dtype=src.dtype, device=rows.device)
     fill = _NAN32 if eb == 4 else _NAN64
@@ -102,8 +162,8 @@ def sort_merge_join(left_keys, left_values: dict, right_keys, right_values: dict
     """Single-key join; returns (joined keys, {left_<c>/right_<c>: column})."""
     _lib.require_gpu()
     if how == "inner":
-        keys, li, ri = inner_join_rows(left_keys, right_keys)
-    elif how in ("left", "right", "outer"):
+        return _inner_join_fused(left_keys, left_values, right_keys, right_values)
+    if how in ("left", "right", "outer"):
         lk, sl, lperm, lorig = _sorted_side(left_keys)
         rk, sr, rperm, rorig = _sorted_side(right_keys)
         if lk.dtype != rk.dtype:

@@ -56,10 +56,7 @@ class Frame:
         self.sorted_keys = sk
         self.perm = perm                                  # uint32 row index (int32 storage)
         self.sorted_values = self.values[perm.long()] if self.n else self.values
-        # identity rows: a join fill then reports sorted positions, and the left
-        # payload comes from sorted_values in (nearly) sequential order
-        self.iota = torch.arange(self.n, dtype=torch.int32, device=dev)
-        parts = [self.keys, self.values, self.sorted_keys, self.perm, self.sorted_values, self.iota]
+        parts = [self.keys, self.values, self.sorted_keys, self.perm, self.sorted_values]
         self.indexed = self.values.dtype in (torch.float32, torch.float64) and self.n > 0
         if self.indexed:
             # (key, value) order: stable by value, then stable by key
@@ -87,6 +84,11 @@ class Frame:
             self.prefix = torch.zeros(self.n + 1, dtype=torch.float64, device=dev)
             self.prefix[1:] = torch.cumsum(fin.double(), 0)
             self.ngroups = G
+            # the index columns in jfx_served_query field order
+            self.index_ptrs = (self.kv_values.data_ptr(), self.prefix.data_ptr(),
+                               self.group_starts.data_ptr(), self.group_nan_start.data_ptr(),
+                               self.group_ninf_end.data_ptr(), self.group_pinf_start.data_ptr(),
+                               self.group_keys.data_ptr())
             parts += [self.kv_values, self.group_keys, self.group_starts, self.group_nan_start,
                       self.group_pinf_start, self.group_ninf_end, self.prefix]
         self.nbytes = sum(t.numel() * t.element_size() for t in parts)
@@ -140,6 +142,8 @@ class Request:
     future: Future = field(default_factory=Future)
     rid: int = 0
     footprint: int = 0
+    t_admit: float = 0.0     # perf_counter() when a slot took it
+    t_done: float = 0.0      # when its result was ready on the host
 
 
 def footprint(store: FrameStore, q) -> int:
@@ -183,35 +187,65 @@ def _segreduce(sk, sv, aggs):
 
 
 _INDEXED_AGGS = {"sum", "count", "mean"}
+_SERVED_QMAX = 1024
+
+
+class ServedQuery(ctypes.Structure):
+    """jfx_served_query: one indexed filter -> groupby query of a batch."""
+    _fields_ = [("values", ctypes.c_void_p), ("prefix", ctypes.c_void_p),
+                ("group_starts", ctypes.c_void_p), ("nan_start", ctypes.c_void_p),
+                ("ninf_end", ctypes.c_void_p), ("pinf_start", ctypes.c_void_p),
+                ("group_keys", ctypes.c_void_p), ("ngroups", ctypes.c_uint32),
+                ("cmp", ctypes.c_int32), ("threshold", ctypes.c_double)]
+
+
+def _served_batch(store: FrameStore, qs: list) -> list:
+    """Indexed filter -> groupby queries on frames with one (key type, value
+    type), all in one jfx_served_batch launch; [(keys, {agg: column})]."""
+    from .compaction import OPS
+    frs = [store[q.frame] for q in qs]
+    dev = frs[0].keys.device
+    kdt, vdt = frs[0].group_keys.dtype, frs[0].kv_values.dtype
+    Q = len(qs)
+    table = (ServedQuery * Q)()
+    tot = 0
+    for i, (q, fr) in enumerate(zip(qs, frs)):
+        if q.op not in OPS:
+            raise ValueError(f"unsupported comparison {q.op!r}")
+        table[i] = ServedQuery(*fr.index_ptrs, fr.ngroups, OPS[q.op], float(q.threshold))
+        tot += fr.ngroups
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_served_batch_workspace_size", Q, tot, ctypes.byref(nb))
+    ws = _lib.workspace(nb.value, dev)
+    cap = max(tot, 1)
+    keys = torch.empty(cap, dtype=kdt, device=dev)
+    cnt = torch.empty(cap, dtype=torch.int64, device=dev)
+    sm = torch.empty(cap, dtype=vdt, device=dev)
+    mean = torch.empty(cap, dtype=vdt, device=dev)
+    starts = (ctypes.c_uint64 * (Q + 1))()
+    vt = _lib.JFX_F32 if vdt == torch.float32 else _lib.JFX_F64
+    _lib.call("jfx_served_batch", vt, keys.element_size(), table, Q, _lib.ptr(keys), _lib.ptr(cnt),
+              _lib.ptr(sm), _lib.ptr(mean), starts, _lib.ptr(ws), ws.numel(),
+              ctypes.c_void_p(_lib.stream_handle(dev)))
+    full = {"sum": sm, "count": cnt, "mean": mean}
+    out = []
+    for i, q in enumerate(qs):
+        a, b = starts[i], starts[i + 1]
+        out.append((keys[a:b], {agg: full[agg][a:b] for agg in q.aggs}))
+    return out
 
 
 def run_filter_groupby(store: FrameStore, q: FilterGroupBy):
-    """filter -> groupby. On an indexed frame: one binary-search pass over the
-    groups (jfx_served_groups) and a compaction of the groups that keep rows.
+    """filter -> groupby. On an indexed frame: one launch that evaluates every
+    group by binary search and prefix sums and writes the groups that keep
+    rows compacted (jfx_served_batch with one query).
     Otherwise on the frame's key order: the rows passing the filter are
     already grouped, so a segmented reduction over them finishes the query
     (no sort)."""
-    from .compaction import OPS, compact_rows, filter_rows, take
+    from .compaction import filter_rows, take
     fr = store[q.frame]
     if fr.indexed and set(q.aggs) <= _INDEXED_AGGS:
-        if q.op not in OPS:
-            raise ValueError(f"unsupported comparison {q.op!r}")
-        dev = fr.keys.device
-        G = fr.ngroups
-        vt = _lib.JFX_F32 if fr.kv_values.dtype == torch.float32 else _lib.JFX_F64
-        cnt = torch.empty(G, dtype=torch.int64, device=dev)
-        sm = torch.empty(G, dtype=fr.kv_values.dtype, device=dev)
-        mean = torch.empty(G, dtype=fr.kv_values.dtype, device=dev)
-        mask = torch.empty(G, dtype=torch.uint8, device=dev)
-        val = (ctypes.c_float if vt == _lib.JFX_F32 else ctypes.c_double)(q.threshold)
-        _lib.call("jfx_served_groups", vt, _lib.ptr(fr.kv_values), _lib.ptr(fr.prefix),
-                  _lib.ptr(fr.group_starts), _lib.ptr(fr.group_nan_start),
-                  _lib.ptr(fr.group_ninf_end), _lib.ptr(fr.group_pinf_start), G, OPS[q.op],
-                  ctypes.byref(val), _lib.ptr(cnt), _lib.ptr(sm), _lib.ptr(mean), _lib.ptr(mask),
-                  ctypes.c_void_p(_lib.stream_handle(dev)))
-        pos = compact_rows(mask)
-        full = {"sum": sm, "count": cnt, "mean": mean}
-        return take(fr.group_keys, pos), {a: take(full[a], pos) for a in q.aggs}
+        return _served_batch(store, [q])[0]
     pos = filter_rows(fr.sorted_values, q.op, q.threshold)
     sk = take(fr.sorted_keys, pos)
     sv = take(fr.sorted_values, pos)
@@ -221,7 +255,7 @@ def run_filter_groupby(store: FrameStore, q: FilterGroupBy):
 def run_join(store: FrameStore, q: Join):
     """Inner join of a resident frame with a dimension table: only the
     dimension side is sorted; the frame's key order is reused."""
-    from .joining import gather
+    from .joining import fill_values, payload_source
     from .sorting import radix_sort
     if q.how != "inner":
         raise ValueError("served joins are inner joins")
@@ -241,67 +275,30 @@ def run_join(store: FrameStore, q: Join):
     _lib.call("jfx_join_count", kc, _lib.ptr(fr.sorted_keys), L, _lib.ptr(sd), R, ctypes.byref(m),
               _lib.ptr(ws), ws.numel(), stream)
     M = m.value
-    li = torch.empty(M, dtype=torch.int32, device=dev)
-    ri = torch.empty(M, dtype=torch.int32, device=dev)
-    keys = torch.empty(M, dtype=fr.keys.dtype, device=dev)
-    if M:
-        # left rows as sorted positions (iota): pairs keep the reference's
-        # (key, left row, right row) order because the key order is stable
-        _lib.call("jfx_join_fill", kc, _lib.ptr(fr.sorted_keys), _lib.ptr(fr.iota), L, _lib.ptr(sd),
-                  _lib.ptr(dperm), R, _lib.ptr(li), _lib.ptr(ri), _lib.ptr(keys), _lib.ptr(ws),
-                  ws.numel(), stream)
-    return keys, {"left_value": gather(fr.sorted_values, li),
-                  "right_value": gather(to_device(q.dim_values, dev), ri)}
+    # the frame's payload is already in key order: the fill reads it by sorted
+    # position (nearly sequential); the dimension payload is read by row
+    keys, lv, rv, _, _ = fill_values(kc, fr.sorted_keys, fr.perm, L, sd, dperm, R, M,
+                                     payload_source(fr.sorted_values, dev), False,
+                                     payload_source(q.dim_values, dev), True, ws,
+                                     keys_dtype=fr.keys.dtype)
+    return keys, {"left_value": lv, "right_value": rv}
 
 
 def execute_filter_batch(store: FrameStore, qs: list) -> list:
-    """Several indexed filter -> groupby queries as one batch: per-query
-    group passes into slices of shared buffers, ONE compaction of all the
-    groups that keep rows and one gather per output column, so the batch
-    costs a fixed number of host synchronisations however many queries it
-    holds. Results equal run_filter_groupby's query by query."""
-    from .compaction import OPS, compact_rows, take
+    """Several indexed filter -> groupby queries as one batch: one
+    jfx_served_batch launch evaluates every group of every query and writes
+    the groups that keep rows compacted, with one host synchronisation
+    however many queries the batch holds. Results equal run_filter_groupby's
+    query by query."""
     frs = [store[q.frame] for q in qs]
     kinds = {(fr.group_keys.dtype, fr.kv_values.dtype) for fr in frs}
-    if len(kinds) > 1:   # one sub-batch per (key type, value type)
-        out = [None] * len(qs)
-        for kind in kinds:
-            idx = [i for i, fr in enumerate(frs) if (fr.group_keys.dtype, fr.kv_values.dtype) == kind]
-            for i, r in zip(idx, execute_filter_batch(store, [qs[i] for i in idx])):
+    out = [None] * len(qs)
+    for kind in kinds:   # one launch per (key type, value type) and per 1024 queries
+        idx = [i for i, fr in enumerate(frs) if (fr.group_keys.dtype, fr.kv_values.dtype) == kind]
+        for a in range(0, len(idx), _SERVED_QMAX):
+            part = idx[a:a + _SERVED_QMAX]
+            for i, r in zip(part, _served_batch(store, [qs[i] for i in part])):
                 out[i] = r
-        return out
-    dev = frs[0].keys.device
-    vdt = frs[0].kv_values.dtype
-    Gs = [fr.ngroups for fr in frs]
-    offs = [0]
-    for G in Gs:
-        offs.append(offs[-1] + G)
-    tot = offs[-1]
-    cnt = torch.empty(tot, dtype=torch.int64, device=dev)
-    sm = torch.empty(tot, dtype=vdt, device=dev)
-    mean = torch.empty(tot, dtype=vdt, device=dev)
-    mask = torch.empty(tot, dtype=torch.uint8, device=dev)
-    vt = _lib.JFX_F32 if vdt == torch.float32 else _lib.JFX_F64
-    ctype = ctypes.c_float if vt == _lib.JFX_F32 else ctypes.c_double
-    stream = ctypes.c_void_p(_lib.stream_handle(dev))
-    for q, fr, o, G in zip(qs, frs, offs, Gs):
-        if q.op not in OPS:
-            raise ValueError(f"unsupported comparison {q.op!r}")
-        val = ctype(q.threshold)
-        _lib.call("jfx_served_groups", vt, _lib.ptr(fr.kv_values), _lib.ptr(fr.prefix),
-                  _lib.ptr(fr.group_starts), _lib.ptr(fr.group_nan_start),
-                  _lib.ptr(fr.group_ninf_end), _lib.ptr(fr.group_pinf_start), G, OPS[q.op],
-                  ctypes.byref(val), _lib.ptr(cnt[o:]), _lib.ptr(sm[o:]), _lib.ptr(mean[o:]),
-                  _lib.ptr(mask[o:]), stream)
-    pos = compact_rows(mask)
-    keys_all = torch.cat([fr.group_keys for fr in frs])
-    cuts = torch.searchsorted(pos, torch.tensor(offs, dtype=torch.int64, device=dev)).tolist()
-    k, c, s_, m = take(keys_all, pos), take(cnt, pos), take(sm, pos), take(mean, pos)
-    full = {"sum": s_, "count": c, "mean": m}
-    out = []
-    for i, q in enumerate(qs):
-        a, b = cuts[i], cuts[i + 1]
-        out.append((k[a:b], {agg: full[agg][a:b] for agg in q.aggs}))
     return out
 
 
@@ -361,6 +358,7 @@ class Server:
         self.batchable_fn = batchable_fn
         self.max_batch = max_batch
         self.batches = []   # sizes of the batches run
+        self.requests: list[Request] = []   # every request submitted, for timing reports
         self.footprint_fn = footprint_fn
         self.cv = threading.Condition()
         self.pending: deque[Request] = deque()
@@ -371,7 +369,10 @@ class Server:
         self.admitted_order: list[int] = []
         self.closed = False
         self._rid = 0
-        self.workers = [threading.Thread(target=self._worker, args=(i, use_streams), daemon=True)
+        # one stream per slot, made here so warmup() can prime each stream's
+        # allocator pool before requests arrive
+        self.streams = [torch.cuda.Stream() if use_streams else None for _ in range(slots)]
+        self.workers = [threading.Thread(target=self._worker, args=(i,), daemon=True)
                         for i in range(slots)]
         for w in self.workers:
             w.start()
@@ -387,6 +388,7 @@ class Server:
                     f"request {req.rid} needs {fp} B, more than the {self.budget} B budget"))
                 return req.future
             self.pending.append(req)
+            self.requests.append(req)
             self.cv.notify_all()
         return req.future
 
@@ -406,19 +408,26 @@ class Server:
         return head
 
     def _take_batch(self, head: Request) -> list:
-        # called with the lock held after head was admitted: the batchable
-        # requests queued right behind it join its batch while they fit
+        # called with the lock held after head (the FIFO head) was admitted:
+        # queued batchable requests join its batch, in arrival order, while
+        # they fit the budget. Requests they pass over keep their places; the
+        # FIFO head is always admitted first, so none of them starves.
         batch = [head]
         if self.batch_executor is None or not self.batchable_fn(self.store, head.query):
             return batch
-        while (self.pending and len(batch) < self.max_batch
-               and self.batchable_fn(self.store, self.pending[0].query)
-               and self.reserved + self.pending[0].footprint <= self.budget):
+        keep = deque()
+        while self.pending and len(batch) < self.max_batch:
             r = self.pending.popleft()
-            self.reserved += r.footprint
-            self.peak_reserved = max(self.peak_reserved, self.reserved)
-            self.admitted_order.append(r.rid)
-            batch.append(r)
+            if (self.batchable_fn(self.store, r.query)
+                    and self.reserved + r.footprint <= self.budget):
+                self.reserved += r.footprint
+                self.peak_reserved = max(self.peak_reserved, self.reserved)
+                self.admitted_order.append(r.rid)
+                batch.append(r)
+            else:
+                keep.append(r)
+        keep.extend(self.pending)
+        self.pending = keep
         return batch
 
     def _run(self, batch: list):
@@ -426,8 +435,23 @@ class Server:
             return [self.executor(self.store, batch[0].query)]
         return self.batch_executor(self.store, [r.query for r in batch])
 
-    def _worker(self, i: int, use_streams: bool):
-        stream = torch.cuda.Stream() if use_streams else None
+    def warmup(self, queries: list):
+        """Run queries once on every slot's stream (single queries and, for
+        the batchable ones, one batch), so the first requests do not pay for
+        device allocations; results are discarded."""
+        batchable_qs = [q for q in queries if self.batchable_fn(self.store, q)]
+        for s in self.streams:
+            if s is None:
+                continue
+            with torch.cuda.stream(s):
+                for q in queries:
+                    self.executor(self.store, q)
+                if self.batch_executor is not None and len(batchable_qs) > 1:
+                    self.batch_executor(self.store, batchable_qs)
+            s.synchronize()
+
+    def _worker(self, i: int):
+        stream = self.streams[i]
         while True:
             with self.cv:
                 req = self._admit()
@@ -438,6 +462,9 @@ class Server:
                     req = self._admit()
                 batch = self._take_batch(req)
                 self.batches.append(len(batch))
+            t_admit = time.perf_counter()
+            for r in batch:
+                r.t_admit = t_admit
             try:
                 if stream is not None:
                     with torch.cuda.stream(stream):
@@ -447,6 +474,7 @@ class Server:
                     results = self._run(batch)
                 done = time.perf_counter()
                 for r, res in zip(batch, results):
+                    r.t_done = done
                     r.future.set_result((res, done - r.arrival))
             except BaseException as e:  # delivered to the callers through the futures
                 for r in batch:

@@ -104,3 +104,70 @@ def test_poisson_arrivals_rate():
     a = poisson_arrivals(20000, qps=500.0, seed=1)
     assert all(x < y for x, y in zip(a, a[1:]))
     assert abs(len(a) / a[-1] - 500.0) / 500.0 < 0.05
+
+
+def _make_batching(slots=1, budget=10**9, max_batch=32):
+    runs = []
+
+    def ex(store, q):
+        time.sleep(q.dur)
+        runs.append([q.qid])
+        return q.qid
+
+    def bex(store, qs):
+        runs.append([q.qid for q in qs])
+        return [q.qid for q in qs]
+
+    srv = Server(FrameStore(budget), slots=slots, hbm_budget_bytes=budget, executor=ex,
+                 use_streams=False, footprint_fn=lambda st, q: q.fp, batch_executor=bex,
+                 batchable_fn=lambda st, q: getattr(q, "batch", False), max_batch=max_batch)
+    return srv, runs
+
+
+def _bq(qid, fp=1):
+    q = _Q(qid, fp, dur=0.0)
+    q.batch = True
+    return q
+
+
+def test_queued_batchable_requests_join_the_head_batch():
+    srv, runs = _make_batching(slots=1)
+    blocker = srv.submit(_Q(0, 1, dur=0.1))
+    time.sleep(0.02)                        # the blocker holds the only slot
+    futs = [srv.submit(_bq(1)), srv.submit(_Q(2, 1, dur=0.0)), srv.submit(_bq(3)),
+            srv.submit(_bq(4))]
+    assert [f.result()[0] for f in [blocker] + futs] == [0, 1, 2, 3, 4]
+    srv.close()
+    # the head batch took 1, 3, 4 past the non-batchable 2, which ran next
+    assert runs == [[0], [1, 3, 4], [2]]
+    assert srv.batches == [1, 3, 1]
+
+
+def test_batch_respects_budget_and_max_batch():
+    srv, runs = _make_batching(slots=1, budget=100, max_batch=3)
+    blocker = srv.submit(_Q(0, 10, dur=0.1))
+    time.sleep(0.02)
+    futs = [srv.submit(_bq(i, fp=40)) for i in range(1, 6)]
+    for f in [blocker] + futs:
+        f.result()
+    srv.close()
+    assert srv.peak_reserved <= 100
+    assert all(len(r) <= 3 for r in runs)
+    assert [i for r in runs for i in r] == [0, 1, 2, 3, 4, 5]   # arrival order kept
+
+
+def test_batch_failure_reaches_every_member():
+    def bex(store, qs):
+        raise RuntimeError("batch failed")
+    srv = Server(FrameStore(100), slots=1, hbm_budget_bytes=100, executor=lambda st, q: time.sleep(0.1),
+                 use_streams=False, footprint_fn=lambda st, q: 1, batch_executor=bex,
+                 batchable_fn=lambda st, q: getattr(q, "batch", False))
+    blocker = srv.submit(_Q(0, 1))
+    time.sleep(0.02)
+    futs = [srv.submit(_bq(i)) for i in (1, 2)]
+    blocker.result()
+    for f in futs:
+        with pytest.raises(RuntimeError):
+            f.result()
+    assert srv.running == 0 and srv.reserved == 0
+    srv.close()

@@ -176,3 +176,68 @@ def test_server_batches_queued_filter_queries(store):
     assert srv.admitted_order == sorted(srv.admitted_order)
     for a, r in zip(alone, res):
         _same(a, r, exact_sums=True)
+
+
+def _served_groups_ref(fr, op, t):
+    """jfx_served_groups on every group, then the groups with a passing row."""
+    import ctypes
+    from jaxframes_amd import _lib
+    from jaxframes_amd.compaction import OPS
+    G = fr.ngroups
+    dev = fr.keys.device
+    cnt = torch.empty(G, dtype=torch.int64, device=dev)
+    sm = torch.empty(G, dtype=fr.kv_values.dtype, device=dev)
+    mean = torch.empty(G, dtype=fr.kv_values.dtype, device=dev)
+    mask = torch.empty(G, dtype=torch.uint8, device=dev)
+    vt = _lib.JFX_F32 if fr.kv_values.dtype == torch.float32 else _lib.JFX_F64
+    val = (ctypes.c_float if vt == _lib.JFX_F32 else ctypes.c_double)(t)
+    _lib.call("jfx_served_groups", vt, _lib.ptr(fr.kv_values), _lib.ptr(fr.prefix),
+              _lib.ptr(fr.group_starts), _lib.ptr(fr.group_nan_start), _lib.ptr(fr.group_ninf_end),
+              _lib.ptr(fr.group_pinf_start), G, OPS[op], ctypes.byref(val), _lib.ptr(cnt),
+              _lib.ptr(sm), _lib.ptr(mean), _lib.ptr(mask), 0)
+    keep = mask.bool()
+    return fr.group_keys[keep], cnt[keep], sm[keep], mean[keep]
+
+
+def test_served_batch_abi_with_empty_queries(store):
+    """jfx_served_batch equals jfx_served_groups + compaction per query, with
+    queries of no groups first, in the middle and last."""
+    import ctypes
+    from jaxframes_amd import _lib
+    from jaxframes_amd.compaction import OPS
+    spec = [(None, ">", 0.0), ("f0", ">", 0.3), (None, "<", 0.0), ("f1", "<=", -0.2),
+            ("f2", "!=", 0.5), ("f1", "==", 0.0), (None, ">", 1.0)]
+    Q = len(spec)
+    table = (serving.ServedQuery * Q)()
+    tot = 0
+    for i, (name, op, t) in enumerate(spec):
+        if name is None:
+            table[i] = serving.ServedQuery(0, 0, 0, 0, 0, 0, 0, 0, OPS[op], t)
+        else:
+            fr = store[name]
+            table[i] = serving.ServedQuery(*fr.index_ptrs, fr.ngroups, OPS[op], t)
+            tot += fr.ngroups
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_served_batch_workspace_size", Q, tot, ctypes.byref(nb))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    keys = torch.empty(tot, dtype=torch.int64, device="cuda")
+    cnt = torch.empty(tot, dtype=torch.int64, device="cuda")
+    sm = torch.empty(tot, dtype=torch.float32, device="cuda")
+    mean = torch.empty(tot, dtype=torch.float32, device="cuda")
+    starts = (ctypes.c_uint64 * (Q + 1))()
+    _lib.call("jfx_served_batch", _lib.JFX_F32, 8, table, Q, _lib.ptr(keys), _lib.ptr(cnt),
+              _lib.ptr(sm), _lib.ptr(mean), starts, _lib.ptr(ws), ws.numel(), 0)
+    st = list(starts)
+    assert st[0] == 0 and all(a <= b for a, b in zip(st, st[1:]))
+    for i, (name, op, t) in enumerate(spec):
+        a, b = st[i], st[i + 1]
+        if name is None:
+            assert a == b
+            continue
+        ek, ec, es, em = _served_groups_ref(store[name], op, t)
+        assert torch.equal(keys[a:b], ek)
+        assert torch.equal(cnt[a:b], ec)
+        assert torch.equal(sm[a:b], es)
+        assert torch.equal(mean[a:b], em)
+    assert st[Q] == sum(int(_served_groups_ref(store[n], o, t)[0].shape[0])
+                        for n, o, t in spec if n is not None)
