@@ -195,6 +195,46 @@ def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
                                "keys_per_s": round(2e8 / pdt, 1)}}
 
 
+def only_workload(name: str, reps: int) -> dict:
+    """One workload by itself, for profiling: the kernels it launches, with
+    their HIP-event time per call, and the wall time per call."""
+    import torch
+    from jaxframes_amd import _lib, datagen, joining, partition, sorting
+    if name == "batched":
+        return {"only": name, **batched_vs_plain(400_000_000)}
+    if name == "served":
+        return {"only": name, **served_vs_plain()}
+    if name == "sort":
+        keys = datagen.sort_keys(100_000_000)
+        desc = "radix sort of 100M uniform uint64 keys with the uint32 row index"
+        step = lambda: sorting.radix_sort(keys, None, True, return_index=True,
+                                          index_dtype=torch.int32)
+    elif name == "join":
+        lk, lv, rk, rv, m = datagen.join_tables()
+        desc = (f"inner join of 1e8 x 1e7 int64 keys (right keys twice each, 80% of left rows "
+                f"match by Zipf(1.2)), 2 float32 payloads per side, {m} pairs")
+        step = lambda: joining.sort_merge_join(lk, lv, rk, rv, "inner")
+    elif name == "partition":
+        keys = datagen.partition_keys(200_000_000)
+        desc = "hash partition of 2e8 uniform int64 keys into 256 buckets (fmix32), keys only"
+        step = lambda: partition.hash_partition(keys, 256, return_keys=True, return_rows=False)
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    step()
+    torch.cuda.synchronize()
+    with _lib.KernelTimer() as kt:
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+    ks = kt.stats()
+    return {"only": name, "workload": desc, "ms_per_call": round(dt * 1e3, 3), "reps": reps,
+            "kernels": {k: {"us_per_call": round(v[0] / reps * 1e3, 1),
+                            "launches_per_call": v[1] // reps}
+                        for k, v in sorted(ks.items(), key=lambda kv: -kv[1][0])}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +246,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-rows", type=int, default=400_000_000)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--only", choices=("sort", "join", "partition", "batched", "served"),
+                    help="run one other workload by itself (for profiling) and print its line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 400 MB (float32[1e8]) so that PMC byte "
                          "counters can be scaled against a transfer of known size")
@@ -220,6 +262,9 @@ def main() -> int:
 
     _lib.require_gpu()
     dev = torch.device("cuda", 0)
+    if args.only:
+        print(json.dumps(only_workload(args.only, max(args.steps // 4, 3))), flush=True)
+        return 0
     if args.copy_calibration:
         a = torch.ones(100_000_000, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
