@@ -242,6 +242,28 @@ int jfx_served_groups(int val_type, const void* vals, const double* prefix, cons
                       int64_t* out_count, void* out_sum, void* out_mean, uint8_t* out_mask,
                       void* stream);
 
+/* ---- served join: a resident frame's key groups against a dimension table -- */
+/* Inner join of a frame, given by its key groups (group_keys ascending in the
+ * key order, group g's rows at key-order positions [group_starts[g],
+ * group_starts[g+1])), with n_dim dimension rows: the pairs of
+ * ParallelSortMergeJoin.join (parallel_algorithms.py:938-1075) in its (key,
+ * left row, right row) order (pa:987-994), left rows taken in key order.
+ * Integer keys (JFX_U32/I32/U64/I64). jfx_served_join_count maps the
+ * dimension rows to groups, sorts them by group (fixed plan, no host read),
+ * counts the pairs into *out_pairs and synchronises the stream once;
+ * jfx_served_join_fill then writes, for every pair, the key, the frame's
+ * payload (left_values in key order) and the dimension payload (dim_values
+ * by row), on the same workspace. No counterpart in the reference. */
+int jfx_served_join_workspace_size(uint64_t n_dim, uint32_t ngroups, uint64_t* out_bytes);
+int jfx_served_join_count(int key_type, const void* group_keys, const uint32_t* group_starts,
+                          uint32_t ngroups, const void* dim_keys, uint64_t n_dim,
+                          uint64_t* out_pairs, void* workspace, uint64_t ws_bytes, void* stream);
+int jfx_served_join_fill(int key_type, const void* group_keys, const uint32_t* group_starts,
+                         uint32_t ngroups, int left_elem_bytes, const void* left_values,
+                         uint64_t n_dim, int right_elem_bytes, const void* dim_values,
+                         uint64_t n_pairs, void* out_keys, void* out_left, void* out_right,
+                         void* workspace, uint64_t ws_bytes, void* stream);
+
 /* One indexed filter -> groupby query of a batch: the index columns of
  * jfx_served_groups, the frame's group keys (in group order) and the
  * comparison. threshold is rounded to the value type. */

@@ -68,6 +68,11 @@ _SIGNATURES = {
     "jfx_batched_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, ctypes.c_uint32, c_u64,
                             ctypes.c_uint32, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_u64, c_vp],
+    "jfx_served_join_workspace_size": [c_u64, ctypes.c_uint32, c_u64p],
+    "jfx_served_join_count": [c_int, c_vp, c_vp, ctypes.c_uint32, c_vp, c_u64, c_u64p, c_vp, c_u64,
+                              c_vp],
+    "jfx_served_join_fill": [c_int, c_vp, c_vp, ctypes.c_uint32, c_int, c_vp, c_u64, c_int, c_vp,
+                             c_u64, c_vp, c_vp, c_vp, c_vp, c_u64, c_vp],
     "jfx_served_batch_workspace_size": [ctypes.c_uint32, c_u64, c_u64p],
     "jfx_served_batch": [c_int, c_int, c_vp, ctypes.c_uint32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                          c_u64, c_vp],

@@ -17,8 +17,10 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "blockscan.hpp"
 #include "common.hpp"
 #include "runtime.hpp"
+#include "sortctx.hpp"
 
 namespace jfx {
 
@@ -256,6 +258,273 @@ int served_batch_typed(const SvWs& w, uint32_t Q, uint64_t tot, void* ok, int64_
 
 }  // namespace
 
+// ---- served join: a resident frame's group index against a dimension table --
+// The frame is indexed by its key groups (group_keys ascending, group g's rows
+// at sorted positions [starts[g], starts[g+1])), so the join runs over groups
+// and dimension rows instead of frame rows:
+//   k_sj_code   dimension row -> its group (binary search in group_keys), or G
+//   (radix sort of the codes with the row index, fixed plan: codes < 2^bits)
+//   k_sj_runs   first and end sorted dimension position of every group's run
+//   k_sj_tiles  per tile of SJ_TILE groups: pairs = group rows x dimension rows
+//   k_join_scan tile offsets and the pair count M
+//   k_sj_offs   every group's first output position (off[G] = M)
+//   k_sj_fill   SJ_CHUNK outputs per workgroup: output k of group g is left
+//               position starts[g] + q / c, dimension row of sorted position
+//               dfirst[g] + q % c (q = k - off[g], c = the group's dimension
+//               rows): (key, left row, right row) order, the reference's
+//               pair order (parallel_algorithms.py:987-994).
+constexpr uint32_t SJ_TILE = 1024;
+constexpr uint32_t SJ_CHUNK = 2048;
+
+template <typename K>
+__global__ void __launch_bounds__(256)
+k_sj_code(const K* gk, uint32_t G, const K* dk, uint64_t R, int oc, uint32_t* code) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < R; i += stride) {
+    const K x = ordered(dk[i], oc);
+    uint32_t lo = 0, hi = G;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ordered(gk[mid], oc) < x) lo = mid + 1;
+      else hi = mid;
+    }
+    code[i] = (lo < G && ordered(gk[lo], oc) == x) ? lo : G;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_sj_runs(const uint32_t* sc, uint64_t R, uint32_t G, uint32_t* dfirst, uint32_t* dend) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < R; j += stride) {
+    const uint32_t c = sc[j];
+    if (c >= G) continue;
+    if (j == 0 || sc[j - 1] != c) dfirst[c] = (uint32_t)j;
+    if (j + 1 == R || sc[j + 1] != c) dend[c] = (uint32_t)(j + 1);
+  }
+}
+
+__device__ __forceinline__ uint64_t sj_pairs(const uint32_t* starts, const uint32_t* dfirst,
+                                             const uint32_t* dend, uint32_t g) {
+  return (uint64_t)(starts[g + 1] - starts[g]) * (uint64_t)(dend[g] - dfirst[g]);
+}
+
+// per-thread groups tid*4 .. tid*4+3 of the tile; returns the thread's sum
+__device__ __forceinline__ uint64_t sj_thread_pairs(const uint32_t* starts, const uint32_t* dfirst,
+                                                    const uint32_t* dend, uint32_t G, uint32_t g0,
+                                                    uint64_t* p) {
+  uint64_t t = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t g = g0 + threadIdx.x * 4 + j;
+    p[j] = g < G ? sj_pairs(starts, dfirst, dend, g) : 0ull;
+    t += p[j];
+  }
+  return t;
+}
+
+__global__ void __launch_bounds__(256)
+k_sj_tiles(const uint32_t* starts, const uint32_t* dfirst, const uint32_t* dend, uint32_t G,
+           uint64_t* tile_sum) {
+  __shared__ uint64_t s_w[4];
+  uint64_t p[4];
+  uint64_t t = sj_thread_pairs(starts, dfirst, dend, G, blockIdx.x * SJ_TILE, p);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ void __launch_bounds__(256)
+k_sj_offs(const uint32_t* starts, const uint32_t* dfirst, const uint32_t* dend, uint32_t G,
+          const uint64_t* tile_off, const uint64_t* total, uint64_t* off) {
+  __shared__ uint64_t s_w[5];
+  uint64_t p[4];
+  const uint32_t g0 = blockIdx.x * SJ_TILE;
+  const uint64_t t = sj_thread_pairs(starts, dfirst, dend, G, g0, p);
+  const uint64_t incl = wave_incl_scan64(t);
+  if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+  __syncthreads();
+  uint64_t wbase = 0;
+  for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) wbase += s_w[w];
+  uint64_t run = tile_off[blockIdx.x] + wbase + incl - t;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t g = g0 + threadIdx.x * 4 + j;
+    if (g < G) off[g] = run;
+    run += p[j];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) off[G] = *total;
+}
+
+// Exclusive scan of n tile sums in place by one workgroup of 1024 threads;
+// the total goes to *total.
+__global__ void __launch_bounds__(1024) k_sj_scan(uint64_t* c, uint32_t n, uint64_t* total) {
+  __shared__ uint64_t s_w[17];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per = (n + 1023u) / 1024u;
+  const uint32_t a = tid * per, b = (a + per) < n ? (a + per) : n;
+  uint64_t sum = 0;
+  for (uint32_t i = a; i < b; ++i) sum += c[i];
+  const uint64_t incl = wave_incl_scan64(sum);
+  if ((tid & 63) == 63) s_w[tid >> 6] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    uint64_t acc = 0;
+    for (int w = 0; w < 16; ++w) {
+      const uint64_t v = s_w[w];
+      s_w[w] = acc;
+      acc += v;
+    }
+    s_w[16] = acc;
+  }
+  __syncthreads();
+  uint64_t run = s_w[tid >> 6] + incl - sum;
+  for (uint32_t i = a; i < b; ++i) {
+    const uint64_t v = c[i];
+    c[i] = run;
+    run += v;
+  }
+  if (tid == 0) *total = s_w[16];
+}
+
+template <typename K, typename LT, typename RT>
+__global__ void __launch_bounds__(256)
+k_sj_fill(const K* gk, const uint32_t* starts, const uint32_t* dfirst, const uint32_t* dend,
+          const uint32_t* dperm, const uint64_t* off, uint32_t G, uint64_t M, const LT* lv,
+          const RT* rv, K* out_key, LT* out_lv, RT* out_rv) {
+  __shared__ uint32_t s_g[2];
+  const uint64_t k0 = (uint64_t)blockIdx.x * SJ_CHUNK;
+  const uint64_t k1 = k0 + SJ_CHUNK < M ? k0 + SJ_CHUNK : M;
+  if (threadIdx.x < 2) {  // last group whose first output is <= k0 (resp. k1 - 1)
+    const uint64_t kk = threadIdx.x == 0 ? k0 : k1 - 1;
+    uint32_t lo = 0, hi = G + 1;
+    while (lo + 1 < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (off[mid] <= kk) lo = mid;
+      else hi = mid;
+    }
+    s_g[threadIdx.x] = lo;
+  }
+  __syncthreads();
+  const uint32_t ga = s_g[0], gb = s_g[1];
+  for (uint64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+    uint32_t g = ga;
+    if (ga != gb) {
+      uint32_t lo = ga, hi = gb + 1;
+      while (lo + 1 < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= k) lo = mid;
+        else hi = mid;
+      }
+      g = lo;
+    }
+    const uint64_t q = k - off[g];
+    const uint32_t df = dfirst[g], c = dend[g] - df;
+    uint64_t pidx = q, d = 0;
+    if (c != 1) {
+      pidx = q / c;
+      d = q - pidx * c;
+    }
+    out_key[k] = gk[g];
+    out_lv[k] = lv[starts[g] + pidx];
+    out_rv[k] = rv[dperm[df + d]];
+  }
+}
+
+namespace {
+
+struct SjWs {
+  uint32_t* code;
+  uint32_t* scode;
+  uint32_t* dperm;
+  uint32_t* dfirst;   // dfirst[G] then dend[G], zeroed together
+  uint32_t* dend;
+  uint64_t* tile_off;
+  uint64_t* total;
+  uint64_t* off;
+  void* sort_ws;
+  uint64_t sort_bytes;
+};
+
+uint32_t sj_bits(uint32_t G) {
+  uint32_t b = 1;
+  while (b < 32 && (G >> b) != 0) ++b;
+  return b;
+}
+
+SjWs carve_sj(Carver& c, uint64_t R, uint32_t G) {
+  SjWs w{};
+  const uint64_t r = R ? R : 1;
+  w.code = c.take<uint32_t>(r);
+  w.scode = c.take<uint32_t>(r);
+  w.dperm = c.take<uint32_t>(r);
+  w.dfirst = c.take<uint32_t>(2ull * G + 2);
+  w.dend = w.dfirst + G + 1;
+  const uint64_t nt = (G + SJ_TILE - 1) / SJ_TILE;
+  w.tile_off = c.take<uint64_t>(nt + 1);
+  w.total = c.take<uint64_t>(2);
+  w.off = c.take<uint64_t>((uint64_t)G + 1);
+  w.sort_bytes = sort_ws_bytes(JFX_U32, 4, r);
+  w.sort_ws = c.take<char>(w.sort_bytes);
+  return w;
+}
+
+template <typename K>
+int sj_count_typed(int oc, const void* gk, const uint32_t* starts, uint32_t G, const void* dk,
+                   uint64_t R, const SjWs& w, uint64_t* out_m, hipStream_t s) {
+  const uint32_t cb = (uint32_t)std::min<uint64_t>(4096, (R + 255) / 256);
+  JFX_LAUNCH("sj_code", s, (k_sj_code<K><<<cb, 256, 0, s>>>((const K*)gk, G, (const K*)dk, R, oc,
+                                                            w.code)));
+  JFX_HIP(hipGetLastError());
+  SortCtx ctx;
+  int rc = sort_prepare_known(ctx, JFX_U32, w.code, R, 4, 0, sj_bits(G), w.sort_ws, w.sort_bytes, s);
+  if (rc != JFX_OK) return rc;
+  rc = sort_execute(ctx, w.code, w.scode, 1, nullptr, w.dperm, 1, 0);
+  if (rc != JFX_OK) return rc;
+  JFX_LAUNCH("sj_zero", s, (k_served_zero<<<64, 256, 0, s>>>(w.dfirst, 2ull * G + 2)));
+  JFX_HIP(hipGetLastError());
+  JFX_LAUNCH("sj_runs", s, (k_sj_runs<<<cb, 256, 0, s>>>(w.scode, R, G, w.dfirst, w.dend)));
+  JFX_HIP(hipGetLastError());
+  const uint32_t nt = (G + SJ_TILE - 1) / SJ_TILE;
+  JFX_LAUNCH("sj_tiles", s, (k_sj_tiles<<<nt, 256, 0, s>>>(starts, w.dfirst, w.dend, G, w.tile_off)));
+  JFX_HIP(hipGetLastError());
+  JFX_LAUNCH("sj_scan", s, (k_sj_scan<<<1, 1024, 0, s>>>(w.tile_off, nt, w.total)));
+  JFX_HIP(hipGetLastError());
+  JFX_LAUNCH("sj_offs", s, (k_sj_offs<<<nt, 256, 0, s>>>(starts, w.dfirst, w.dend, G, w.tile_off,
+                                                         w.total, w.off)));
+  JFX_HIP(hipGetLastError());
+  uint64_t* h = static_cast<uint64_t*>(pinned_scratch(sizeof(uint64_t)));
+  JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
+  JFX_HIP(hipMemcpyAsync(h, w.total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  JFX_HIP(hipStreamSynchronize(s));
+  *out_m = *h;
+  return JFX_OK;
+}
+
+template <typename K, typename LT, typename RT>
+int sj_fill_typed(const void* gk, const uint32_t* starts, uint32_t G, const SjWs& w, uint64_t M,
+                  const void* lv, const void* rv, void* ok, void* ol, void* orr, hipStream_t s) {
+  const uint64_t nb = (M + SJ_CHUNK - 1) / SJ_CHUNK;
+  JFX_LAUNCH("sj_fill", s, (k_sj_fill<K, LT, RT><<<(uint32_t)nb, 256, 0, s>>>(
+                               (const K*)gk, starts, w.dfirst, w.dend, w.dperm, w.off, G, M,
+                               (const LT*)lv, (const RT*)rv, (K*)ok, (LT*)ol, (RT*)orr)));
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
+
+template <typename K>
+int sj_fill_key(int lb, int rb, const void* gk, const uint32_t* starts, uint32_t G, const SjWs& w,
+                uint64_t M, const void* lv, const void* rv, void* ok, void* ol, void* orr,
+                hipStream_t s) {
+  if (lb == 4 && rb == 4) return sj_fill_typed<K, uint32_t, uint32_t>(gk, starts, G, w, M, lv, rv, ok, ol, orr, s);
+  if (lb == 4) return sj_fill_typed<K, uint32_t, uint64_t>(gk, starts, G, w, M, lv, rv, ok, ol, orr, s);
+  if (rb == 4) return sj_fill_typed<K, uint64_t, uint32_t>(gk, starts, G, w, M, lv, rv, ok, ol, orr, s);
+  return sj_fill_typed<K, uint64_t, uint64_t>(gk, starts, G, w, M, lv, rv, ok, ol, orr, s);
+}
+
+}  // namespace
+
 }  // namespace jfx
 
 extern "C" {
@@ -289,6 +558,67 @@ int jfx_served_groups(int val_type, const void* vals, const double* prefix, cons
   }
   JFX_HIP(hipGetLastError());
   return JFX_OK;
+}
+
+int jfx_served_join_workspace_size(uint64_t n_dim, uint32_t ngroups, uint64_t* out_bytes) {
+  JFX_CHECK(out_bytes != nullptr, JFX_ERR_INVALID, "out_bytes is null");
+  jfx::Carver c(nullptr, 0);
+  jfx::carve_sj(c, n_dim, ngroups);
+  *out_bytes = c.used();
+  return JFX_OK;
+}
+
+int jfx_served_join_count(int key_type, const void* group_keys, const uint32_t* group_starts,
+                          uint32_t ngroups, const void* dim_keys, uint64_t n_dim,
+                          uint64_t* out_pairs, void* workspace, uint64_t ws_bytes, void* stream) {
+  JFX_CHECK(key_type == JFX_U32 || key_type == JFX_I32 || key_type == JFX_U64 ||
+                key_type == JFX_I64,
+            JFX_ERR_UNSUPPORTED, "served joins take integer keys");
+  JFX_CHECK(out_pairs != nullptr, JFX_ERR_INVALID, "out_pairs is null");
+  JFX_CHECK(n_dim < (1ull << 32), JFX_ERR_UNSUPPORTED, "the dimension table must have < 2^32 rows");
+  JFX_CHECK(ngroups < 0xFFFFFFFFu, JFX_ERR_UNSUPPORTED, "too many groups");
+  *out_pairs = 0;
+  if (n_dim == 0 || ngroups == 0) return JFX_OK;
+  JFX_CHECK(group_keys && group_starts && dim_keys && workspace, JFX_ERR_INVALID, "null argument");
+  jfx::Carver c(workspace, ws_bytes);
+  jfx::SjWs w = jfx::carve_sj(c, n_dim, ngroups);
+  JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "served join workspace too small");
+  const int oc = jfx::order_code(key_type);
+  hipStream_t s = jfx::as_stream(stream);
+  if (jfx::key_bytes(key_type) == 4)
+    return jfx::sj_count_typed<uint32_t>(oc, group_keys, group_starts, ngroups, dim_keys, n_dim, w,
+                                         out_pairs, s);
+  return jfx::sj_count_typed<uint64_t>(oc, group_keys, group_starts, ngroups, dim_keys, n_dim, w,
+                                       out_pairs, s);
+}
+
+int jfx_served_join_fill(int key_type, const void* group_keys, const uint32_t* group_starts,
+                         uint32_t ngroups, int left_elem_bytes, const void* left_values,
+                         uint64_t n_dim, int right_elem_bytes, const void* dim_values,
+                         uint64_t n_pairs, void* out_keys, void* out_left, void* out_right,
+                         void* workspace, uint64_t ws_bytes, void* stream) {
+  JFX_CHECK(key_type == JFX_U32 || key_type == JFX_I32 || key_type == JFX_U64 ||
+                key_type == JFX_I64,
+            JFX_ERR_UNSUPPORTED, "served joins take integer keys");
+  JFX_CHECK((left_elem_bytes == 4 || left_elem_bytes == 8) &&
+                (right_elem_bytes == 4 || right_elem_bytes == 8),
+            JFX_ERR_UNSUPPORTED, "payload elements must be 4 or 8 bytes");
+  if (n_pairs == 0) return JFX_OK;
+  JFX_CHECK(group_keys && group_starts && left_values && dim_values && out_keys && out_left &&
+                out_right && workspace,
+            JFX_ERR_INVALID, "null argument");
+  JFX_CHECK(n_pairs / jfx::SJ_CHUNK < 0xFFFFFFFFull, JFX_ERR_UNSUPPORTED, "too many pairs");
+  jfx::Carver c(workspace, ws_bytes);
+  jfx::SjWs w = jfx::carve_sj(c, n_dim, ngroups);
+  JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "served join workspace too small");
+  hipStream_t s = jfx::as_stream(stream);
+  if (jfx::key_bytes(key_type) == 4)
+    return jfx::sj_fill_key<uint32_t>(left_elem_bytes, right_elem_bytes, group_keys, group_starts,
+                                      ngroups, w, n_pairs, left_values, dim_values, out_keys,
+                                      out_left, out_right, s);
+  return jfx::sj_fill_key<uint64_t>(left_elem_bytes, right_elem_bytes, group_keys, group_starts,
+                                    ngroups, w, n_pairs, left_values, dim_values, out_keys, out_left,
+                                    out_right, s);
 }
 
 int jfx_served_batch_workspace_size(uint32_t nqueries, uint64_t total_groups,

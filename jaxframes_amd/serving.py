@@ -19,6 +19,7 @@
 from __future__ import annotations
 
 import ctypes
+import sys
 import threading
 import time
 from collections import deque
@@ -57,6 +58,17 @@ class Frame:
         self.perm = perm                                  # uint32 row index (int32 storage)
         self.sorted_values = self.values[perm.long()] if self.n else self.values
         parts = [self.keys, self.values, self.sorted_keys, self.perm, self.sorted_values]
+        self.ngroups = 0
+        if self.n:
+            # key groups of the key order: joins run over them
+            gid, starts = multikey_heads([sk])
+            G = starts.shape[0]
+            self.ngroups = G
+            self.group_keys = sk[starts]
+            self.group_starts = torch.empty(G + 1, dtype=torch.int32, device=dev)
+            self.group_starts[:G] = starts.to(torch.int32)
+            self.group_starts[G] = self.n
+            parts += [self.group_keys, self.group_starts]
         self.indexed = self.values.dtype in (torch.float32, torch.float64) and self.n > 0
         if self.indexed:
             # (key, value) order: stable by value, then stable by key
@@ -64,12 +76,6 @@ class Frame:
             _, _, p2 = radix_sort(self.keys[pv], None, True, want_keys=False, return_index=True)
             kv = pv[p2]
             self.kv_values = self.values[kv]
-            gid, starts = multikey_heads([sk])
-            G = starts.shape[0]
-            self.group_keys = sk[starts]
-            self.group_starts = torch.empty(G + 1, dtype=torch.int32, device=dev)
-            self.group_starts[:G] = starts.to(torch.int32)
-            self.group_starts[G] = self.n
             nan = torch.isnan(self.kv_values)
             nn = torch.bincount(gid[nan], minlength=G)
             npos = torch.bincount(gid[torch.isposinf(self.kv_values)], minlength=G)
@@ -83,14 +89,13 @@ class Frame:
                               torch.zeros_like(self.kv_values))
             self.prefix = torch.zeros(self.n + 1, dtype=torch.float64, device=dev)
             self.prefix[1:] = torch.cumsum(fin.double(), 0)
-            self.ngroups = G
             # the index columns in jfx_served_query field order
             self.index_ptrs = (self.kv_values.data_ptr(), self.prefix.data_ptr(),
                                self.group_starts.data_ptr(), self.group_nan_start.data_ptr(),
                                self.group_ninf_end.data_ptr(), self.group_pinf_start.data_ptr(),
                                self.group_keys.data_ptr())
-            parts += [self.kv_values, self.group_keys, self.group_starts, self.group_nan_start,
-                      self.group_pinf_start, self.group_ninf_end, self.prefix]
+            parts += [self.kv_values, self.group_nan_start, self.group_pinf_start,
+                      self.group_ninf_end, self.prefix]
         self.nbytes = sum(t.numel() * t.element_size() for t in parts)
 
 
@@ -152,14 +157,14 @@ def footprint(store: FrameStore, q) -> int:
     kb, vb = fr.keys.element_size(), fr.values.element_size()
     if isinstance(q, FilterGroupBy):
         if fr.indexed:
-            # per-group count / sum / mean / mask, positions, the gathered groups
-            return fr.ngroups * (8 + 2 * vb + 1 + 4 + kb + 8 + 2 * vb) + (1 << 20)
+            # per-group key, count, sum and mean outputs, lookback scratch
+            return fr.ngroups * (kb + 8 + 2 * vb + 1) + (1 << 20)
         # positions, gathered keys + values, group outputs, segreduce scratch
         return fr.n * (4 + kb + vb) + fr.n * (kb + 3 * 8) + fr.n * 16 + (1 << 22)
     nd = q.dim_keys.shape[0]
-    # sorted dimension side, join scratch; pairs bounded by n * duplicates of
-    # the dimension keys (unique dimension keys: at most one pair per row)
-    return nd * (2 * kb + 8 + 4) + fr.n * (4 + 4 + kb + 2 * vb) + (1 << 22)
+    # dimension codes, their sort and per-group runs and offsets; the pairs
+    # (key + 2 payloads) are bounded by n when the dimension keys are unique
+    return nd * (3 * 4 + 2 * 8) + fr.ngroups * (8 + 8) + fr.n * (kb + 2 * vb) + (1 << 22)
 
 
 # ---- executors -----------------------------------------------------------------------
@@ -264,8 +269,10 @@ def run_join(store: FrameStore, q: Join):
     dk, _ = widen_key(to_device(q.dim_keys, dev))
     if dk.dtype != fr.keys.dtype:
         dk = dk.to(fr.keys.dtype)
-    sd, _, dperm = radix_sort(dk, None, True, return_index=True, index_dtype=torch.int32)
     kc = key_code(fr.keys)
+    if fr.ngroups and kc in _INT_KEYS:
+        return _join_by_groups(fr, dk, q.dim_values, kc)
+    sd, _, dperm = radix_sort(dk, None, True, return_index=True, index_dtype=torch.int32)
     L, R = fr.n, dk.shape[0]
     nb = ctypes.c_uint64(0)
     _lib.call("jfx_join_workspace_size", L, ctypes.byref(nb))
@@ -277,10 +284,43 @@ def run_join(store: FrameStore, q: Join):
     M = m.value
     # the frame's payload is already in key order: the fill reads it by sorted
     # position (nearly sequential); the dimension payload is read by row
-    keys, lv, rv, _, _ = fill_values(kc, fr.sorted_keys, fr.perm, L, sd, dperm, R, M,
+    keys, lv, rv, _, _ = fill_values(kc, fr.sorted_keys, None, L, sd, dperm, R, M,
                                      payload_source(fr.sorted_values, dev), False,
                                      payload_source(q.dim_values, dev), True, ws,
                                      keys_dtype=fr.keys.dtype)
+    return keys, {"left_value": lv, "right_value": rv}
+
+
+_INT_KEYS = (_lib.JFX_U32, _lib.JFX_I32, _lib.JFX_U64, _lib.JFX_I64)
+
+
+def _join_by_groups(fr: Frame, dk: torch.Tensor, dim_values, kc: int):
+    """Inner join over the frame's key groups (jfx_served_join_count / _fill):
+    dimension rows are mapped to groups and sorted by group with a fixed
+    plan; each group's rows are contiguous in key order, so the fill streams
+    the frame's key-ordered payload."""
+    from .joining import payload_source
+    dev = dk.device
+    R, G = dk.shape[0], fr.ngroups
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_served_join_workspace_size", R, G, ctypes.byref(nb))
+    ws = _lib.workspace(nb.value, dev)
+    stream = ctypes.c_void_p(_lib.stream_handle(dev))
+    m = ctypes.c_uint64(0)
+    _lib.call("jfx_served_join_count", kc, _lib.ptr(fr.group_keys), _lib.ptr(fr.group_starts), G,
+              _lib.ptr(dk), R, ctypes.byref(m), _lib.ptr(ws), ws.numel(), stream)
+    M = m.value
+    lsrc = payload_source(fr.sorted_values, dev)
+    rsrc = payload_source(dim_values, dev)
+    if rsrc.shape[0] != R:
+        raise ValueError("dimension keys and values must have the same length")
+    keys = torch.empty(M, dtype=fr.keys.dtype, device=dev)
+    lv = torch.empty(M, dtype=lsrc.dtype, device=dev)
+    rv = torch.empty(M, dtype=rsrc.dtype, device=dev)
+    if M:
+        _lib.call("jfx_served_join_fill", kc, _lib.ptr(fr.group_keys), _lib.ptr(fr.group_starts), G,
+                  lsrc.element_size(), _lib.ptr(lsrc), R, rsrc.element_size(), _lib.ptr(rsrc), M,
+                  _lib.ptr(keys), _lib.ptr(lv), _lib.ptr(rv), _lib.ptr(ws), ws.numel(), stream)
     return keys, {"left_value": lv, "right_value": rv}
 
 
@@ -349,7 +389,16 @@ class Server:
 
     def __init__(self, store: FrameStore, slots: int = 4, hbm_budget_bytes: int | None = None,
                  executor=execute, use_streams: bool = True, footprint_fn=footprint,
-                 batch_executor=execute_filter_batch, batchable_fn=batchable, max_batch: int = 32):
+                 batch_executor=execute_filter_batch, batchable_fn=batchable, max_batch: int = 32,
+                 switch_interval: float | None = 1e-4):
+        # Slot threads spend most of their time in GPU waits, which release
+        # the GIL; a thread whose wait has ended must get the GIL back before
+        # it can launch the next step. With the interpreter's default 5 ms
+        # switch interval that hand-off, not the GPU, bounds throughput, so
+        # the server shortens it while it runs (restored by close()).
+        self._prev_switch = sys.getswitchinterval()
+        if switch_interval is not None:
+            sys.setswitchinterval(switch_interval)
         self.store = store
         self.slots = slots
         self.budget = int(hbm_budget_bytes if hbm_budget_bytes is not None else store.budget)
@@ -492,6 +541,7 @@ class Server:
             self.cv.notify_all()
         for w in self.workers:
             w.join()
+        sys.setswitchinterval(self._prev_switch)
 
 
 def poisson_arrivals(n: int, qps: float, seed: int = 0):

@@ -171,3 +171,12 @@ def test_batch_failure_reaches_every_member():
             f.result()
     assert srv.running == 0 and srv.reserved == 0
     srv.close()
+
+
+def test_switch_interval_set_while_serving_and_restored():
+    import sys
+    before = sys.getswitchinterval()
+    srv, _ = _make(slots=1)
+    assert abs(sys.getswitchinterval() - 1e-4) < 1e-9
+    srv.close()
+    assert sys.getswitchinterval() == before

@@ -241,3 +241,41 @@ def test_served_batch_abi_with_empty_queries(store):
         assert torch.equal(mean[a:b], em)
     assert st[Q] == sum(int(_served_groups_ref(store[n], o, t)[0].shape[0])
                         for n, o, t in spec if n is not None)
+
+
+@pytest.mark.parametrize("kdt,vdt,ddt", [(np.int64, np.float32, np.float32),
+                                          (np.int32, np.float64, np.int32),
+                                          (np.int64, np.int64, np.float64)])
+def test_join_by_groups_duplicates_and_types(kdt, vdt, ddt):
+    """Dimension keys repeated (several right rows per group, in row order),
+    absent from the frame, and of every integer key width; payloads promoted
+    as the reference does (ints to floats)."""
+    rng = np.random.default_rng(11)
+    n = 120_000
+    k = rng.integers(-300, 300, n).astype(kdt)
+    v = (rng.standard_normal(n) * 100).astype(vdt)
+    st = serving.FrameStore(1 << 30)
+    st.load("j", cuda(k), cuda(v))
+    dk = rng.integers(-400, 400, 5000).astype(kdt)           # duplicates, misses
+    dv = (rng.standard_normal(5000) * 100).astype(ddt)
+    keys, out = serving.execute(st, serving.Join("j", cuda(dk), cuda(dv)))
+    ek, eli, eri = oracle.inner_join_pairs(k, dk)
+    assert np.array_equal(keys.cpu().numpy(), ek)
+    lv = out["left_value"].cpu().numpy()
+    rv = out["right_value"].cpu().numpy()
+    exp_l = v[eli] if np.issubdtype(vdt, np.floating) else v[eli].astype(np.float64)
+    exp_r = dv[eri] if np.issubdtype(ddt, np.floating) else dv[eri].astype(np.float32)
+    assert lv.dtype == exp_l.dtype and rv.dtype == exp_r.dtype
+    assert np.array_equal(lv, exp_l)
+    assert np.array_equal(rv, exp_r)
+    # equals the plain sort-merge join
+    pk, pout = serving.run_plain(st, serving.Join("j", cuda(dk), cuda(dv)))
+    assert torch.equal(keys, pk)
+    assert torch.equal(out["left_value"], pout["left_value"])
+    assert torch.equal(out["right_value"], pout["right_value"])
+    # no dimension rows, and no matches
+    keys, out = serving.execute(st, serving.Join("j", cuda(dk[:0]), cuda(dv[:0])))
+    assert keys.shape[0] == 0 and out["left_value"].shape[0] == 0
+    far = (np.arange(50) + 10_000).astype(kdt)
+    keys, out = serving.execute(st, serving.Join("j", cuda(far), cuda(far.astype(np.float32))))
+    assert keys.shape[0] == 0
