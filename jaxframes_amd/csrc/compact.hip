@@ -179,11 +179,7 @@ int compact_run(const Pred& p, uint64_t n, uint32_t* out_rows, uint64_t* out_cou
                                         p, n, g.chunk, w.counts, out_rows)));
   JFX_HIP(hipGetLastError());
   if (out_count) {
-    uint64_t* h = static_cast<uint64_t*>(pinned_scratch(sizeof(uint64_t)));
-    JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
-    JFX_HIP(hipMemcpyAsync(h, w.total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    JFX_HIP(hipStreamSynchronize(s));
-    *out_count = *h;
+    return read_back(s, w.total, sizeof(uint64_t), out_count);
   }
   return JFX_OK;
 }

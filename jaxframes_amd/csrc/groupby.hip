@@ -521,12 +521,10 @@ int segreduce_dispatch(const SegWs& w, int kt, int vt, const SegCall& c, hipStre
 }
 
 int read_ngroups(const SegWs& w, uint64_t* out, hipStream_t s) {
-  uint32_t* h = static_cast<uint32_t*>(pinned_scratch(sizeof(uint32_t)));
-  JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
-  JFX_HIP(hipMemcpyAsync(h, w.ngroups, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  JFX_HIP(hipStreamSynchronize(s));
-  *out = *h;
-  return JFX_OK;
+  uint32_t g = 0;
+  const int rc = read_back(s, w.ngroups, sizeof(uint32_t), &g);
+  *out = g;
+  return rc;
 }
 
 bool valid_vt(int vt) { return vt >= JFX_U32 && vt <= JFX_F64; }

@@ -296,12 +296,7 @@ int join_count_typed(int oc, const void* sl, uint64_t L, const void* sr, uint64_
   JFX_HIP(hipGetLastError());
   JFX_LAUNCH("join_scan", s, (k_join_scan<<<1, 1024, 0, s>>>(w.tcnt, nt, w.total)));
   JFX_HIP(hipGetLastError());
-  uint64_t* h = static_cast<uint64_t*>(pinned_scratch(sizeof(uint64_t)));
-  JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
-  JFX_HIP(hipMemcpyAsync(h, w.total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  JFX_HIP(hipStreamSynchronize(s));
-  *out_m = *h;
-  return JFX_OK;
+  return read_back(s, w.total, sizeof(uint64_t), out_m);
 }
 
 template <typename K, typename LT = uint32_t, typename RT = uint32_t>

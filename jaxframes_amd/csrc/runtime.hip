@@ -1,6 +1,7 @@
 // Host runtime: error strings, pinned staging, device error word, version.
 #include "runtime.hpp"
 
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -40,6 +41,67 @@ void* pinned_scratch(size_t bytes) {
     t_pinned.cap = cap;
   }
   return t_pinned.p;
+}
+
+namespace {
+struct Coherent {
+  uint32_t* h = nullptr;   // host view
+  uint32_t* d = nullptr;   // device view
+  uint64_t seq = 0;
+  ~Coherent() {
+    if (h) (void)hipHostFree(h);
+  }
+};
+thread_local Coherent t_coh;
+constexpr size_t RB_BYTES = READBACK_MAX + 64;
+constexpr size_t RB_SEQ = READBACK_MAX / sizeof(uint64_t);  // index of the sequence word
+}  // namespace
+
+__global__ void __launch_bounds__(256)
+k_read_back(const uint32_t* src, uint32_t nwords, uint32_t* dst, uint64_t* seq_slot, uint64_t seq) {
+  for (uint32_t i = threadIdx.x; i < nwords; i += 256) dst[i] = src[i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(seq_slot, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+int read_back(hipStream_t s, const void* dev_src, size_t bytes, void* host_out) {
+  JFX_CHECK(bytes % 4 == 0 && bytes <= READBACK_MAX, JFX_ERR_INVALID, "bad read-back size");
+  if (!t_coh.h) {
+    void* p = nullptr;
+    JFX_HIP(hipHostMalloc(&p, RB_BYTES, hipHostMallocCoherent | hipHostMallocMapped));
+    void* dp = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&dp, p, 0);
+    if (e != hipSuccess) {
+      (void)hipHostFree(p);
+      return hip_fail(e, "hipHostGetDevicePointer");
+    }
+    t_coh.h = static_cast<uint32_t*>(p);
+    t_coh.d = static_cast<uint32_t*>(dp);
+  }
+  uint64_t* hseq = reinterpret_cast<uint64_t*>(t_coh.h) + RB_SEQ;
+  uint64_t* dseq = reinterpret_cast<uint64_t*>(t_coh.d) + RB_SEQ;
+  const uint64_t seq = ++t_coh.seq;
+  JFX_LAUNCH("read_back", s, (k_read_back<<<1, 256, 0, s>>>(static_cast<const uint32_t*>(dev_src),
+                                                           (uint32_t)(bytes / 4), t_coh.d, dseq,
+                                                           seq)));
+  JFX_HIP(hipGetLastError());
+  const auto t0 = std::chrono::steady_clock::now();
+  uint32_t polls = 0;
+  while (__atomic_load_n(hseq, __ATOMIC_ACQUIRE) != seq) {
+    if ((++polls & 255u) == 0 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+      JFX_HIP(hipStreamSynchronize(s));
+      JFX_CHECK(__atomic_load_n(hseq, __ATOMIC_ACQUIRE) == seq, JFX_ERR_HIP,
+                "read-back did not complete");
+      break;
+    }
+  }
+  std::memcpy(host_out, t_coh.h, bytes);
+  return JFX_OK;
 }
 
 __device__ uint32_t g_device_error;

@@ -44,6 +44,16 @@ struct Carver {
 // A per-thread pinned host buffer for small device-to-host reads.
 void* pinned_scratch(size_t bytes);
 
+// Reads `bytes` (a multiple of 4, at most READBACK_MAX) of device memory
+// written by earlier work on stream s into host_out: a one-workgroup kernel
+// copies them into this thread's coherent pinned buffer and then publishes a
+// sequence number; the host spins on it (about 6 us after the data is ready,
+// against about 15 us for a copy plus hipStreamSynchronize), and after 2 ms
+// falls back to hipStreamSynchronize so that long waits block and stream
+// errors surface.
+constexpr size_t READBACK_MAX = 16384 - 64;
+int read_back(hipStream_t s, const void* dev_src, size_t bytes, void* host_out);
+
 // Device error word (one per device, in device memory owned by the library
 // image as a __device__ variable).
 uint32_t* device_error_word();

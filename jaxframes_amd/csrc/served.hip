@@ -494,12 +494,7 @@ int sj_count_typed(int oc, const void* gk, const uint32_t* starts, uint32_t G, c
   JFX_LAUNCH("sj_offs", s, (k_sj_offs<<<nt, 256, 0, s>>>(starts, w.dfirst, w.dend, G, w.tile_off,
                                                          w.total, w.off)));
   JFX_HIP(hipGetLastError());
-  uint64_t* h = static_cast<uint64_t*>(pinned_scratch(sizeof(uint64_t)));
-  JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
-  JFX_HIP(hipMemcpyAsync(h, w.total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  JFX_HIP(hipStreamSynchronize(s));
-  *out_m = *h;
-  return JFX_OK;
+  return read_back(s, w.total, sizeof(uint64_t), out_m);
 }
 
 template <typename K, typename LT, typename RT>
@@ -666,7 +661,7 @@ int jfx_served_batch(int val_type, int key_bytes, const jfx_served_query* querie
   hipStream_t s = jfx::as_stream(stream);
   // query table and the read-back of the output starts share one pinned block
   const size_t tb = sizeof(jfx::SvDev) * nqueries, sb = sizeof(uint64_t) * (nqueries + 1);
-  char* h = static_cast<char*>(jfx::pinned_scratch(tb + sb));
+  char* h = static_cast<char*>(jfx::pinned_scratch(tb));
   JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
   jfx::SvDev* ht = reinterpret_cast<jfx::SvDev*>(h);
   uint64_t off = 0;
@@ -684,11 +679,9 @@ int jfx_served_batch(int val_type, int key_bytes, const jfx_served_query* querie
     rc = key_bytes == 4 ? jfx::served_batch_typed<double, uint32_t>(w, nqueries, tot, out_keys, out_count, out_sum, out_mean, s)
                         : jfx::served_batch_typed<double, uint64_t>(w, nqueries, tot, out_keys, out_count, out_sum, out_mean, s);
   if (rc != JFX_OK) return rc;
-  uint64_t* hs = reinterpret_cast<uint64_t*>(h + tb);
-  JFX_HIP(hipMemcpyAsync(hs, w.start, sb, hipMemcpyDeviceToHost, s));
-  JFX_HIP(hipStreamSynchronize(s));
-  for (uint32_t i = 0; i <= nqueries; ++i) out_starts[i] = hs[i];
-  return JFX_OK;
+  // the read-back runs after the table upload on the stream, so the staging
+  // block is free again once it returns
+  return jfx::read_back(s, w.start, sb, out_starts);
 }
 
 }  // extern "C"

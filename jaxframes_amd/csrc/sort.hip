@@ -514,10 +514,9 @@ int sort_prepare(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_by
   JFX_HIP(hipGetLastError());
   int rc = c.kb == 4 ? stats_launch<uint32_t>(c, keys_in) : stats_launch<uint64_t>(c, keys_in);
   if (rc != JFX_OK) return rc;
-  unsigned long long* h = static_cast<unsigned long long*>(pinned_scratch(4 * sizeof(unsigned long long)));
-  JFX_CHECK(h != nullptr, JFX_ERR_HIP, "pinned host allocation failed");
-  JFX_HIP(hipMemcpyAsync(h, c.stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  JFX_HIP(hipStreamSynchronize(s));
+  unsigned long long h[4];
+  rc = read_back(s, c.stats, sizeof(h), h);
+  if (rc != JFX_OK) return rc;
   c.mn = h[0];
   c.mx = h[1];
   c.band = h[2];
