@@ -295,8 +295,17 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # per-kernel breakdown from an untimed pass with events around every launch
+    nb = max(3, args.warmup)
+    with _lib.KernelTimer() as kt_all:
+        for _ in range(nb):
+            step()
+        torch.cuda.synchronize()
+    breakdown = kt_all.stats()
+    dname = max(breakdown.items(), key=lambda kv: kv[1][0])[0]
 
-    with _lib.KernelTimer() as kt:
+    # timed region: events around the dominant kernel only
+    with _lib.KernelTimer(prefix=dname) as kt:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -320,9 +329,8 @@ def main() -> int:
     assert _lib.device_error() == 0
     ngroups = int(uk.shape[0])
 
-    # roofline of the dominant kernel
-    dom = max(kstats.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dcount) = dom
+    # roofline of the dominant kernel, from its events in the timed region
+    dms, dcount = kstats[dname]
     avg_s = dms / 1e3 / dcount
     dbytes = _kernel_bytes(dname, n, 4, 4)
     roof = None
@@ -332,17 +340,16 @@ def main() -> int:
                 "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBPS, 4),
                 "traffic": _traffic(dname), "avg_launch_us": round(avg_s * 1e6, 1),
                 "alg_bytes_per_launch": dbytes}
-    kernels = {k: {"us_per_step": round(v[0] / args.steps * 1e3, 1),
-                   "launches_per_step": v[1] // args.steps} for k, v in kstats.items()}
+    kernels = {k: {"us_per_step": round(v[0] / nb * 1e3, 1),
+                   "launches_per_step": v[1] // nb} for k, v in breakdown.items()}
     # both small-range kernels are launched and the unselected one returns at
     # once: count the bytes of the one that ran (the longer)
-    ran = dict(kstats)
+    ran = dict(breakdown)
     alts = [k for k in ran if k.startswith("wave_table") or k.startswith("tile_reduce")]
     for k in alts:
         if k != max(alts, key=lambda a: ran[a][0]):
             ran.pop(k)
-    step_bytes = sum((_kernel_bytes(k, n, 4, 4) or 0) * (v[1] // args.steps)
-                     for k, v in ran.items())
+    step_bytes = sum((_kernel_bytes(k, n, 4, 4) or 0) * (v[1] // nb) for k, v in ran.items())
 
     batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows)
     served_line = None if args.no_served else served_vs_plain()

@@ -37,6 +37,7 @@ extern "C" {
 #define JFX_ERR_UNSUPPORTED 3  /* type/size combination not supported */
 #define JFX_ERR_WORKSPACE 4    /* workspace too small */
 #define JFX_ERR_DEVICE 5       /* a kernel raised the device error word */
+#define JFX_ERR_CAPACITY 6     /* outputs too small for the groups the call may produce */
 
 /* key / value type codes */
 #define JFX_U32 0
@@ -122,6 +123,17 @@ int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, 
                 void* out_keys, void* out_sum, int64_t* out_count, void* out_min, void* out_max,
                 void* out_mean, uint64_t* out_ngroups, int digit_bits, void* workspace,
                 uint64_t ws_bytes, void* stream);
+
+/* jfx_groupby with outputs of out_capacity entries (jfx_groupby: n). When
+ * the path chosen after the key statistics may produce more groups than
+ * that -- more than min(n, max - min + 1) for a small integer key range, else
+ * more than n -- nothing is written, *out_ngroups is set to the capacity needed
+ * and JFX_ERR_CAPACITY is returned; a caller that expects few groups passes
+ * small outputs and retries with the capacity returned. */
+int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
+                   void* out_keys, void* out_sum, int64_t* out_count, void* out_min,
+                   void* out_max, void* out_mean, uint64_t* out_ngroups, uint64_t out_capacity,
+                   int digit_bits, void* workspace, uint64_t ws_bytes, void* stream);
 
 /* ---- hashing and the local hash partition ----------------------------- */
 /* Row hash of 1..8 key columns (device pointers, one per column, n rows
@@ -343,6 +355,9 @@ int jfx_filter_rows(int col_type, const void* col, int cmp, const void* value, u
  * pending events). */
 int jfx_prof_enable(int on);
 int jfx_prof_reset(void);
+/* Time only launches whose kernel name starts with name_prefix ("" or NULL:
+ * all), so a timed region can carry events around one kernel only. */
+int jfx_prof_filter(const char* name_prefix);
 int jfx_prof_count(int* out_n);
 int jfx_prof_get(int i, char* name_buf, uint64_t buf_len, double* total_ms, uint64_t* count);
 

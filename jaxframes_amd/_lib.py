@@ -15,6 +15,7 @@ import torch
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libjfx.so"
 
 JFX_U32, JFX_I32, JFX_F32, JFX_U64, JFX_I64, JFX_F64 = range(6)
+JFX_ERR_CAPACITY = 6
 AGG_SUM, AGG_COUNT, AGG_MIN, AGG_MAX, AGG_MEAN = 1, 2, 4, 8, 16
 CMP_CODES = {">": 0, ">=": 1, "<": 2, "<=": 3, "==": 4, "!=": 5}
 # device error word bits (csrc/common.hpp)
@@ -22,7 +23,7 @@ DEVERR_LOOKBACK = 1   # a bounded cross-workgroup wait ran out
 DEVERR_BOUNDS = 2     # a computed position or key fell outside its declared range
 
 _ERRNAMES = {1: "invalid argument", 2: "HIP error", 3: "unsupported", 4: "workspace too small",
-             5: "device error"}
+             5: "device error", 6: "output capacity too small"}
 
 
 class JfxError(RuntimeError):
@@ -50,6 +51,7 @@ _SIGNATURES = {
                       c_u64p, c_vp, c_u64, c_vp],
     "jfx_prof_enable": [c_int],
     "jfx_prof_reset": [],
+    "jfx_prof_filter": [ctypes.c_char_p],
     "jfx_prof_count": [ctypes.POINTER(c_int)],
     "jfx_prof_get": [c_int, ctypes.c_char_p, c_u64, ctypes.POINTER(ctypes.c_double), c_u64p],
     "jfx_hash_rows": [ctypes.POINTER(c_vp), ctypes.POINTER(c_int), c_int, c_u64, c_vp, c_vp],
@@ -88,6 +90,8 @@ _SIGNATURES = {
     "jfx_groupby_workspace_size": [c_int, c_int, c_u64, c_u64p],
     "jfx_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64p,
                     c_int, c_vp, c_u64, c_vp],
+    "jfx_groupby_ex": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64p,
+                       c_u64, c_int, c_vp, c_u64, c_vp],
 }
 
 
@@ -166,13 +170,18 @@ class KernelTimer:
     stream each kernel is launched on). Use as a context manager around the
     timed region, then read .stats() -> {name: (total_ms, launches)}."""
 
+    def __init__(self, prefix: str = ""):
+        self.prefix = prefix
+
     def __enter__(self):
         call("jfx_prof_reset")
+        call("jfx_prof_filter", self.prefix.encode())
         call("jfx_prof_enable", 1)
         return self
 
     def __exit__(self, *exc):
         call("jfx_prof_enable", 0)
+        call("jfx_prof_filter", b"")
 
     @staticmethod
     def stats() -> dict:

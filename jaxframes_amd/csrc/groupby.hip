@@ -724,6 +724,14 @@ int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, 
                 void* out_keys, void* out_sum, int64_t* out_count, void* out_min, void* out_max,
                 void* out_mean, uint64_t* out_ngroups, int digit_bits, void* workspace,
                 uint64_t ws_bytes, void* stream) {
+  return jfx_groupby_ex(key_type, keys, val_type, vals, n, out_keys, out_sum, out_count, out_min,
+                        out_max, out_mean, out_ngroups, n, digit_bits, workspace, ws_bytes, stream);
+}
+
+int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
+                   void* out_keys, void* out_sum, int64_t* out_count, void* out_min,
+                   void* out_max, void* out_mean, uint64_t* out_ngroups, uint64_t out_capacity,
+                   int digit_bits, void* workspace, uint64_t ws_bytes, void* stream) {
   JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64, JFX_ERR_INVALID, "bad key type");
   JFX_CHECK(jfx::valid_vt(val_type), JFX_ERR_INVALID, "bad value type");
   JFX_CHECK(n < (1ull << 30), JFX_ERR_UNSUPPORTED, "n must be below 2^30 rows");
@@ -749,7 +757,15 @@ int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, 
   int rc = jfx::sort_prepare(ctx, key_type, keys, n, vb, digit_bits, sort_ws, sws, s);
   if (rc != JFX_OK) return rc;
   const int oc = jfx::order_code(key_type);
-  if (jfx::int_key(key_type) && ctx.single_pass()) {
+  const bool small = jfx::int_key(key_type) && ctx.single_pass();
+  // groups are at most the rows, and at most the key range on the small path
+  const uint64_t need = small ? std::min<uint64_t>(ctx.range_radix(), n) : n;
+  if (need > out_capacity) {
+    *out_ngroups = need;
+    return jfx::set_error(JFX_ERR_CAPACITY, "outputs hold fewer entries than the groups this "
+                                            "call may produce; *out_ngroups holds the capacity needed");
+  }
+  if (small) {
     // Small key range: every distinct key is one digit of a single counting
     // sort; each tile is sorted and reduced in LDS (tile_reduce), so neither
     // sorted keys nor sorted values are written to HBM.

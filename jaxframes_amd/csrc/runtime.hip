@@ -132,7 +132,7 @@ struct ProfState {
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
-  hipEvent_t cur_start = nullptr;
+  std::string filter;   // time only kernels whose name starts with this
   struct Acc {
     double ms = 0.0;
     uint64_t count = 0;
@@ -170,22 +170,28 @@ struct ProfState {
   }
 };
 ProfState g_prof;
+thread_local hipEvent_t t_prof_start = nullptr;  // start event of this thread's open launch
 }  // namespace
 
-bool prof_on() { return g_prof.on; }
+bool prof_wants(const char* name) {
+  if (!g_prof.on) return false;
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  return g_prof.filter.empty() ||
+         std::strncmp(name, g_prof.filter.c_str(), g_prof.filter.size()) == 0;
+}
 
 void prof_begin(hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_prof.mu);
-  g_prof.cur_start = g_prof.get();
-  (void)hipEventRecord(g_prof.cur_start, s);
+  t_prof_start = g_prof.get();
+  (void)hipEventRecord(t_prof_start, s);
 }
 
 void prof_end(const char* name, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_prof.mu);
   hipEvent_t b = g_prof.get();
   (void)hipEventRecord(b, s);
-  g_prof.pending.push_back({name, g_prof.cur_start, b});
-  g_prof.cur_start = nullptr;
+  g_prof.pending.push_back({name, t_prof_start, b});
+  t_prof_start = nullptr;
 }
 
 }  // namespace jfx
@@ -195,6 +201,12 @@ extern "C" {
 int jfx_prof_enable(int on) {
   std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
   jfx::g_prof.on = on != 0;
+  return JFX_OK;
+}
+
+int jfx_prof_filter(const char* name_prefix) {
+  std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
+  jfx::g_prof.filter = name_prefix ? name_prefix : "";
   return JFX_OK;
 }
 

@@ -63,7 +63,7 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // Per-kernel timing with HIP events on the launch stream (enabled by
 // jfx_prof_enable). prof_begin/prof_end bracket one launch; when profiling
 // is off they cost one branch.
-bool prof_on();
+bool prof_wants(const char* name);
 void prof_begin(hipStream_t s);
 void prof_end(const char* name, hipStream_t s);
 
@@ -71,7 +71,7 @@ void prof_end(const char* name, hipStream_t s);
 // arguments under `name`.
 #define JFX_LAUNCH(name, stream, ...)                    \
   do {                                                   \
-    const bool _jfx_prof = ::jfx::prof_on();             \
+    const bool _jfx_prof = ::jfx::prof_wants(name);      \
     if (_jfx_prof) ::jfx::prof_begin(stream);            \
     __VA_ARGS__;                                         \
     if (_jfx_prof) ::jfx::prof_end(name, stream);        \

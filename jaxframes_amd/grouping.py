@@ -67,9 +67,15 @@ def _out_ptrs(fn, out):
     return [_lib.ptr(p[k]) for k in ("sum", "count", "min", "max", "mean")]
 
 
+# inputs of at least this many rows try outputs of _SMALL_CAP entries first
+_SMALL_CAP = 2048
+_SMALL_CAP_MIN_ROWS = 1 << 22
+
+
 def _shrink(t: torch.Tensor, g: int) -> torch.Tensor:
+    """The first g entries; copied out when they would pin a much larger buffer."""
     v = t[:g]
-    return v.clone() if g < t.shape[0] // 2 else v
+    return v.clone() if t.shape[0] > 4 * _SMALL_CAP and g < t.shape[0] // 2 else v
 
 
 def _finish_col(fn, out, orig_dtype, g):
@@ -102,7 +108,6 @@ def groupby_aggregate(keys, values: dict, agg_funcs: dict, *, digit_bits: int = 
         return k[:0].to(korig), {c: v[:0] for c, (v, _) in cols.items()}
     stream = ctypes.c_void_p(_lib.stream_handle(dev))
     g = ctypes.c_uint64(0)
-    out_keys = torch.empty(n, dtype=k.dtype, device=dev)
     results = {}
     if len(agg_funcs) == 1:
         (col, fn), = agg_funcs.items()
@@ -111,32 +116,46 @@ def groupby_aggregate(keys, values: dict, agg_funcs: dict, *, digit_bits: int = 
         nb = ctypes.c_uint64(0)
         _lib.call("jfx_groupby_workspace_size", kc, vcode, n, ctypes.byref(nb))
         ws = _lib.workspace(nb.value, dev)
-        out = _alloc_out(fn, v, n, dev)
-        _lib.call("jfx_groupby", kc, _lib.ptr(k), vcode, _lib.ptr(v), n, _lib.ptr(out_keys),
-                  *_out_ptrs(fn, out), ctypes.byref(g), digit_bits, _lib.ptr(ws), ws.numel(),
-                  stream)
+        # large inputs first try outputs sized for a small key range; the call
+        # reports the capacity it needs when the keys span more
+        cap = _SMALL_CAP if n >= _SMALL_CAP_MIN_ROWS else n
+        while True:
+            out_keys = torch.empty(cap, dtype=k.dtype, device=dev)
+            out = _alloc_out(fn, v, cap, dev)
+            st = _lib.lib().jfx_groupby_ex(kc, _lib.ptr(k), vcode, _lib.ptr(v), n,
+                                           _lib.ptr(out_keys), *_out_ptrs(fn, out),
+                                           ctypes.byref(g), cap, digit_bits, _lib.ptr(ws),
+                                           ws.numel(), stream)
+            if st == _lib.JFX_ERR_CAPACITY and cap < n:
+                cap = g.value
+                continue
+            _lib.check(st, "jfx_groupby_ex")
+            break
         G = g.value
         results[col] = _finish_col(fn, out, vorig, G)
-    else:
-        sk, _, perm = radix_sort(k, None, True, return_index=True, digit_bits=digit_bits)
-        perm32 = perm.to(torch.int32)
-        G = None
-        for i, (col, fn) in enumerate(agg_funcs.items()):
-            v, vorig = cols[col]
-            vcode = _VAL_CODE[v.dtype]
-            ws = _segreduce_ws(vcode, n, dev)
-            out = _alloc_out(fn, v, n, dev)
-            _lib.call("jfx_segreduce", kc, _lib.ptr(sk), vcode, _lib.ptr(v), _lib.ptr(perm32), n,
-                      _lib.ptr(out_keys if i == 0 else None), *_out_ptrs(fn, out),
-                      ctypes.byref(g) if i == 0 else None, _lib.ptr(ws), ws.numel(), stream)
-            if i == 0:
-                G = g.value
-            results[col] = (fn, out, vorig)
-        results = {c: _finish_col(fn, out, vorig, G) for c, (fn, out, vorig) in results.items()}
+        return _finish_keys(out_keys, G, korig), results
+    out_keys = torch.empty(n, dtype=k.dtype, device=dev)
+    sk, _, perm = radix_sort(k, None, True, return_index=True, digit_bits=digit_bits)
+    perm32 = perm.to(torch.int32)
+    G = None
+    for i, (col, fn) in enumerate(agg_funcs.items()):
+        v, vorig = cols[col]
+        vcode = _VAL_CODE[v.dtype]
+        ws = _segreduce_ws(vcode, n, dev)
+        out = _alloc_out(fn, v, n, dev)
+        _lib.call("jfx_segreduce", kc, _lib.ptr(sk), vcode, _lib.ptr(v), _lib.ptr(perm32), n,
+                  _lib.ptr(out_keys if i == 0 else None), *_out_ptrs(fn, out),
+                  ctypes.byref(g) if i == 0 else None, _lib.ptr(ws), ws.numel(), stream)
+        if i == 0:
+            G = g.value
+        results[col] = (fn, out, vorig)
+    results = {c: _finish_col(fn, out, vorig, G) for c, (fn, out, vorig) in results.items()}
+    return _finish_keys(out_keys, G, korig), results
+
+
+def _finish_keys(out_keys, G, korig):
     uk = _shrink(out_keys, G)
-    if uk.dtype != korig:
-        uk = uk.to(korig)
-    return uk, results
+    return uk.to(korig) if uk.dtype != korig else uk
 
 
 def groupby_stats(keys, values, *, digit_bits: int = 0):

@@ -159,3 +159,35 @@ def test_groupby_empty():
 def test_no_device_error_after_groupby():
     from jaxframes_amd import _lib
     assert _lib.device_error() == 0
+
+
+def test_output_capacity_retry_and_small_outputs():
+    """jfx_groupby_ex reports the capacity it needs instead of writing past
+    small outputs; groupby_aggregate's small-output first try and retry give
+    the same groups as full-size outputs."""
+    import ctypes
+    from jaxframes_amd import _lib
+    from jaxframes_amd.grouping import _SMALL_CAP_MIN_ROWS
+    rng = np.random.default_rng(21)
+    n = _SMALL_CAP_MIN_ROWS + 1000
+    wide = torch.from_numpy(rng.integers(-10**9, 10**9, n).astype(np.int64)).cuda()   # general path
+    narrow = torch.from_numpy(rng.integers(5, 1500, n).astype(np.int32)).cuda()       # small range
+    vals = torch.from_numpy(rng.standard_normal(n).astype(np.float32)).cuda()
+    for keys in (wide, narrow):
+        uk, res = grouping.groupby_aggregate(keys, {"v": vals}, {"v": "count"})
+        ek, ec = np.unique(keys.cpu().numpy(), return_counts=True)
+        assert np.array_equal(uk.cpu().numpy(), ek)
+        assert np.array_equal(res["v"].cpu().numpy(), ec)
+    # the C ABI: too small a capacity writes nothing and reports the need
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_groupby_workspace_size", _lib.JFX_I64, _lib.JFX_F32, n, ctypes.byref(nb))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    out_k = torch.full((16,), 7, dtype=torch.int64, device="cuda")
+    out_s = torch.full((16,), 7.0, dtype=torch.float32, device="cuda")
+    g = ctypes.c_uint64(0)
+    st = _lib.lib().jfx_groupby_ex(_lib.JFX_I64, _lib.ptr(wide), _lib.JFX_F32, _lib.ptr(vals), n,
+                                   _lib.ptr(out_k), _lib.ptr(out_s), None, None, None, None,
+                                   ctypes.byref(g), 16, 0, _lib.ptr(ws), ws.numel(), 0)
+    assert st == _lib.JFX_ERR_CAPACITY and g.value == n
+    torch.cuda.synchronize()
+    assert torch.all(out_k == 7) and torch.all(out_s == 7.0)
