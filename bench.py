@@ -15,9 +15,10 @@ and the served stream (request loop over resident frames, Poisson arrivals)
 against their plain per-query paths, and the 2e8-key hash partition.
 
 Prints one JSON line with rows/s, per-step p50/p99, the roofline of the
-dominant kernel (HIP-event durations measured inside the timed region on the
-launch stream), and a CPU baseline (the numpy restatement timed on a bounded
-sample on this host).
+dominant kernel (HIP events recorded around that kernel's launches only, on
+its launch stream, inside the timed region; the per-kernel breakdown comes
+from an untimed pass with events around every launch), and a CPU baseline
+(the numpy restatement timed on a bounded sample on this host).
 """
 
 from __future__ import annotations
