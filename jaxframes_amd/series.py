@@ -19,7 +19,13 @@ _SCALARS = (int, float, np.number)
 
 class JaxSeries:
     def __init__(self, data=None, name: str | None = None, index=None, lazy: bool = False,
-                 parent_frame=None, expr=None, dtype: str | None = None):
+                 parent_frame=None, expr=None, dtype: str | None = None, string_vocab=None):
+        self._string_vocab = string_vocab
+        if isinstance(data, np.ndarray) and dtype is None and string_vocab is None:
+            from .strings import encode_string_array, is_string_array
+            if data.dtype.kind in "OUS" and is_string_array(data):
+                data, self._string_vocab = encode_string_array(data)
+                dtype = "string"
         self._lazy = lazy
         self._parent_frame = parent_frame
         self._expr = expr
@@ -30,8 +36,8 @@ class JaxSeries:
             None if data is None else _dtype_name(data))
 
     @classmethod
-    def _from_processed_data(cls, data, name=None, index=None, dtype=None):
-        return cls(data, name=name, index=index, dtype=dtype)
+    def _from_processed_data(cls, data, name=None, index=None, dtype=None, string_vocab=None):
+        return cls(data, name=name, index=index, dtype=dtype, string_vocab=string_vocab)
 
     @property
     def dtype(self):
@@ -53,9 +59,15 @@ class JaxSeries:
 
     def to_numpy(self) -> np.ndarray:
         from ._tensors import to_numpy
+        if self._dtype == "string":
+            from .strings import decode_string_codes
+            return decode_string_codes(self.data, self._string_vocab or ())
         if isinstance(self.data, np.ndarray):
             return self.data
         return to_numpy(self.data)
+
+    def tolist(self) -> list:
+        return self.to_numpy().tolist()
 
     def __array__(self, dtype=None, copy=None):
         a = self.to_numpy()
@@ -78,18 +90,37 @@ class JaxSeries:
                              expr=ComparisonOp(op=op, left=self._expr_of(), right=right))
         if isinstance(self.data, np.ndarray):
             raise TypeError("comparisons on object columns are not supported")
+        fn = {ComparisonOpType.GT: torch.gt, ComparisonOpType.LT: torch.lt,
+              ComparisonOpType.GE: torch.ge, ComparisonOpType.LE: torch.le,
+              ComparisonOpType.EQ: torch.eq, ComparisonOpType.NE: torch.ne}[op]
+        lhs = self.data
         if isinstance(other, JaxSeries):
             if other._lazy:
                 raise ValueError("Cannot mix lazy and eager series in comparisons")
             rhs = other.data
+            if self._dtype == "string" and other._dtype == "string":
+                from .strings import align_string_code_arrays
+                lhs, rhs, _ = align_string_code_arrays(self.data, self._string_vocab or (),
+                                                       other.data, other._string_vocab or ())
+        elif self._dtype == "string" and (other is None or isinstance(other, str)):
+            # core/series.py:441-460: exact code for == / !=, insertion rank otherwise
+            from .strings import string_scalar_code, string_scalar_rank
+            vocab = self._string_vocab or ()
+            code = string_scalar_code(vocab, other)
+            if op in (ComparisonOpType.EQ, ComparisonOpType.NE):
+                if code is None:
+                    res = torch.full_like(lhs, op is ComparisonOpType.NE, dtype=torch.bool)
+                    return JaxSeries._from_processed_data(res, name=self.name, dtype="bool")
+                rhs = code
+            else:
+                if other is None:
+                    return NotImplemented
+                rhs = string_scalar_rank(vocab, other)
         elif isinstance(other, _SCALARS):
             rhs = other
         else:
             return NotImplemented
-        fn = {ComparisonOpType.GT: torch.gt, ComparisonOpType.LT: torch.lt,
-              ComparisonOpType.GE: torch.ge, ComparisonOpType.LE: torch.le,
-              ComparisonOpType.EQ: torch.eq, ComparisonOpType.NE: torch.ne}[op]
-        return JaxSeries._from_processed_data(fn(self.data, rhs), name=self.name, dtype="bool")
+        return JaxSeries._from_processed_data(fn(lhs, rhs), name=self.name, dtype="bool")
 
     def __gt__(self, other):
         return self._comparison_op(other, ComparisonOpType.GT)

@@ -26,7 +26,8 @@ __all__ = [
     "segment_sums_f64", "inner_join_pairs", "inner_join_pairs_bruteforce",
     "sort_merge_join", "fmix32", "hash_multi_columns", "hash_partition",
     "bucketize_padded", "compact_rows", "filter_mask_compact", "join_multi_column",
-    "gather_join_column",
+    "gather_join_column", "encode_string_array", "string_join_plan",
+    "groupby_encoded_strings",
     "murmur3_x86_32", "sum_error_metrics", "PINNED_BY",
 ]
 
@@ -505,6 +506,105 @@ def sum_error_metrics(got, sorted_vals, starts):
     return (float(rel.max()) if rel.size else 0.0, float(nrm.max()) if nrm.size else 0.0)
 
 
+# ---- dictionary-encoded strings (core/string_encoding.py, core/string_fastpaths.py) ----
+
+def _null_string(v) -> bool:
+    return v is None or (isinstance(v, float) and np.isnan(v))
+
+
+def encode_string_array(arr):
+    """(int32 codes, vocab): vocab = sorted distinct non-null strings, -1 for
+    null (string_encoding.py:38-51)."""
+    obj = np.asarray(arr, dtype=object)
+    vocab = tuple(sorted({str(v) for v in obj if not _null_string(v)}))
+    pos = {v: i for i, v in enumerate(vocab)}
+    codes = np.array([-1 if _null_string(v) else pos[str(v)] for v in obj], dtype=np.int32)
+    return codes, vocab
+
+
+def string_join_plan(left_codes, right_codes, num_codes, how):
+    """(joined codes, left rows, right rows), -1 for none, of a join over
+    aligned codes; null keys are one more code (string_fastpaths.py:95-119).
+    Right codes unique: every left row looks up its right row, pairs in left
+    row order (:200-243). Otherwise, code by code ascending, the cartesian
+    product of the code's left and right rows, left-major, or its unmatched
+    rows (:246-309)."""
+    null = num_codes
+    ld = np.where(np.asarray(left_codes) < 0, null, left_codes).astype(np.int64)
+    rd = np.where(np.asarray(right_codes) < 0, null, right_codes).astype(np.int64)
+    restore = lambda c: np.where(np.asarray(c) == null, -1, c).astype(np.int32)
+    rcount = np.bincount(rd, minlength=null + 1) if rd.size else np.zeros(null + 1, np.int64)
+    if np.all(rcount <= 1):
+        lookup = np.full(null + 1, -1, dtype=np.int64)
+        lookup[rd] = np.arange(rd.size)
+        matched = lookup[ld] if ld.size else np.zeros(0, np.int64)
+        m = matched >= 0
+        lp = np.arange(ld.size)
+        if how == "inner":
+            return restore(ld[m]), lp[m].astype(np.int32), matched[m].astype(np.int32)
+        if how == "left":
+            return restore(ld), lp.astype(np.int32), matched.astype(np.int32)
+        hit = np.zeros(rd.size, bool)
+        hit[matched[m]] = True
+        um = np.flatnonzero(~hit)
+        none = np.full(um.size, -1)
+        if how == "right":
+            return (restore(np.concatenate([ld[m], rd[um]])),
+                    np.concatenate([lp[m], none]).astype(np.int32),
+                    np.concatenate([matched[m], um]).astype(np.int32))
+        return (restore(np.concatenate([ld, rd[um]])), np.concatenate([lp, none]).astype(np.int32),
+                np.concatenate([matched, um]).astype(np.int32))
+    codes, lo, ro = [], [], []
+    top = int(max(ld.max(initial=0), rd.max(initial=0)))
+    for c in range(top + 1):
+        lrows = np.flatnonzero(ld == c)
+        rrows = np.flatnonzero(rd == c)
+        if lrows.size and rrows.size:
+            codes.append(np.full(lrows.size * rrows.size, c))
+            lo.append(np.repeat(lrows, rrows.size))
+            ro.append(np.tile(rrows, lrows.size))
+        elif lrows.size and how in ("left", "outer"):
+            codes.append(np.full(lrows.size, c))
+            lo.append(lrows)
+            ro.append(np.full(lrows.size, -1))
+        elif rrows.size and how in ("right", "outer"):
+            codes.append(np.full(rrows.size, c))
+            lo.append(np.full(rrows.size, -1))
+            ro.append(rrows)
+    if not codes:
+        e = np.zeros(0, np.int32)
+        return e, e, e
+    return (restore(np.concatenate(codes)), np.concatenate(lo).astype(np.int32),
+            np.concatenate(ro).astype(np.int32))
+
+
+def groupby_encoded_strings(keys, values, agg_funcs, vocab_size):
+    """Observed codes ascending and the aggregates over the non-null rows
+    (string_fastpaths.py:41-74, 161-197); sums in float64 (the FP64 oracle),
+    count as int32."""
+    keys = np.asarray(keys)
+    valid = keys >= 0
+    vk = keys[valid]
+    counts = np.bincount(vk, minlength=vocab_size) if vk.size else np.zeros(vocab_size, np.int64)
+    obs = np.flatnonzero(counts > 0)
+    out = {}
+    for c, agg in agg_funcs.items():
+        v = np.asarray(values[c])[valid]
+        if agg == "count":
+            out[c] = counts[obs].astype(np.int32)
+        elif agg in ("sum", "mean"):
+            s = np.bincount(vk, weights=v.astype(np.float64), minlength=vocab_size)[obs]
+            out[c] = s if agg == "sum" else s / counts[obs]
+        elif agg in ("min", "max"):
+            f = np.minimum if agg == "min" else np.maximum
+            r = np.full(vocab_size, np.inf if agg == "min" else -np.inf)
+            f.at(r, vk, v.astype(np.float64))
+            out[c] = r[obs]
+        else:
+            raise ValueError(f"Unsupported aggregation function: {agg}")
+    return obs.astype(np.int32), out
+
+
 # How each function is pinned; tests/test_oracle.py runs every check named here.
 PINNED_BY = {
     "ordered_bits": "closed form: unsigned order == numpy order incl. -0.0/NaN",
@@ -530,4 +630,10 @@ PINNED_BY = {
                          "nested-loop brute force for left/right/outer",
     "gather_join_column": "hand-checked cases of the three branches (string / all valid / "
                           "unmatched int -> float32 NaN)",
+    "encode_string_array": "golden vectors (test_string_operations.py:12-117) + np.unique",
+    "string_join_plan": "golden vectors (test_string_operations.py:55-117) + pandas merge "
+                        "(pair multiset; inner many-to-many order = sorted (key, left, right)) "
+                        "+ nested-loop brute force of the unique-right lookup",
+    "groupby_encoded_strings": "golden vectors (test_string_operations.py:31-53) + pandas groupby "
+                               "with dropna",
 }

@@ -79,12 +79,14 @@ def test_errors():
         df["zz"]
     with pytest.raises(TypeError):
         df[3.5]
-    eager = JaxFrame({"s": np.array(["x", "y"], dtype=object), "v": np.array([1.0, 2.0])})
-    assert eager._dtypes["s"] == "object"
+    eager = JaxFrame({"s": np.array(["x", "y"], dtype=object),
+                      "o": np.array([{"a": 1}, [2]], dtype=object), "v": np.array([1.0, 2.0])})
+    assert eager._dtypes["s"] == "string" and eager._string_vocabs["s"] == ("x", "y")
+    assert eager._dtypes["o"] == "object"
     with pytest.raises(TypeError):
-        eager.sort_values("s")
+        eager.sort_values("o")
     with pytest.raises(TypeError):
-        eager.groupby("s")
+        eager.groupby("o")
     with pytest.raises(ValueError):
         JaxFrame({"a": np.array([1, 2]), "b": np.array([1.0])})
 

@@ -217,12 +217,15 @@ def test_merge_matches_oracle_and_pruning():
         assert torch.equal(eager.data[c], full.data[c]), c
 
 
-def test_object_columns_ride_along():
-    names = np.array([f"r{i}" for i in range(6)], dtype=object)
-    df = JaxFrame({"k": np.array([3, 1, 2, 1, 3, 0]), "name": names})
+def test_object_and_string_columns_ride_along():
+    names = np.array([f"r{i}" for i in range(6)], dtype=object)          # encoded strings
+    objs = np.array([{"i": i} for i in range(6)], dtype=object)          # host objects
+    df = JaxFrame({"k": np.array([3, 1, 2, 1, 3, 0]), "name": names, "obj": objs})
+    assert df._dtypes["name"] == "string" and df._dtypes["obj"] == "object"
     s = df.sort_values("k")
-    assert list(s.data["name"]) == ["r5", "r1", "r3", "r2", "r0", "r4"]
-    f = df[df["k"] > 1]
-    assert list(f.data["name"]) == ["r0", "r2", "r4"]
     pdf = s.to_pandas()
-    assert list(pdf.columns) == ["k", "name"] and pdf.k.tolist() == [0, 1, 1, 2, 3, 3]
+    assert pdf["name"].tolist() == ["r5", "r1", "r3", "r2", "r0", "r4"]
+    assert [o["i"] for o in s.data["obj"]] == [5, 1, 3, 2, 0, 4]
+    f = df[df["k"] > 1]
+    assert f.to_pandas()["name"].tolist() == ["r0", "r2", "r4"]
+    assert list(pdf.columns) == ["k", "name", "obj"] and pdf.k.tolist() == [0, 1, 1, 2, 3, 3]

@@ -312,3 +312,118 @@ def test_gather_join_column_branches():
     assert s.tolist() == [6, -1]
     f = oracle.gather_join_column(np.array([1.5, 2.5]), np.array([-1, 1]))
     assert np.isnan(f[0]) and f[1] == 2.5
+
+
+# ---- dictionary-encoded strings ---------------------------------------------------------
+
+def _align(lc, lv, rc, rv):
+    """Both code arrays in the merged sorted vocabulary (string_encoding.py:97-112)."""
+    if lv == rv:
+        return lc, rc, lv
+    merged = tuple(sorted(set(lv) | set(rv)))
+    pos = {v: i for i, v in enumerate(merged)}
+    remap = lambda c, v: np.array([-1 if x < 0 else pos[v[x]] for x in c], dtype=np.int32)
+    return remap(lc, lv), remap(rc, rv), merged
+
+
+def _decode(codes, vocab):
+    return [None if c < 0 else vocab[c] for c in codes]
+
+
+def test_string_golden_vectors():
+    for case in _golden_section("strings"):
+        kind = case["kind"]
+        if kind == "compare":
+            codes, vocab = oracle.encode_string_array(np.array(case["values"], dtype=object))
+            code = vocab.index(case["scalar"])
+            rank = int(np.searchsorted(np.array(vocab, dtype=object), case["scalar"], "left"))
+            assert (codes == code).tolist() == case["eq"]
+            assert (codes != code).tolist() == case["ne"]
+            assert (codes > rank).tolist() == case["gt"]
+        elif kind == "sort":
+            codes, vocab = oracle.encode_string_array(np.array(case["keys"], dtype=object))
+            order = np.argsort(np.where(codes < 0, len(vocab), codes), kind="stable")
+            assert _decode(codes[order], vocab) == case["sorted_keys"]
+            assert np.asarray(case["values"])[order].tolist() == case["sorted_values"]
+        elif kind == "groupby":
+            codes, vocab = oracle.encode_string_array(np.array(case["keys"], dtype=object))
+            g, r = oracle.groupby_encoded_strings(codes, {"v": np.array(case["values"])},
+                                                  {"v": case["agg"]}, len(vocab))
+            assert _decode(g, vocab) == case["groups"]
+            assert r["v"].tolist() == case["result"]
+        else:
+            lc, lv = oracle.encode_string_array(np.array(case["left_keys"], dtype=object))
+            rc, rv = oracle.encode_string_array(np.array(case["right_keys"], dtype=object))
+            lc, rc, vocab = _align(lc, lv, rc, rv)
+            codes, li, ri = oracle.string_join_plan(lc, rc, len(vocab), case["how"])
+            keys = _decode(codes, vocab)
+            lvals = np.asarray(case["left_values"], dtype=object if isinstance(case["left_values"][0], str) else None)
+            rvals = np.asarray(case["right_values"], dtype=object if isinstance(case["right_values"][0], str) else None)
+
+            def gather(vals, idx):
+                if vals.dtype == object:
+                    return [None if i < 0 else vals[i] for i in idx]
+                out = oracle.gather_join_column(vals, idx)
+                return [None if (isinstance(x, float) and np.isnan(x)) else x for x in out.tolist()]
+            rows = list(zip(keys, gather(lvals, li), gather(rvals, ri)))
+            if case["sorted_by"] == ["key", "left"]:
+                rows.sort(key=lambda t: (t[0], t[1]))
+            elif case["sorted_by"] == ["key"]:
+                rows.sort(key=lambda t: t[0])
+            assert [r[0] for r in rows] == case["keys"]
+            assert [r[1] for r in rows] == case["left"]
+            assert [r[2] for r in rows] == case["right"]
+
+
+def test_encode_string_array_matches_np_unique():
+    rng = np.random.default_rng(3)
+    words = np.array([f"w{i:03d}" for i in rng.integers(0, 60, 500)], dtype=object)
+    words[rng.random(500) < 0.1] = None
+    codes, vocab = oracle.encode_string_array(words)
+    nn = np.array([w is not None for w in words])
+    assert list(vocab) == np.unique(words[nn].astype(str)).tolist()
+    assert np.all(codes[~nn] == -1)
+    assert [vocab[c] for c in codes[nn]] == words[nn].tolist()
+
+
+def test_string_join_plan_vs_pandas_and_bruteforce():
+    rng = np.random.default_rng(5)
+    for trial in range(6):
+        nv = 12
+        lc = rng.integers(-1, nv, 80).astype(np.int32)
+        if trial % 2:   # unique right keys (a permutation with a null)
+            rc = rng.permutation(np.arange(-1, nv))[: nv - 2].astype(np.int32)
+        else:
+            rc = rng.integers(-1, nv, 40).astype(np.int32)
+        codes, li, ri = oracle.string_join_plan(lc, rc, nv, "inner")
+        dom = lambda c: np.where(c < 0, nv, c)
+        m = pd.DataFrame({"k": dom(lc), "l": np.arange(lc.size)}).merge(
+            pd.DataFrame({"k": dom(rc), "r": np.arange(rc.size)}), on="k")
+        got = sorted(zip(dom(codes).tolist(), li.tolist(), ri.tolist()))
+        assert got == sorted(zip(m.k.tolist(), m.l.tolist(), m.r.tolist()))
+        if trial % 2:   # unique right: left row order, brute force lookup
+            exp = [(i, j) for i in range(lc.size) for j in range(rc.size) if dom(lc)[i] == dom(rc)[j]]
+            assert list(zip(li.tolist(), ri.tolist())) == exp
+        else:           # many-to-many: (code, left, right) order
+            assert list(zip(dom(codes).tolist(), li.tolist(), ri.tolist())) == got
+        for how in ("left", "right", "outer"):
+            c2, l2, r2 = oracle.string_join_plan(lc, rc, nv, how)
+            inner = sorted((a, b) for a, b in zip(l2.tolist(), r2.tolist()) if a >= 0 and b >= 0)
+            assert inner == sorted(zip(li.tolist(), ri.tolist()))
+            if how in ("left", "outer"):
+                assert set(range(lc.size)) <= set(l2.tolist())
+            if how in ("right", "outer"):
+                assert set(range(rc.size)) <= set(r2.tolist())
+
+
+def test_groupby_encoded_strings_vs_pandas():
+    rng = np.random.default_rng(8)
+    codes = rng.integers(-1, 20, 1000).astype(np.int32)
+    v = rng.standard_normal(1000)
+    g, r = oracle.groupby_encoded_strings(codes, {"s": v, "c": v, "m": v},
+                                          {"s": "sum", "c": "count", "m": "max"}, 25)
+    p = pd.DataFrame({"k": np.where(codes < 0, np.nan, codes), "v": v}).groupby("k", dropna=True).v
+    assert g.tolist() == p.sum().index.astype(int).tolist()
+    np.testing.assert_allclose(r["s"], p.sum().values, rtol=1e-12)
+    assert r["c"].tolist() == p.count().values.tolist() and r["c"].dtype == np.int32
+    assert r["m"].tolist() == p.max().values.tolist()
