@@ -86,10 +86,11 @@ def cpu_baseline(n_sample: int, seed: int):
     return n_sample / dt, dt
 
 
-def batched_vs_plain(total_rows: int, reps: int = 5, plain_frames: int = 256):
+def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 256):
     """Batched groupby of 4096 frames (row counts log-uniform in [1K, 4M]
-    scaled to total_rows, int32 keys with per-frame cardinality log-uniform
-    in [10, 1M], float32 N(0,1) values) as one HIP-graph replay, against the
+    scaled to total_rows, or unscaled when it is None; int32 keys with
+    per-frame cardinality log-uniform in [10, 1M], float32 N(0,1) values) as
+    HIP-graph replays (one per pipeline of at most 2^30 rows), against the
     plain path (one groupby_aggregate call per frame) on the first
     plain_frames frames."""
     import torch
@@ -125,7 +126,8 @@ def batched_vs_plain(total_rows: int, reps: int = 5, plain_frames: int = 256):
            "plain_rows_per_s": round(plain_rps, 1),
            "plain_sample": f"first {plain_frames} frames, {rows_plain} rows",
            "speedup_vs_plain": round(n / dt / plain_rps, 2),
-           "hip_graph": True, "digit_bits": plan.digit_bits}
+           "hip_graph": True, "digit_bits": plan.digit_bits,
+           "pipelines": 1 if plan.parts is None else len(plan.parts)}
     del plan, keys, vals
     torch.cuda.empty_cache()
     return out
@@ -205,13 +207,13 @@ def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
                                "keys_per_s": round(2e8 / pdt, 1)}}
 
 
-def only_workload(name: str, reps: int) -> dict:
+def only_workload(name: str, reps: int, batched_rows: int = 400_000_000) -> dict:
     """One workload by itself, for profiling: the kernels it launches, with
     their HIP-event time per call, and the wall time per call."""
     import torch
     from jaxframes_amd import _lib, datagen, joining, partition, sorting
     if name == "batched":
-        return {"only": name, **batched_vs_plain(400_000_000)}
+        return {"only": name, **batched_vs_plain(batched_rows or None)}
     if name == "served":
         return {"only": name, **served_vs_plain()}
     if name == "sort":
@@ -254,7 +256,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--batched-rows", type=int, default=400_000_000)
+    ap.add_argument("--batched-rows", type=int, default=400_000_000,
+                    help="rows of the batched groupby (the 4096 frames' row counts are scaled "
+                         "to it); 0 = the unscaled distribution (about 2e9 rows)")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--only", choices=("sort", "join", "partition", "batched", "served"),
                     help="run one other workload by itself (for profiling) and print its line")
@@ -273,7 +277,8 @@ def main() -> int:
     _lib.require_gpu()
     dev = torch.device("cuda", 0)
     if args.only:
-        print(json.dumps(only_workload(args.only, max(args.steps // 4, 3))), flush=True)
+        print(json.dumps(only_workload(args.only, max(args.steps // 4, 3), args.batched_rows)),
+              flush=True)
         return 0
     if args.copy_calibration:
         a = torch.ones(100_000_000, dtype=torch.float32, device=dev)
@@ -352,7 +357,7 @@ def main() -> int:
             ran.pop(k)
     step_bytes = sum((_kernel_bytes(k, n, 4, 4) or 0) * (v[1] // nb) for k, v in ran.items())
 
-    batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows)
+    batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows or None)
     served_line = None if args.no_served else served_vs_plain()
 
     cpu = None

@@ -94,6 +94,10 @@ class BatchResult:
         return self.keys[a:b], {c: v[a:b] for c, v in self.values.items()}
 
 
+# rows one device pipeline takes (its row indices and lookback counts are 30-bit)
+MAX_ROWS = (1 << 30) - 1
+
+
 class BatchedGroupBy:
     """A reusable plan for one table shape: frame offsets, key range, value
     type and aggregations fixed at construction.
@@ -101,10 +105,12 @@ class BatchedGroupBy:
     keys / values: the concatenated columns (device tensors). frame_offsets:
     int64 offsets, length nframes + 1. key_range: (min, max) of the keys over
     all frames if known, else measured once here. aggs: subset of AGGS.
+    A table of more than max_rows rows is cut at frame boundaries into
+    several pipelines, launched back to back (each frame must fit one).
     """
 
     def __init__(self, keys, values, frame_offsets, *, key_range=None, aggs=("sum",),
-                 digit_bits: int = 0, tuner: Autotuner | None = TUNER):
+                 digit_bits: int = 0, tuner: Autotuner | None = TUNER, max_rows: int = MAX_ROWS):
         _lib.require_gpu()
         for a in aggs:
             if a not in AGGS:
@@ -124,6 +130,11 @@ class BatchedGroupBy:
             else:
                 mn, mx = torch.aminmax(self.keys)
                 key_range = (int(mn), int(mx))
+        self.parts = None
+        self.aggs = tuple(aggs)
+        if self.n > max_rows:
+            self._split(key_range, digit_bits, tuner, max_rows)
+            return
         self.kmin, kmax = int(key_range[0]), int(key_range[1])
         span = kmax - self.kmin
         self.key_bits = max(1, span.bit_length())
@@ -150,6 +161,26 @@ class BatchedGroupBy:
         else:
             self.digit_bits = 8
 
+    def _split(self, key_range, digit_bits, tuner, max_rows):
+        offs = self.offsets.cpu().tolist()
+        cuts, start = [0], 0
+        for f in range(self.nframes):
+            if offs[f + 1] - offs[f] > max_rows:
+                raise ValueError(f"frame {f} has {offs[f + 1] - offs[f]} rows; at most {max_rows}")
+            if offs[f + 1] - offs[start] > max_rows:
+                cuts.append(f)
+                start = f
+        cuts.append(self.nframes)
+        self.parts = []
+        for a, b in zip(cuts, cuts[1:]):
+            r0, r1 = offs[a], offs[b]
+            sub = torch.tensor([o - r0 for o in offs[a:b + 1]], dtype=torch.int64)
+            self.parts.append(BatchedGroupBy(self.keys[r0:r1], self.vals[r0:r1], sub,
+                                             key_range=key_range, aggs=self.aggs,
+                                             digit_bits=digit_bits, tuner=tuner,
+                                             max_rows=max_rows))
+        self.digit_bits = self.parts[0].digit_bits
+
     def _launch(self, digit_bits: int | None = None, stream=None):
         s = stream if stream is not None else torch.cuda.current_stream(self.dev)
         ptr = lambda a: _lib.ptr(self.outs.get(a))
@@ -162,6 +193,8 @@ class BatchedGroupBy:
 
     def capture(self):
         """Record the pipeline in a HIP graph (replayed by run(graph=True))."""
+        if self.parts is not None:
+            return [p.capture() for p in self.parts]
         side = torch.cuda.Stream(device=self.dev)
         side.wait_stream(torch.cuda.current_stream(self.dev))
         with torch.cuda.stream(side):
@@ -175,6 +208,10 @@ class BatchedGroupBy:
 
     def launch(self, graph: bool = False):
         """Enqueue one batch (graph replay or eager launches); no host sync."""
+        if self.parts is not None:
+            for p in self.parts:
+                p.launch(graph)
+            return
         if graph:
             if self.graph is None:
                 self.capture()
@@ -184,6 +221,14 @@ class BatchedGroupBy:
 
     def result(self) -> BatchResult:
         """Read the group count (one sync) and split the composite keys."""
+        if self.parts is not None:
+            rs = [p.result() for p in self.parts]
+            offs, base = [torch.zeros(1, dtype=torch.int64, device=self.dev)], 0
+            for r in rs:
+                offs.append(r.offsets[1:] + base)
+                base += int(r.offsets[-1])
+            return BatchResult(torch.cat(offs), torch.cat([r.keys for r in rs]),
+                               {a: torch.cat([r.values[a] for r in rs]) for a in self.aggs})
         G = int(self.ngroups.item())
         comp = self.out_keys[:G]
         c64 = (comp.long() & 0xFFFFFFFF) if self.comp_dtype == torch.int32 else comp
