@@ -618,7 +618,14 @@ int tile_reduce_mm(const SortCtx& c, const SegWs& w, const void* keys, const voi
   a.n = c.n;
   a.chunk = c.chunk;
   a.part = static_cast<AggState<V, MM>*>(w.part);
-  const SmallRangePath path = small_range_path();
+  SmallRangePath path = small_range_path();
+  // no key can make the lanes of a wave collide often enough to pick the
+  // tile-sorted kernel when even the bound on the largest key count stays
+  // under the selector's threshold: launch the wave-table kernel alone
+  const uint32_t te = c.range_radix() <= 1024 ? 1024u : 2048u;
+  const uint64_t lanes = 64u / wt_lane_groups(c.range_radix(), te);
+  if (path == SR_AUTO && c.maxbin_sum != ~0ull && c.maxbin_sum * lanes * 3u <= c.n * 8u)
+    path = SR_WAVE_TABLE;
   const uint32_t* sel = path == SR_AUTO ? w.ngroups + 1 : nullptr;
   uint32_t wave_parts = 0;
   if (path != SR_TILE_SORT) {

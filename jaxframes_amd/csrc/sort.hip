@@ -89,7 +89,7 @@ struct Stats4 {
 template <typename K>
 __global__ void __launch_bounds__(STATS_BLOCK)
 k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uint32_t* counts,
-             unsigned long long* g_stats) {
+             unsigned long long* g_stats, uint32_t* g_maxbin) {
   __shared__ uint32_t s_low[2048];
   for (int i = threadIdx.x; i < 2048; i += STATS_BLOCK) s_low[i] = 0u;
   __syncthreads();
@@ -166,25 +166,48 @@ k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uin
     g_stats[(size_t)blockIdx.x * 4 + 2] = a2;
     g_stats[(size_t)blockIdx.x * 4 + 3] = a3;
   }
-  for (int t = threadIdx.x; t < 2048; t += STATS_BLOCK) counts[(size_t)blockIdx.x * 2048 + t] = s_low[t];
+  // the chunk's largest bin: summed over chunks it bounds the largest digit
+  // count from above (the small-range groupby uses it to rule out skew)
+  uint32_t mb = 0u;
+  for (int t = threadIdx.x; t < 2048; t += STATS_BLOCK) {
+    const uint32_t c = s_low[t];
+    counts[(size_t)blockIdx.x * 2048 + t] = c;
+    mb = c > mb ? c : mb;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(mb, off, 64);
+    mb = o > mb ? o : mb;
+  }
+  __shared__ uint32_t s_mb[STATS_BLOCK / 64];
+  if (lane_id() == 0) s_mb[wid] = mb;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < STATS_BLOCK / 64; ++w) mb = s_mb[w] > mb ? s_mb[w] : mb;
+    g_maxbin[blockIdx.x] = mb;
+  }
 }
 
-// Reduce the per-chunk statistics (min, max, and, or) into out[0..3].
+// Reduce the per-chunk statistics (min, max, and, or) into out[0..3] and the
+// sum of the chunks' largest low-11-bit bins into out[4].
 __global__ void __launch_bounds__(256)
-k_stats_reduce(const unsigned long long* part, uint32_t nchunks, unsigned long long* out) {
-  __shared__ unsigned long long s[4][256];
-  unsigned long long a0 = ~0ull, a1 = 0ull, a2 = ~0ull, a3 = 0ull;
+k_stats_reduce(const unsigned long long* part, const uint32_t* maxbin, uint32_t nchunks,
+               unsigned long long* out) {
+  __shared__ unsigned long long s[5][256];
+  unsigned long long a0 = ~0ull, a1 = 0ull, a2 = ~0ull, a3 = 0ull, a4 = 0ull;
   for (uint32_t b = threadIdx.x; b < nchunks; b += 256) {
     const unsigned long long* p = part + (size_t)b * 4;
     a0 = p[0] < a0 ? p[0] : a0;
     a1 = p[1] > a1 ? p[1] : a1;
     a2 &= p[2];
     a3 |= p[3];
+    a4 += maxbin[b];
   }
   s[0][threadIdx.x] = a0;
   s[1][threadIdx.x] = a1;
   s[2][threadIdx.x] = a2;
   s[3][threadIdx.x] = a3;
+  s[4][threadIdx.x] = a4;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) {
@@ -193,6 +216,7 @@ k_stats_reduce(const unsigned long long* part, uint32_t nchunks, unsigned long l
       s[1][threadIdx.x] = s[1][o] > s[1][threadIdx.x] ? s[1][o] : s[1][threadIdx.x];
       s[2][threadIdx.x] &= s[2][o];
       s[3][threadIdx.x] |= s[3][o];
+      s[4][threadIdx.x] += s[4][o];
     }
     __syncthreads();
   }
@@ -201,6 +225,7 @@ k_stats_reduce(const unsigned long long* part, uint32_t nchunks, unsigned long l
     out[1] = s[1][0];
     out[2] = s[2][0];
     out[3] = s[3][0];
+    out[4] = s[4][0];
   }
 }
 
@@ -300,8 +325,9 @@ void carve(SortCtx& c, Carver& cv) {
   c.total = cv.take<uint32_t>(2048);
   c.hist = cv.take<uint32_t>(SORT_MAXP * SORT_MAXR);
   c.base = cv.take<uint32_t>(SORT_MAXP * SORT_MAXR);
-  c.stats = cv.take<unsigned long long>(4);
+  c.stats = cv.take<unsigned long long>(5);
   c.stats_part = cv.take<unsigned long long>((uint64_t)SORT_MAXCHUNKS * 4);
+  c.maxbin = cv.take<uint32_t>(SORT_MAXCHUNKS);
   c.counters = cv.take<uint32_t>(64);
   c.hdr = cv.take<uint32_t>(16);
   c.desc[0] = cv.take<uint32_t>(os_tiles(c.n) * SORT_MAXR);
@@ -351,9 +377,11 @@ struct Passes {
 template <typename K>
 int stats_launch(SortCtx& c, const void* keys) {
   JFX_LAUNCH("sort_stats", c.s, (k_sort_stats<K><<<c.nchunks, STATS_BLOCK, 0, c.s>>>(
-                                   static_cast<const K*>(keys), c.n, c.chunk, c.oc, c.counts, c.stats_part)));
+                                   static_cast<const K*>(keys), c.n, c.chunk, c.oc, c.counts, c.stats_part,
+                                   c.maxbin)));
   JFX_HIP(hipGetLastError());
-  JFX_LAUNCH("stats_reduce", c.s, (k_stats_reduce<<<1, 256, 0, c.s>>>(c.stats_part, c.nchunks, c.stats)));
+  JFX_LAUNCH("stats_reduce", c.s, (k_stats_reduce<<<1, 256, 0, c.s>>>(c.stats_part, c.maxbin,
+                                                                      c.nchunks, c.stats)));
   JFX_HIP(hipGetLastError());
   return JFX_OK;
 }
@@ -514,9 +542,10 @@ int sort_prepare(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_by
   JFX_HIP(hipGetLastError());
   int rc = c.kb == 4 ? stats_launch<uint32_t>(c, keys_in) : stats_launch<uint64_t>(c, keys_in);
   if (rc != JFX_OK) return rc;
-  unsigned long long h[4];
+  unsigned long long h[5];
   rc = read_back(s, c.stats, sizeof(h), h);
   if (rc != JFX_OK) return rc;
+  c.maxbin_sum = h[4];
   c.mn = h[0];
   c.mx = h[1];
   c.band = h[2];

@@ -35,6 +35,10 @@ struct SortCtx {
   uint32_t* base = nullptr;     // [MAXP][MAXR] layout digit offsets
   unsigned long long* stats = nullptr;  // min, max, and, or
   unsigned long long* stats_part = nullptr;  // [nchunks][4] per-chunk statistics
+  uint32_t* maxbin = nullptr;   // [nchunks] largest low-11-bit bin of each chunk
+  // sum of the chunks' largest bins (an upper bound of the largest digit
+  // count of a small key range); unknown (all ones) for known-range sorts
+  uint64_t maxbin_sum = ~0ull;
   uint32_t* counters = nullptr;
   uint32_t* hdr = nullptr;
   uint32_t* desc[2] = {nullptr, nullptr};
