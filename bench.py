@@ -357,6 +357,7 @@ def main() -> int:
             ran.pop(k)
     step_bytes = sum((_kernel_bytes(k, n, 4, 4) or 0) * (v[1] // nb) for k, v in ran.items())
 
+    sort_based = n * 4 + 2 * n * 8 + n * 8 + ngroups * (4 + 16)
     batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows or None)
     served_line = None if args.no_served else served_vs_plain()
 
@@ -390,6 +391,11 @@ def main() -> int:
                    "path": "small key range: key statistics pass, then wave-private LDS tables"},
         "step_alg_bytes": step_bytes,
         "step_hbm_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / PEAK_HBM_GBPS, 4),
+        # the same step priced at the sort-based groupby's algorithmic bytes:
+        # key histogram N*kb + one LSD pass 2N(kb+vb) + segmented-reduce read
+        # N(kb+vb) + group outputs G*(kb+16); this path reads N*kb + N(kb+vb)
+        "step_bytes_sort_based": sort_based,
+        "step_frac_sort_based": round(sort_based / (ms_per_step / 1e3) / 1e9 / PEAK_HBM_GBPS, 4),
         "roofline": roof,
         "batched": batched_line,
         "served": served_line,
