@@ -127,7 +127,30 @@ def check(status: int, what: str):
         raise JfxError(f"{what}: {_ERRNAMES.get(status, status)}: {msg}")
 
 
+_inject_lock = threading.Lock()
+_inject: dict = {}   # entry-point name prefix -> [status, remaining calls]
+
+
+def inject_failure(prefix: str, status: int = 2, count: int = 1):
+    """Test hook: the next `count` calls whose entry-point name starts with
+    prefix fail with `status` (a JFX_ERR_* code) without running, as if the
+    library had returned it. count 0 removes the hook."""
+    with _inject_lock:
+        if count <= 0:
+            _inject.pop(prefix, None)
+        else:
+            _inject[prefix] = [status, count]
+
+
 def call(name: str, *args):
+    if _inject:
+        with _inject_lock:
+            for prefix, slot in list(_inject.items()):
+                if name.startswith(prefix):
+                    slot[1] -= 1
+                    if slot[1] <= 0:
+                        del _inject[prefix]
+                    raise JfxError(f"{name}: {_ERRNAMES.get(slot[0], slot[0])}: injected failure")
     check(getattr(lib(), name)(*args), name)
 
 

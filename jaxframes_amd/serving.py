@@ -149,6 +149,7 @@ class Request:
     footprint: int = 0
     t_admit: float = 0.0     # perf_counter() when a slot took it
     t_done: float = 0.0      # when its result was ready on the host
+    deadline: float = float("inf")   # perf_counter() by which it must be admitted
 
 
 def footprint(store: FrameStore, q) -> int:
@@ -417,6 +418,7 @@ class Server:
         self.peak_reserved = 0
         self.admitted_order: list[int] = []
         self.closed = False
+        self._timed = False   # some request was given an admission timeout
         self._rid = 0
         # one stream per slot, made here so warmup() can prime each stream's
         # allocator pool before requests arrive
@@ -426,12 +428,18 @@ class Server:
         for w in self.workers:
             w.start()
 
-    def submit(self, query, arrival: float | None = None) -> Future:
+    def submit(self, query, arrival: float | None = None, timeout: float | None = None) -> Future:
+        """Queue a query; the future resolves to (result, latency_s). timeout:
+        seconds the request may wait for admission before it fails with
+        TimeoutError. future.cancel() withdraws a request not yet admitted."""
         fp = self.footprint_fn(self.store, query)
         with self.cv:
             self._rid += 1
-            req = Request(query, time.perf_counter() if arrival is None else arrival, rid=self._rid,
-                          footprint=fp)
+            now = time.perf_counter()
+            req = Request(query, now if arrival is None else arrival, rid=self._rid, footprint=fp,
+                          deadline=now + timeout if timeout is not None else float("inf"))
+            if timeout is not None:
+                self._timed = True
             if fp > self.budget:
                 req.future.set_exception(MemoryError(
                     f"request {req.rid} needs {fp} B, more than the {self.budget} B budget"))
@@ -441,14 +449,36 @@ class Server:
             self.cv.notify_all()
         return req.future
 
+    def _expire(self):
+        # called with the lock held: fail the requests whose admission
+        # deadline has passed (they never reach a slot) and drop cancelled
+        # ones. Only servers that were given a timeout pay for the sweep;
+        # otherwise cancelled requests are dropped when they are popped.
+        if not self._timed:
+            return
+        now = time.perf_counter()
+        keep = deque()
+        for r in self.pending:
+            if r.future.cancelled():
+                continue
+            if now > r.deadline:
+                r.future.set_exception(TimeoutError(
+                    f"request {r.rid} was not admitted within its timeout"))
+                continue
+            keep.append(r)
+        self.pending = keep
+
     def _admit(self) -> Request | None:
         # called with the lock held: FIFO head, if a slot and the memory allow
+        self._expire()
         if not self.pending or self.running >= self.slots:
             return None
         head = self.pending[0]
         if self.reserved + head.footprint > self.budget:
             return None
         self.pending.popleft()
+        if not head.future.set_running_or_notify_cancel():
+            return self._admit()      # cancelled between the sweep and now
         self.running += 1
         self.reserved += head.footprint
         self.peak_running = max(self.peak_running, self.running)
@@ -467,8 +497,11 @@ class Server:
         keep = deque()
         while self.pending and len(batch) < self.max_batch:
             r = self.pending.popleft()
+            if r.future.cancelled():
+                continue
             if (self.batchable_fn(self.store, r.query)
-                    and self.reserved + r.footprint <= self.budget):
+                    and self.reserved + r.footprint <= self.budget
+                    and r.future.set_running_or_notify_cancel()):
                 self.reserved += r.footprint
                 self.peak_reserved = max(self.peak_reserved, self.reserved)
                 self.admitted_order.append(r.rid)
@@ -507,7 +540,12 @@ class Server:
                 while req is None:
                     if self.closed and not self.pending:
                         return
-                    self.cv.wait()
+                    wait = None
+                    if self._timed:   # wake up for the nearest admission deadline
+                        nxt = min((r.deadline for r in self.pending), default=float("inf"))
+                        if nxt != float("inf"):
+                            wait = max(nxt - time.perf_counter(), 0.0) + 1e-4
+                    self.cv.wait(wait)
                     req = self._admit()
                 batch = self._take_batch(req)
                 self.batches.append(len(batch))

@@ -180,3 +180,61 @@ def test_switch_interval_set_while_serving_and_restored():
     assert abs(sys.getswitchinterval() - 1e-4) < 1e-9
     srv.close()
     assert sys.getswitchinterval() == before
+
+
+def test_admission_timeout_fails_only_the_late_request():
+    srv, _ = _make(slots=1, budget=100)
+    blocker = srv.submit(_Q(0, 10, dur=0.25))
+    time.sleep(0.02)
+    late = srv.submit(_Q(1, 10, dur=0.0), timeout=0.05)
+    patient = srv.submit(_Q(2, 10, dur=0.0))
+    with pytest.raises(TimeoutError):
+        late.result(timeout=5)
+    assert blocker.result()[0] == 0 and patient.result()[0] == 2
+    assert 2 not in srv.admitted_order
+    srv.close()
+    assert srv.running == 0 and srv.reserved == 0
+
+
+def test_cancelled_request_never_runs():
+    log = []
+    srv, _ = _make(slots=1, budget=100, log=log)
+    blocker = srv.submit(_Q(0, 10, dur=0.15))
+    time.sleep(0.02)
+    gone = srv.submit(_Q(1, 10, dur=0.0))
+    kept = srv.submit(_Q(2, 10, dur=0.0))
+    assert gone.cancel()
+    assert blocker.result()[0] == 0 and kept.result()[0] == 2
+    srv.close()
+    assert [q for _, q, _ in log] == [0, 2]
+    assert not blocker.cancel()     # admitted requests cannot be withdrawn
+
+
+def test_injected_library_error_fails_one_request_without_wedging():
+    from jaxframes_amd import _lib
+
+    import ctypes
+
+    def ex(store, q):
+        nb = ctypes.c_uint64(0)          # a real C-ABI call that needs no GPU
+        _lib.call("jfx_compact_workspace_size", 0, ctypes.byref(nb))
+        return q.qid
+
+    srv = Server(FrameStore(100), slots=2, hbm_budget_bytes=100, executor=ex, use_streams=False,
+                 footprint_fn=lambda st, q: 1, batch_executor=None)
+    _lib.inject_failure("jfx_compact_workspace_size", status=2, count=1)
+    try:
+        futs = [srv.submit(_Q(i, 1)) for i in range(6)]
+        outcomes = []
+        for f in futs:
+            try:
+                outcomes.append(f.result(timeout=5)[0])
+            except _lib.JfxError as e:
+                assert "injected" in str(e)
+                outcomes.append("error")
+    finally:
+        _lib.inject_failure("jfx_compact_workspace_size", count=0)
+        srv.close()
+    assert outcomes.count("error") == 1
+    assert sorted(o for o in outcomes if o != "error") == sorted(set(range(6)) - {outcomes.index("error")})
+    assert srv.running == 0 and srv.reserved == 0
