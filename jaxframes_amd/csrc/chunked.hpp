@@ -29,7 +29,6 @@ struct ChunkGeom {
 enum CountMode : int {
   CM_DIRECT = 0,   // counts[b][d], stride R
   CM_ROTATE = 1,   // low-11-bit counts rotated by `sub`: counts[b][(d + sub) & 2047]
-  CM_FOLD8 = 2,    // low-11-bit counts folded to 8 bits: sum over k of counts[b][d + 256k]
 };
 
 // Column scan, one wave per digit: lane l sums chunks [l*per, (l+1)*per),
@@ -40,11 +39,7 @@ __device__ __forceinline__ uint32_t chunk_count(const uint32_t* counts, uint32_t
                                                 uint32_t sub, uint32_t b, uint32_t d) {
   const uint32_t* row = counts + (size_t)b * cstride;
   if (mode == CM_DIRECT) return row[d];
-  if (mode == CM_ROTATE) return row[(d + sub) & 2047u];
-  uint32_t c = 0;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) c += row[d + 256 * k];
-  return c;
+  return row[(d + sub) & 2047u];
 }
 
 static __global__ void __launch_bounds__(256)
