@@ -761,7 +761,10 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
   JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "groupby workspace too small");
   hipStream_t s = jfx::as_stream(stream);
   jfx::SortCtx ctx;
-  int rc = jfx::sort_prepare(ctx, key_type, keys, n, vb, digit_bits, sort_ws, sws, s);
+  // float keys never take the small-range path: their statistics pass also
+  // makes the layout histograms of the multi-pass sort
+  int rc = jfx::sort_prepare(ctx, key_type, keys, n, vb, digit_bits, sort_ws, sws, s,
+                             !jfx::int_key(key_type));
   if (rc != JFX_OK) return rc;
   const int oc = jfx::order_code(key_type);
   const bool small = jfx::int_key(key_type) && ctx.single_pass();

@@ -5,14 +5,15 @@
 // each varying digit is one scatter pass over HBM. Pipeline per call:
 //   k_sort_stats     one read of the keys, one workgroup per chunk: min, max,
 //                    AND and OR of the ordered keys, and each chunk's
-//                    histogram of the low 11 bits
+//                    histogram of the low 11 bits; for jfx_sort with 8-bit
+//                    digits also every layout digit's histogram
 //   host             reads the four statistics (one stream sync)
 //   max-min < 2048:  one chunked pass with digit = ordered(key) - min, its
 //                    per-chunk counts being the chunk histograms rotated by min
-//   otherwise:       k_sort_hist (histograms of the varying layout digits),
-//                    k_sort_plan (digit offsets), then one pass per varying
-//                    digit: the first (digit 0) chunked, the rest onesweep
-//                    with decoupled lookback. Digits on which all keys agree
+//   otherwise:       k_sort_hist (histograms of the varying layout digits,
+//                    unless the statistics pass made them), k_sort_plan
+//                    (digit offsets), then one onesweep pass with decoupled
+//                    lookback per varying digit. Digits on which all keys agree
 //                    are skipped (the reference's skip test at :512 only
 //                    skips leading zero digits and never fires for signed keys).
 // Descending order is the reversed ascending order (:552-555), written by the
@@ -86,13 +87,38 @@ struct Stats4 {
 };
 
 // One workgroup per chunk: statistics and the chunk's low-11-bit histogram.
-template <typename K>
+// With HIST it also adds the chunk's 8-bit digit histograms of every layout
+// pass into g_hist[pass][256], so a multi-pass sort reads its keys once
+// before the scatter passes instead of twice: digit 0 is folded from the
+// low-11-bit histogram, the higher digits take one LDS atomic each (one per
+// wave when the whole wave agrees on the digit).
+template <typename K, bool HIST>
 __global__ void __launch_bounds__(STATS_BLOCK)
 k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uint32_t* counts,
-             unsigned long long* g_stats, uint32_t* g_maxbin) {
+             unsigned long long* g_stats, uint32_t* g_maxbin, uint32_t* g_hist) {
+  constexpr int NP = (int)sizeof(K);  // 8-bit layout digits
+  constexpr int NH = HIST ? (NP - 1) * 256 : 1;
   __shared__ uint32_t s_low[2048];
+  __shared__ uint32_t s_h[NH];
   for (int i = threadIdx.x; i < 2048; i += STATS_BLOCK) s_low[i] = 0u;
+  if (HIST)
+    for (int i = threadIdx.x; i < NH; i += STATS_BLOCK) s_h[i] = 0u;
   __syncthreads();
+  const uint32_t lane = lane_id();
+  auto digits = [&](K o) {
+    if (!HIST) return;
+    const uint64_t act = ballot64(true);
+#pragma unroll
+    for (int p = 1; p < NP; ++p) {
+      const uint32_t bin = (uint32_t)((o >> (p * 8)) & (K)255);
+      const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
+      if (ballot64(bin == b0) == act) {
+        if (lane == (uint32_t)__builtin_ctzll(act)) atomicAdd(&s_h[(p - 1) * 256 + b0], (uint32_t)__popcll(act));
+      } else {
+        atomicAdd(&s_h[(p - 1) * 256 + bin], 1u);
+      }
+    }
+  };
   const uint64_t cbeg = (uint64_t)blockIdx.x * chunk;
   const uint64_t cend = (cbeg + chunk) < n ? (cbeg + chunk) : n;
   Stats4<K> st;
@@ -114,6 +140,7 @@ k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uin
         const K o = ordered(e[j], oc);
         st.add(o);
         atomicAdd(&s_low[(uint32_t)o & 2047u], 1u);
+        digits(o);
       }
     }
   }
@@ -125,12 +152,14 @@ k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uin
       const K o = ordered(e[j], oc);
       st.add(o);
       atomicAdd(&s_low[(uint32_t)o & 2047u], 1u);
+      digits(o);
     }
   }
   for (uint64_t t = cbeg + nv * VEC + threadIdx.x; t < cend; t += STATS_BLOCK) {
     const K o = ordered(keys[t], oc);
     st.add(o);
     atomicAdd(&s_low[(uint32_t)o & 2047u], 1u);
+    digits(o);
   }
   unsigned long long a0 = st.mn, a1 = st.mx, a2 = st.an, a3 = st.orr;
   if (sizeof(K) == 4) a2 |= 0xFFFFFFFF00000000ull;
@@ -185,6 +214,19 @@ k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uin
   if (threadIdx.x == 0) {
     for (int w = 1; w < STATS_BLOCK / 64; ++w) mb = s_mb[w] > mb ? s_mb[w] : mb;
     g_maxbin[blockIdx.x] = mb;
+  }
+  if (HIST) {
+    // s_low and s_h are complete: the barrier above follows every atomic
+    if (threadIdx.x < 256) {
+      uint32_t d0 = 0u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) d0 += s_low[threadIdx.x + 256 * j];
+      if (d0) atomicAdd(&g_hist[threadIdx.x], d0);
+    }
+    for (int t = threadIdx.x; t < NH; t += STATS_BLOCK) {
+      const uint32_t v = s_h[t];
+      if (v) atomicAdd(&g_hist[256 + t], v);
+    }
   }
 }
 
@@ -375,11 +417,18 @@ struct Passes {
 };
 
 template <typename K>
-int stats_launch(SortCtx& c, const void* keys) {
-  JFX_LAUNCH("sort_stats", c.s, (k_sort_stats<K><<<c.nchunks, STATS_BLOCK, 0, c.s>>>(
-                                   static_cast<const K*>(keys), c.n, c.chunk, c.oc, c.counts, c.stats_part,
-                                   c.maxbin)));
+int stats_launch(SortCtx& c, const void* keys, bool with_hist) {
+  if (with_hist) {
+    JFX_LAUNCH("sort_stats_hist", c.s, (k_sort_stats<K, true><<<c.nchunks, STATS_BLOCK, 0, c.s>>>(
+                                          static_cast<const K*>(keys), c.n, c.chunk, c.oc, c.counts,
+                                          c.stats_part, c.maxbin, c.hist)));
+  } else {
+    JFX_LAUNCH("sort_stats", c.s, (k_sort_stats<K, false><<<c.nchunks, STATS_BLOCK, 0, c.s>>>(
+                                     static_cast<const K*>(keys), c.n, c.chunk, c.oc, c.counts,
+                                     c.stats_part, c.maxbin, nullptr)));
+  }
   JFX_HIP(hipGetLastError());
+  c.hist_done = with_hist;
   JFX_LAUNCH("stats_reduce", c.s, (k_stats_reduce<<<1, 256, 0, c.s>>>(c.stats_part, c.maxbin,
                                                                       c.nchunks, c.stats)));
   JFX_HIP(hipGetLastError());
@@ -423,9 +472,11 @@ int multi_pass_typed(SortCtx& c, const void* keys_in, void* keys_out, int want_k
   uint32_t vary = 0;
   for (int p = 0; p < npass; ++p)
     if ((diff >> (p * D)) & dmask) vary |= 1u << p;
-  // histograms of the varying digits and their offsets
+  // histograms of the varying digits (unless the statistics pass already
+  // made all of them) and their offsets
   const uint32_t ub = 512;
-  if (D == 8) {
+  if (c.hist_done && D == 8) {
+  } else if (D == 8) {
     JFX_LAUNCH("sort_hist", c.s, (k_sort_hist<K, 8><<<ub, HIST_BLOCK, 0, c.s>>>(
                                      static_cast<const K*>(keys_in), c.n, c.oc, vary, c.hist)));
   } else {
@@ -509,7 +560,7 @@ int default_digit_bits(int kt, uint64_t n) {
 }
 
 int sort_prepare(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_bytes,
-                 int digit_bits, void* ws, uint64_t ws_bytes, hipStream_t s) {
+                 int digit_bits, void* ws, uint64_t ws_bytes, hipStream_t s, bool with_hist) {
   c.kt = kt;
   c.kb = key_bytes(kt);
   c.oc = order_code(kt);
@@ -527,7 +578,8 @@ int sort_prepare(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_by
   JFX_LAUNCH("sort_init", s, (k_sort_init<<<64, 256, 0, s>>>(c.stats, c.counters, 64, c.hist,
                                                              SORT_MAXP * SORT_MAXR)));
   JFX_HIP(hipGetLastError());
-  int rc = c.kb == 4 ? stats_launch<uint32_t>(c, keys_in) : stats_launch<uint64_t>(c, keys_in);
+  const bool hist = with_hist && c.digit_bits == 8;
+  int rc = c.kb == 4 ? stats_launch<uint32_t>(c, keys_in, hist) : stats_launch<uint64_t>(c, keys_in, hist);
   if (rc != JFX_OK) return rc;
   unsigned long long h[5];
   rc = read_back(s, c.stats, sizeof(h), h);
@@ -567,7 +619,8 @@ int sort_prepare_known(SortCtx& c, int kt, const void* keys_in, uint64_t n, int 
   // a single counting pass (keys below 2048) takes its chunk histograms from
   // the statistics kernel; layout passes need only the digit histograms
   if (top < 2048ull) {
-    const int rc = c.kb == 4 ? stats_launch<uint32_t>(c, keys_in) : stats_launch<uint64_t>(c, keys_in);
+    const int rc = c.kb == 4 ? stats_launch<uint32_t>(c, keys_in, false)
+                             : stats_launch<uint64_t>(c, keys_in, false);
     if (rc != JFX_OK) return rc;
   }
   c.mn = 0;
@@ -610,7 +663,7 @@ int sort_dispatch(int kt, const void* keys_in, void* keys_out, int want_keys, in
                   const void* pay_in, void* pay_out, uint64_t n, int descending, int digit_bits,
                   void* ws, uint64_t ws_bytes, hipStream_t s) {
   SortCtx c;
-  int rc = sort_prepare(c, kt, keys_in, n, pay_bytes, digit_bits, ws, ws_bytes, s);
+  int rc = sort_prepare(c, kt, keys_in, n, pay_bytes, digit_bits, ws, ws_bytes, s, true);
   if (rc != JFX_OK) return rc;
   const int iota = (pay_bytes == 4 && pay_in == nullptr) ? 1 : 0;
   return sort_execute(c, keys_in, keys_out, want_keys, pay_in, pay_out, iota, descending);

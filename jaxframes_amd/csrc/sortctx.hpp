@@ -39,6 +39,8 @@ struct SortCtx {
   // sum of the chunks' largest bins (an upper bound of the largest digit
   // count of a small key range); unknown (all ones) for known-range sorts
   uint64_t maxbin_sum = ~0ull;
+  // the statistics pass also left every 8-bit layout histogram in hist
+  bool hist_done = false;
   uint32_t* counters = nullptr;
   uint32_t* hdr = nullptr;
   uint32_t* desc[2] = {nullptr, nullptr};
@@ -52,9 +54,12 @@ uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n);
 int default_digit_bits(int kt, uint64_t n);
 
 // Carve the workspace and run the statistics pass; synchronises the stream
-// once to read min / max / AND / OR of the ordered keys into ctx.
+// once to read min / max / AND / OR of the ordered keys into ctx. With
+// with_hist (8-bit digits only) the same read also makes the layout
+// histograms, for callers that will most likely run a multi-pass sort.
 int sort_prepare(SortCtx& ctx, int kt, const void* keys_in, uint64_t n, int pay_bytes,
-                 int digit_bits, void* ws, uint64_t ws_bytes, hipStream_t s);
+                 int digit_bits, void* ws, uint64_t ws_bytes, hipStream_t s,
+                 bool with_hist = false);
 
 // Run all scatter passes (single chunked pass when max-min < 2048, else the
 // layout passes with constant digits skipped).
