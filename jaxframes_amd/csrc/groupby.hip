@@ -565,7 +565,10 @@ constexpr int wt_waves(int te, size_t esz) {
 template <typename K, typename V, bool MM, int TE>
 int wave_table_te(const SortCtx& c, const SegWs& w, const void* keys, const void* vals,
                   const uint32_t* sel, hipStream_t s, uint32_t& nparts) {
-  constexpr int W = wt_waves(TE, sizeof(WAcc<V, MM>) + sizeof(uint32_t));
+  // sums of float32 or integer values: LDS-atomic tables (one word per digit)
+  constexpr bool ATOMIC = !MM && !std::is_same<V, double>::value;
+  constexpr size_t ESZ = ATOMIC ? sizeof(typename WaAcc<V>::T) : sizeof(WAcc<V, MM>) + sizeof(uint32_t);
+  constexpr int W = wt_waves(TE, ESZ);
   constexpr uint64_t STEP = 64u * WT_RPL;
   WaveTabArgs<K, V, MM> a{};
   a.keys = static_cast<const K*>(keys);
@@ -576,9 +579,10 @@ int wave_table_te(const SortCtx& c, const SegWs& w, const void* keys, const void
   a.lg = wt_lane_groups(a.R, TE);
   a.rt = (uint32_t)TE / a.lg;
   a.n = c.n;
-  // Three blocks per CU fit in LDS: up to 768 blocks, each wave a whole
-  // number of steps, within the partial buffer (SORT_MAXCHUNKS x 2048).
-  const uint64_t cap = std::min<uint64_t>(768, (uint64_t)SORT_MAXCHUNKS * 2048 / a.R);
+  // As many blocks as fit in LDS at once (160 KB per CU, 256 CUs), each wave
+  // a whole number of steps, within the partial buffer (SORT_MAXCHUNKS x 2048).
+  const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / ((uint64_t)W * TE * ESZ));
+  const uint64_t cap = std::min<uint64_t>(256 * per_cu, (uint64_t)SORT_MAXCHUNKS * 2048 / a.R);
   const uint64_t want = (c.n + (uint64_t)W * STEP - 1) / ((uint64_t)W * STEP);
   uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, want));
   a.rows_per_wave = ((c.n + (uint64_t)grid * W - 1) / ((uint64_t)grid * W) + STEP - 1) / STEP * STEP;
@@ -588,7 +592,13 @@ int wave_table_te(const SortCtx& c, const SegWs& w, const void* keys, const void
   a.part = static_cast<AggState<V, MM>*>(w.part);
   a.err = c.err;
   const char* nm = TE == 1024 ? "wave_table_1k" : "wave_table_2k";
-  if (a.vec) {
+  if constexpr (ATOMIC) {
+    if (a.vec) {
+      JFX_LAUNCH(nm, s, (k_wave_atomic<K, V, TE, W, true><<<grid, W * 64, 0, s>>>(a)));
+    } else {
+      JFX_LAUNCH(nm, s, (k_wave_atomic<K, V, TE, W, false><<<grid, W * 64, 0, s>>>(a)));
+    }
+  } else if (a.vec) {
     JFX_LAUNCH(nm, s, (k_wave_table<K, V, MM, TE, W, true><<<grid, W * 64, 0, s>>>(a)));
   } else {
     JFX_LAUNCH(nm, s, (k_wave_table<K, V, MM, TE, W, false><<<grid, W * 64, 0, s>>>(a)));

@@ -50,13 +50,16 @@ template <> struct ValTraits<uint64_t> {
   __device__ static uint64_t highest() { return ~0ull; }
 };
 
-// Exact rounding error of a + b (TwoSum, no branches, no FMA involved).
+// Exact rounding error of a + b (TwoSum, no branches, no FMA involved). When
+// the sum is not finite the error is 0, so that a sum that reached +-inf
+// stays +-inf (the formula would give inf - inf = NaN).
 template <typename V>
 __device__ __forceinline__ V two_sum(V a, V b, V& err) {
   const V s = a + b;
   const V bp = s - a;
   const V ap = s - bp;
-  err = (a - ap) + (b - bp);
+  const V e = (a - ap) + (b - bp);
+  err = __builtin_isfinite(s) ? e : V(0);
   return s;
 }
 

@@ -30,6 +30,8 @@
 // block order.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "segreduce.hpp"
 
@@ -334,6 +336,129 @@ __global__ void __launch_bounds__(W * 64) k_wave_table(WaveTabArgs<K, V, MM> a) 
     for (int w = 0; w < W; ++w)
       for (uint32_t g = 0; g < a.lg; ++g) acc.merge(wacc_state(s_acc[w * TE + g * a.rt + d]));
     a.part[(size_t)blockIdx.x * a.R + d] = acc;
+  }
+}
+
+// ---- sums without min / max: tables updated by LDS atomics -------------------
+//
+// Without min / max an entry is only a sum, and one LDS atomic add per
+// flushed run replaces the tag round: a double per digit for float32 values
+// (ds_add_f64; measured ~27 cycles per wave instruction on gfx950 against
+// ~190 for ds_add_f32 and ~80 for a tag round with its reads and write), the
+// value type's unsigned integer for integer values (ds_add_u32 / u64, exact
+// wrapping sums). Runs are folded in the same accumulator type in registers.
+// Adding float32 values in double rounds each add at 2^-53 of the partial
+// sum, no worse than the compensated float pair of the tag-round kernel.
+// Lanes of one instruction that add to the same entry are serialised by the
+// LDS atomic unit in a fixed order and the tables are wave-private, so the
+// result is still a fixed function of the row layout. float64 values keep the
+// tag-round kernel (their compensated double-double sums need two words).
+template <typename V> struct WaAcc {
+  using T = typename ValTraits<V>::U;  // integer values: wrapping unsigned sums
+};
+template <> struct WaAcc<float> {
+  using T = double;
+};
+
+template <typename K, typename V, int TE, int W, bool VEC>
+__global__ void __launch_bounds__(W * 64) k_wave_atomic(WaveTabArgs<K, V, false> a) {
+  static_assert(!std::is_same<V, double>::value, "float64 sums use k_wave_table");
+  using T = typename WaAcc<V>::T;
+  using St = AggState<V, false>;
+  constexpr uint64_t STEP = 64u * WT_RPL;
+  __shared__ T s_acc[W * TE];
+  if (a.sel && *a.sel != 0u) return;
+
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = lane_id();
+  T* accs = s_acc + wid * TE;
+  for (uint32_t i = lane; i < (uint32_t)TE; i += 64) accs[i] = T(0);
+  lds_compiler_fence();
+
+  accs += (lane & (a.lg - 1u)) * a.rt;
+  const uint32_t dmask = a.rt - 1u;
+  const uint64_t wave = (uint64_t)blockIdx.x * W + wid;
+  const uint64_t wbeg = wave * a.rows_per_wave;
+  const uint64_t wend = (wbeg + a.rows_per_wave) < a.n ? (wbeg + a.rows_per_wave) : a.n;
+  const K rawmin = a.sub;
+
+  uint32_t run_d = WT_NO_RUN;
+  T run = T(0);
+  auto flush = [&](uint32_t d, T v, bool pend) {
+    if (pend) __hip_atomic_fetch_add(&accs[d], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+  };
+  // fold one row into the lane's run, flushing the run it ends
+  auto row = [&](K key, V x) {
+    const uint32_t d = (uint32_t)(key - rawmin) & dmask;
+    const T xv = (T)x;
+    flush(run_d & dmask, run, d != run_d && run_d != WT_NO_RUN);
+    run = d == run_d ? run + xv : xv;
+    run_d = d;
+  };
+
+  uint64_t base = wbeg;
+  if (VEC && wbeg + STEP <= wend) {
+    // the loads are unconditional (clamped to the last whole step) so that
+    // the in-order wait counters let the next step stay in flight
+    const uint64_t last = wbeg + ((wend - wbeg) / STEP - 1) * STEP;
+    K ka[WT_RPL], kb[WT_RPL];
+    V va[WT_RPL], vb[WT_RPL];
+    auto load = [&](uint64_t b, K (&kk)[WT_RPL], V (&vv)[WT_RPL]) {
+      const uint64_t bc = (b < last ? b : last) + (uint64_t)lane * WT_RPL;
+      wt_load_rows(a.keys + bc, kk);
+      wt_load_rows(a.vals + bc, vv);
+    };
+    auto fold = [&](K (&kk)[WT_RPL], V (&vv)[WT_RPL]) {
+#pragma unroll
+      for (int i = 0; i < WT_RPL; ++i) row(kk[i], vv[i]);
+    };
+    load(base, ka, va);
+    __builtin_amdgcn_sched_barrier(0);
+    load(base + STEP, kb, vb);
+    __builtin_amdgcn_sched_barrier(0);
+    while (true) {
+      fold(ka, va);
+      base += STEP;
+      if (base > last) break;
+      load(base + STEP, ka, va);
+      __builtin_amdgcn_sched_barrier(0);
+      fold(kb, vb);
+      base += STEP;
+      if (base > last) break;
+      load(base + STEP, kb, vb);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // the lane's last run, then rows [base, wend) one row per lane per
+  // iteration, each added at once
+  flush(run_d & dmask, run, run_d != WT_NO_RUN);
+  for (uint64_t r0 = base; r0 < wend; r0 += 64) {
+    const uint64_t r = r0 + lane;
+    const bool valid = r < wend;
+    uint32_t d = 0u;
+    T v = T(0);
+    if (valid) {
+      v = (T)a.vals[r];
+      d = (uint32_t)(a.keys[r] - rawmin) & dmask;
+    }
+    flush(d, v, valid);
+  }
+
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < a.R; d += W * 64) {
+    T acc = T(0);
+    for (int w = 0; w < W; ++w)
+      for (uint32_t g = 0; g < a.lg; ++g) acc += s_acc[w * TE + g * a.rt + d];
+    St st = St::identity();
+    if constexpr (std::is_same<V, float>::value) {
+      // the double as a float pair s + c (c the rounding error of s, 0 when
+      // s is not finite)
+      st.s = (float)acc;
+      st.c = __builtin_isfinite(st.s) ? (float)(acc - (double)st.s) : 0.0f;
+    } else {
+      st.s = (V)acc;
+    }
+    a.part[(size_t)blockIdx.x * a.R + d] = st;
   }
 }
 

@@ -191,3 +191,98 @@ def test_output_capacity_retry_and_small_outputs():
     assert st == _lib.JFX_ERR_CAPACITY and g.value == n
     torch.cuda.synchronize()
     assert torch.all(out_k == 7) and torch.all(out_s == 7.0)
+
+
+def _wrapping_sums(sv, starts):
+    """Integer group sums wrapping in the value dtype (the device's rule)."""
+    u = np.uint64 if sv.dtype.itemsize == 8 else np.uint32
+    wide = sv.view(u).astype(np.uint64)
+    return np.add.reduceat(wide, starts).astype(u).view(sv.dtype)
+
+
+@pytest.mark.parametrize("vdt", [np.float32, np.int32, np.int64, np.uint32, np.uint64, np.float64])
+@pytest.mark.parametrize("R", [1, 9, 64, 200, 1000, 1024, 1500, 2048])
+def test_small_range_sum_only(vdt, R):
+    """Sum-only groupbys over a small key range (R distinct keys, negative
+    minimum): float32 and integer values take the LDS-atomic tables, float64
+    the tag-round tables; lane-group tables (R <= 256) and 2048-entry tables
+    (R > 1024) included. An unaligned view takes the row-by-row loads."""
+    rng = np.random.default_rng(R * 7 + np.dtype(vdt).itemsize)
+    n = 300_007
+    keys = (rng.integers(0, R, n) - 3).astype(np.int32)
+    if np.dtype(vdt).kind == "f":
+        vals = (rng.standard_normal(n) * 100).astype(vdt)
+    else:
+        info = np.iinfo(vdt)
+        vals = rng.integers(info.min, info.max, n, dtype=vdt, endpoint=True)   # sums wrap
+    for off in (0, 1):
+        kk, vv = keys[off:], vals[off:]
+        uk, res = grouping.groupby_aggregate(cuda(keys)[off:], {"v": cuda(vals)[off:]}, {"v": "sum"})
+        sk, sv, starts = _oracle_groups(kk, vv)
+        assert np.array_equal(to_numpy(uk), sk[starts])
+        got = to_numpy(res["v"])
+        if np.dtype(vdt).kind == "f":
+            assert got.dtype == vdt
+            _check_float_sums(got, sv, starts, vdt)
+        else:
+            assert np.array_equal(got, _wrapping_sums(sv, starts))
+
+
+def test_small_range_sum_only_skewed_and_clustered():
+    """One key holding most rows (the skew selector may pick the tile-sorted
+    kernel) and sorted keys (long register runs) give the same sums."""
+    rng = np.random.default_rng(3)
+    n = 1_000_003
+    skew = np.where(rng.random(n) < 0.9, 17, rng.integers(0, 1000, n)).astype(np.int32)
+    clustered = np.sort(rng.integers(0, 1000, n)).astype(np.int32)
+    vals = rng.standard_normal(n).astype(np.float32)
+    for keys in (skew, clustered):
+        uk, res = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "sum"})
+        sk, sv, starts = _oracle_groups(keys, vals)
+        assert np.array_equal(to_numpy(uk), sk[starts])
+        _check_float_sums(to_numpy(res["v"]), sv, starts, np.float32)
+
+
+@pytest.mark.parametrize("vdt", [np.float32, np.float64])
+@pytest.mark.parametrize("span", [1, 10**7])
+@pytest.mark.parametrize("stats", [False, True])
+def test_float_sums_with_infinities(vdt, span, stats):
+    """Groups holding +inf, -inf, both, NaN, or (float32) values whose sum
+    overflows: +-inf, NaN and inf as the FP64 sum rounded to the value type
+    gives them, on the small-range kernels (span 1) and on the sort-based path
+    (span 1e7); sum-only and with min / max."""
+    rng = np.random.default_rng(5)
+    n = 200_003
+    g = rng.integers(0, 50, n)
+    keys = (g * span).astype(np.int64)
+    vals = rng.standard_normal(n).astype(vdt)
+    vals[np.flatnonzero(g == 0)[::97]] = np.inf
+    vals[np.flatnonzero(g == 1)[::89]] = -np.inf
+    both = np.flatnonzero(g == 2)
+    vals[both[::50]] = np.inf
+    vals[both[25::50]] = -np.inf
+    vals[np.flatnonzero(g == 3)[::101]] = np.nan
+    if vdt == np.float32:
+        vals[np.flatnonzero(g == 4)[:3]] = 3e38
+    kt, vt = cuda(keys), cuda(vals)
+    if stats:
+        uk, res = grouping.groupby_stats(kt, vt)
+        got = to_numpy(res["sum"])
+    else:
+        uk, res = grouping.groupby_aggregate(kt, {"v": vt}, {"v": "sum"})
+        got = to_numpy(res["v"])
+    sk, sv, starts = _oracle_groups(keys, vals)
+    assert np.array_equal(to_numpy(uk), sk[starts])
+    with np.errstate(over="ignore", invalid="ignore"):
+        ref = np.add.reduceat(sv.astype(np.float64), starts).astype(vdt)
+    fin = np.isfinite(ref)
+    assert np.array_equal(got[~fin], ref[~fin], equal_nan=True), (got[:6], ref[:6])
+    assert (~fin).sum() == (5 if vdt == np.float32 else 4)
+    # finite groups against correctly rounded sums (math.fsum)
+    import math
+    ends = np.append(starts[1:], len(sv))
+    exact = np.array([math.fsum(sv[a:b].astype(np.float64)) if f else 0.0
+                      for a, b, f in zip(starts, ends, fin)])
+    absx = np.add.reduceat(np.abs(sv.astype(np.float64)), starts)
+    r = np.abs(got[fin].astype(np.float64) - exact[fin]) / absx[fin]
+    assert r.max() <= (SUM_NORM_TOL if vdt == np.float32 else SUM64_NORM_TOL), r.max()
