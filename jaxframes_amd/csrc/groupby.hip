@@ -14,6 +14,7 @@
 //                      trailing partial.
 //   k_segreduce_fix    one workgroup scans the per-tile partials and writes
 //                      the groups that cross tile boundaries, plus G.
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
@@ -24,6 +25,7 @@
 #include "sortctx.hpp"
 #include "tilereduce.hpp"
 #include "wavetable.hpp"
+#include "onepass.hpp"
 #include "batched.hpp"
 
 namespace jfx {
@@ -416,6 +418,8 @@ struct SegWs {
   uint32_t* seg_start;
   uint32_t* dgid;   // [2048] group index of each digit (small-range path)
   void* part;       // [chunks][2048] chunk accumulators (small-range path)
+  uint64_t* op_mm;  // [SORT_MAXCHUNKS * 2][2] one-pass: block min / max of the ordered keys
+  uint32_t* op_ctl; // [4] one-pass: stop word, (unused), result words
 };
 
 uint64_t seg_tiles(uint64_t n) { return (n + SEG_TILE - 1) / SEG_TILE; }
@@ -433,6 +437,8 @@ SegWs carve_seg(Carver& c, int vb, uint64_t n) {
   w.seg_start = c.take<uint32_t>(2049);
   w.dgid = c.take<uint32_t>(2048);
   w.part = c.take<char>((uint64_t)SORT_MAXCHUNKS * 2048 * st);
+  w.op_mm = c.take<uint64_t>((uint64_t)SORT_MAXCHUNKS * 2 * 2);
+  w.op_ctl = c.take<uint32_t>(4);
   return w;
 }
 
@@ -683,6 +689,81 @@ int tile_reduce_key(const SortCtx& c, const SegWs& w, int vt, const void* keys, 
   return set_error(JFX_ERR_INVALID, "bad value type");
 }
 
+// ---- one-pass path ------------------------------------------------------------
+// Integer keys, float32 or integer values, no min / max, at least 2^16 rows
+// (JFX_ONEPASS=0 turns it off). res receives the combine's result words.
+
+std::atomic<uint32_t> g_op_epoch{0};
+constexpr uint64_t OP_MIN_ROWS = 1ull << 16;
+
+bool onepass_wanted(int kt, int vt, uint64_t n, const void* out_min, const void* out_max) {
+  if (!int_key(kt) || vt == JFX_F64 || out_min || out_max || n < OP_MIN_ROWS) return false;
+  const char* e = getenv("JFX_ONEPASS");
+  return !(e && strcmp(e, "0") == 0);
+}
+
+template <typename K, typename V, typename MT>
+int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const SegWs& w,
+                  void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t* res,
+                  hipStream_t s) {
+  constexpr uint64_t STEP = 64u * WT_RPL;
+  constexpr uint64_t ESZ = sizeof(typename WaAcc<V>::T) + sizeof(uint32_t);
+  OnePassArgs<K, V> a{};
+  a.keys = static_cast<const K*>(keys);
+  a.vals = static_cast<const V*>(vals);
+  a.n = n;
+  a.oc = oc;
+  a.epoch = ++g_op_epoch;
+  a.part = static_cast<AggState<V, false>*>(w.part);
+  a.mm = w.op_mm;
+  a.ctl = w.op_ctl;
+  // as many blocks as fit in LDS at once, within the partial buffer
+  const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / ((uint64_t)OP_W * OP_TE * ESZ));
+  const uint64_t cap = std::min<uint64_t>(256 * per_cu, (uint64_t)SORT_MAXCHUNKS * 2048 / OP_TE);
+  const uint64_t want = (n + (uint64_t)OP_W * STEP - 1) / ((uint64_t)OP_W * STEP);
+  uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, want));
+  a.rows_per_wave = ((n + (uint64_t)grid * OP_W - 1) / ((uint64_t)grid * OP_W) + STEP - 1) / STEP * STEP;
+  grid = (uint32_t)((n + a.rows_per_wave * OP_W - 1) / (a.rows_per_wave * OP_W));
+  const bool vec = (uintptr_t)keys % 16 == 0 && (uintptr_t)vals % 16 == 0;
+  if (vec) {
+    JFX_LAUNCH("onepass_table", s, (k_onepass_table<K, V, true><<<grid, OP_W * 64, 0, s>>>(a)));
+  } else {
+    JFX_LAUNCH("onepass_table", s, (k_onepass_table<K, V, false><<<grid, OP_W * 64, 0, s>>>(a)));
+  }
+  JFX_HIP(hipGetLastError());
+  GroupOut<V, MT> o{static_cast<V*>(sc.out_sum), sc.out_count, nullptr, nullptr,
+                    static_cast<MT*>(sc.out_mean)};
+  // level-1 merges go to the part buffer after the block tables (it holds
+  // SORT_MAXCHUNKS x 2048 min / max states, more than both need)
+  const uint32_t ng1 = (grid + OP_GROUP - 1) / OP_GROUP;
+  const uint64_t part_bytes = (uint64_t)SORT_MAXCHUNKS * 2048 *
+                              (sizeof(V) == 4 ? sizeof(AggState<float>) : sizeof(AggState<double>));
+  JFX_CHECK(((uint64_t)grid + ng1) * OP_TE * sizeof(AggState<V, false>) <= part_bytes,
+            JFX_ERR_WORKSPACE, "one-pass partials exceed the partial buffer");
+  AggState<V, false>* l1 = a.part + (size_t)grid * OP_TE;
+  JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V><<<ng1, 1024, 0, s>>>(a.part, grid, a.ctl,
+                                                                          a.epoch, l1)));
+  JFX_HIP(hipGetLastError());
+  JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT><<<1, 1024, 0, s>>>(
+                                      l1, ng1, a.mm, grid, a.ctl, a.epoch, oc,
+                                      static_cast<K*>(out_keys), o, capacity, w.op_ctl + 2)));
+  JFX_HIP(hipGetLastError());
+  return read_back(s, w.op_ctl + 2, 2 * sizeof(uint32_t), res);
+}
+
+template <typename K>
+int onepass_key(int oc, int vt, const void* keys, const void* vals, uint64_t n, const SegWs& w,
+                void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t* res, hipStream_t s) {
+  switch (vt) {
+    case JFX_F32: return onepass_typed<K, float, float>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
+    case JFX_I32: return onepass_typed<K, int32_t, float>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
+    case JFX_I64: return onepass_typed<K, int64_t, double>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
+    case JFX_U32: return onepass_typed<K, uint32_t, float>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
+    case JFX_U64: return onepass_typed<K, uint64_t, double>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
+  }
+  return set_error(JFX_ERR_INVALID, "bad value type for the one-pass groupby");
+}
+
 }  // namespace
 
 }  // namespace jfx
@@ -770,6 +851,27 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
   void* sort_ws = cv.take<char>(sws);
   JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "groupby workspace too small");
   hipStream_t s = jfx::as_stream(stream);
+  if (jfx::onepass_wanted(key_type, val_type, n, out_min, out_max)) {
+    // one read of keys and values when the key range turns out below 1024;
+    // on a miss the statistics-first path below runs
+    jfx::SegCall oc{nullptr, jfx::order_code(key_type), vals, nullptr, n, out_keys, out_sum,
+                    out_count, nullptr, nullptr, out_mean, true, 0u};
+    uint32_t res[2] = {0u, 0u};
+    int rc = kb == 4 ? jfx::onepass_key<uint32_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc,
+                                                  out_capacity, res, s)
+                     : jfx::onepass_key<uint64_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc,
+                                                  out_capacity, res, s);
+    if (rc != JFX_OK) return rc;
+    if (res[0] == jfx::OP_HIT) {
+      *out_ngroups = res[1];
+      return JFX_OK;
+    }
+    if (res[0] == jfx::OP_CAPACITY) {
+      *out_ngroups = res[1];
+      return jfx::set_error(JFX_ERR_CAPACITY, "outputs hold fewer entries than the groups this "
+                                              "call produces; *out_ngroups holds the capacity needed");
+    }
+  }
   jfx::SortCtx ctx;
   // float keys never take the small-range path: their statistics pass also
   // makes the layout histograms of the multi-pass sort

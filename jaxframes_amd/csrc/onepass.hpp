@@ -1,0 +1,356 @@
+// One-pass groupby for integer keys whose range turns out to be below 1024:
+// sums, counts and means (no min / max) of float32 or integer values.
+//
+// Distinct integers inside any window of 1024 consecutive values have
+// distinct residues mod 1024, so wave tables indexed by key & 1023 need no
+// minimum key in advance. k_onepass_table reads keys and values once: each
+// lane folds runs of equal residues in registers (as the wave-table kernel
+// does) and adds every finished run into its wave's table with two LDS
+// atomics (ds_add_f64 / integer add for the sum, ds_add_u32 for the row
+// count), and it tracks the ordered min / max of every key it reads.
+// k_onepass_merge and k_onepass_final merge the block tables per residue in
+// block order, take the global min / max, and when max - min < 1024 write the
+// groups in key order (digit d = ordered(key) - min, residue = (min + d) &
+// 1023). Otherwise it reports a miss and the caller runs the statistics-first
+// path. The result of a hit is a fixed function of the row layout.
+//
+// Early exit: a wave first reads 64 keys spread over the whole input, and
+// checks its own keys again after its first step of rows; when either spans
+// 1024 or more values the final check cannot pass, so the wave records that
+// span in its block's min / max, raises the call's stop word (epoch-tagged:
+// no clearing needed) and skips its rows; other waves poll the stop word once
+// per step.
+#pragma once
+
+#include "blockscan.hpp"
+#include "common.hpp"
+#include "segreduce.hpp"
+#include "tilereduce.hpp"
+#include "wavetable.hpp"
+
+namespace jfx {
+
+constexpr uint32_t OP_TE = 1024;  // table entries (residues) per wave
+constexpr int OP_W = 4;           // waves per block
+constexpr uint32_t OP_NO_RUN = 0xFFFFFFFFu;
+
+template <typename K, typename V>
+struct OnePassArgs {
+  const K* keys;             // raw key bits (integer keys)
+  const V* vals;
+  uint64_t n;
+  uint64_t rows_per_wave;    // multiple of the wave step
+  int oc;                    // key order code
+  uint32_t epoch;            // this call's tag for the stop word
+  AggState<V, false>* part;  // [gridDim.x][OP_TE] block tables (sum, count in n)
+  uint64_t* mm;              // [gridDim.x][2] ordered min and max of the block's keys
+  uint32_t* ctl;             // [0] stop word
+};
+
+template <typename K>
+__device__ __forceinline__ K wave_min_k(K v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const K o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+template <typename K>
+__device__ __forceinline__ K wave_max_k(K v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const K o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+template <typename K, typename V, bool VEC>
+__global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a) {
+  using T = typename WaAcc<V>::T;
+  using St = AggState<V, false>;
+  constexpr uint64_t STEP = 64u * WT_RPL;
+  constexpr uint32_t M = OP_TE - 1u;
+  __shared__ T s_acc[OP_W * OP_TE];
+  __shared__ uint32_t s_cnt[OP_W * OP_TE];
+  __shared__ K s_mn[OP_W], s_mx[OP_W];
+  __shared__ uint32_t s_stop;
+  if (threadIdx.x == 0) s_stop = 0u;
+  __syncthreads();
+
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t lane = lane_id();
+  T* accs = s_acc + wid * OP_TE;
+  uint32_t* cnts = s_cnt + wid * OP_TE;
+  for (uint32_t i = lane; i < OP_TE; i += 64) {
+    accs[i] = T(0);
+    cnts[i] = 0u;
+  }
+  lds_compiler_fence();
+
+  K omin = ~K(0), omax = K(0);
+  auto see = [&](K key) {
+    const K o = ordered(key, a.oc);
+    omin = o < omin ? o : omin;
+    omax = o > omax ? o : omax;
+  };
+  // the wave's keys so far span OP_TE or more values: the final check fails
+  auto too_wide = [&]() -> bool {
+    const K lo = wave_min_k(omin), hi = wave_max_k(omax);
+    return hi >= lo && hi - lo >= (K)OP_TE;
+  };
+  auto raise_stop = [&]() {
+    if (lane == 0) __hip_atomic_store(&a.ctl[0], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  auto stopped = [&]() -> bool {
+    return __hip_atomic_load(&a.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
+  };
+
+  // 64 keys spread over the whole input (index lane * n / 64 < n)
+  see(a.keys[(uint64_t)lane * a.n / 64u]);
+  bool stop = too_wide();
+  if (stop) raise_stop();
+
+  const uint64_t wave = (uint64_t)blockIdx.x * OP_W + wid;
+  const uint64_t wbeg = wave * a.rows_per_wave;
+  const uint64_t wend = (wbeg + a.rows_per_wave) < a.n ? (wbeg + a.rows_per_wave) : a.n;
+
+  uint32_t run_r = OP_NO_RUN;
+  T run = T(0);
+  uint32_t run_n = 0u;
+  auto flush = [&](uint32_t r, T v, uint32_t c, bool pend) {
+    if (pend) {
+      __hip_atomic_fetch_add(&accs[r], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      __hip_atomic_fetch_add(&cnts[r], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+  };
+  auto row = [&](K key, V x) {
+    see(key);
+    const uint32_t r = (uint32_t)key & M;
+    flush(run_r, run, run_n, r != run_r && run_r != OP_NO_RUN);
+    const bool same = r == run_r;
+    run = same ? run + (T)x : (T)x;
+    run_n = same ? run_n + 1u : 1u;
+    run_r = r;
+  };
+
+  uint64_t base = wbeg;
+  if (!stop && VEC && wbeg + STEP <= wend) {
+    // the loads are unconditional (clamped to the last whole step) so that
+    // the in-order wait counters let the next step stay in flight
+    const uint64_t last = wbeg + ((wend - wbeg) / STEP - 1) * STEP;
+    K ka[WT_RPL], kb[WT_RPL];
+    V va[WT_RPL], vb[WT_RPL];
+    auto load = [&](uint64_t b, K (&kk)[WT_RPL], V (&vv)[WT_RPL]) {
+      const uint64_t bc = (b < last ? b : last) + (uint64_t)lane * WT_RPL;
+      wt_load_rows(a.keys + bc, kk);
+      wt_load_rows(a.vals + bc, vv);
+    };
+    auto fold = [&](K (&kk)[WT_RPL], V (&vv)[WT_RPL]) {
+#pragma unroll
+      for (int i = 0; i < WT_RPL; ++i) row(kk[i], vv[i]);
+    };
+    bool first = true;
+    // after a step: the first step's own span, then the stop word
+    auto check = [&]() -> bool {
+      if (first) {
+        first = false;
+        if (too_wide()) {
+          raise_stop();
+          return true;
+        }
+      }
+      return stopped();
+    };
+    load(base, ka, va);
+    __builtin_amdgcn_sched_barrier(0);
+    load(base + STEP, kb, vb);
+    __builtin_amdgcn_sched_barrier(0);
+    while (true) {
+      fold(ka, va);
+      base += STEP;
+      if (base > last || (stop = check())) break;
+      load(base + STEP, ka, va);
+      __builtin_amdgcn_sched_barrier(0);
+      fold(kb, vb);
+      base += STEP;
+      if (base > last || (stop = check())) break;
+      load(base + STEP, kb, vb);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  if (!stop) {
+    // the lane's last run, then rows [base, wend) one row per lane
+    flush(run_r, run, run_n, run_r != OP_NO_RUN);
+    for (uint64_t r0 = base; r0 < wend; r0 += 64) {
+      const uint64_t r = r0 + lane;
+      const bool valid = r < wend;
+      K key = K(0);
+      T v = T(0);
+      if (valid) {
+        key = a.keys[r];
+        v = (T)a.vals[r];
+        see(key);
+      }
+      flush((uint32_t)key & M, v, 1u, valid);
+    }
+  }
+
+  const K lo = wave_min_k(omin), hi = wave_max_k(omax);
+  if (lane == 0) {
+    s_mn[wid] = lo;
+    s_mx[wid] = hi;
+    if (stop) s_stop = 1u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    K bl = s_mn[0], bh = s_mx[0];
+    for (int w = 1; w < OP_W; ++w) {
+      bl = s_mn[w] < bl ? s_mn[w] : bl;
+      bh = s_mx[w] > bh ? s_mx[w] : bh;
+    }
+    a.mm[(size_t)blockIdx.x * 2 + 0] = (uint64_t)bl;
+    a.mm[(size_t)blockIdx.x * 2 + 1] = (uint64_t)bh;
+  }
+  if (s_stop) return;  // block-uniform: the result will be a miss
+  for (uint32_t r = threadIdx.x; r < OP_TE; r += OP_W * 64) {
+    T acc = T(0);
+    uint32_t c = 0u;
+    for (int w = 0; w < OP_W; ++w) {
+      acc += s_acc[w * OP_TE + r];
+      c += s_cnt[w * OP_TE + r];
+    }
+    St st = St::identity();
+    if constexpr (std::is_same<V, float>::value) {
+      st.s = (float)acc;
+      st.c = __builtin_isfinite(st.s) ? (float)(acc - (double)st.s) : 0.0f;
+    } else {
+      st.s = (V)acc;
+    }
+    st.n = c;
+    a.part[(size_t)blockIdx.x * OP_TE + r] = st;
+  }
+}
+
+// Result words written by k_onepass_final: [0] status, [1] groups.
+constexpr uint32_t OP_MISS = 0u, OP_HIT = 1u, OP_CAPACITY = 2u;
+
+// Two-level merge, thread = residue (coalesced reads): block b of
+// k_onepass_merge merges block tables [OP_GROUP * b, OP_GROUP * (b + 1)) in
+// block order into l1[b]; k_onepass_final (one block) merges l1 in order,
+// checks the key range and writes the groups in key order. (A last-block
+// ticket in one kernel measured slower: its release fences write back L2.)
+constexpr uint32_t OP_GROUP = 16;
+
+// st := st then p[0], p[stride], ... (count <= OP_GROUP entries, in order);
+// the loads are all issued before the first merge.
+template <typename St>
+__device__ __forceinline__ void merge_group(St& st, const St* p, size_t stride, uint32_t count) {
+  St v[OP_GROUP];
+#pragma unroll
+  for (uint32_t i = 0; i < OP_GROUP; ++i)
+    if (i < count) v[i] = p[i * stride];
+#pragma unroll
+  for (uint32_t i = 0; i < OP_GROUP; ++i)
+    if (i < count) st.merge(v[i]);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(1024)
+k_onepass_merge(const AggState<V, false>* part, uint32_t nblocks, const uint32_t* ctl,
+                uint32_t epoch, AggState<V, false>* l1) {
+  using St = AggState<V, false>;
+  if (ctl[0] == epoch) return;  // stopped: the result is a miss
+  const uint32_t c0 = blockIdx.x * OP_GROUP;
+  const uint32_t c1 = (c0 + OP_GROUP) < nblocks ? (c0 + OP_GROUP) : nblocks;
+  St st = St::identity();
+  merge_group(st, part + (size_t)c0 * OP_TE + threadIdx.x, OP_TE, c1 - c0);
+  l1[(size_t)blockIdx.x * OP_TE + threadIdx.x] = st;
+}
+
+template <typename K, typename V, typename MT>
+__global__ void __launch_bounds__(1024)
+k_onepass_final(const AggState<V, false>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
+                const uint32_t* ctl, uint32_t epoch, int oc, K* out_keys, GroupOut<V, MT> out,
+                uint64_t capacity, uint32_t* res) {
+  using St = AggState<V, false>;
+  __shared__ K s_lo[16], s_hi[16];
+  __shared__ St s_st[OP_TE];
+  __shared__ uint32_t s_gid[OP_TE];
+  __shared__ uint32_t s_tmp[1024 / 64 + 1];
+  const int tid = threadIdx.x;
+  const int wid = tid >> 6;
+  const uint32_t lane = lane_id();
+  const bool stopped = ctl[0] == epoch;
+  if (!stopped) {
+    St st = St::identity();
+    for (uint32_t b = 0; b < nl1; b += OP_GROUP)
+      merge_group(st, l1 + (size_t)b * OP_TE + tid, OP_TE, (nl1 - b) < OP_GROUP ? (nl1 - b) : OP_GROUP);
+    s_st[tid] = st;
+  }
+
+  K lo = ~K(0), hi = K(0);
+  for (uint32_t b = tid; b < nblocks; b += 1024) {
+    const K l = (K)mm[(size_t)b * 2], h = (K)mm[(size_t)b * 2 + 1];
+    lo = l < lo ? l : lo;
+    hi = h > hi ? h : hi;
+  }
+  lo = wave_min_k(lo);
+  hi = wave_max_k(hi);
+  if (lane == 0) {
+    s_lo[wid] = lo;
+    s_hi[wid] = hi;
+  }
+  __syncthreads();
+  lo = s_lo[0];
+  hi = s_hi[0];
+  for (int w = 1; w < 16; ++w) {
+    lo = s_lo[w] < lo ? s_lo[w] : lo;
+    hi = s_hi[w] > hi ? s_hi[w] : hi;
+  }
+  const bool hit = !stopped && hi >= lo && hi - lo < (K)OP_TE;
+  if (!hit) {
+    if (tid == 0) {
+      res[0] = OP_MISS;
+      res[1] = 0u;
+    }
+    return;
+  }
+  // digit d = ordered key - lo, in key order; its residue holds its group
+  const uint32_t span = (uint32_t)(hi - lo);
+  const uint32_t d = (uint32_t)tid;
+  const K o = lo + (K)d;
+  const uint32_t r = (uint32_t)o & (OP_TE - 1u);
+  St st = St::identity();
+  if (d <= span) st = s_st[r];
+  const bool present = d <= span && st.n != 0u;
+  s_gid[d] = present ? 1u : 0u;
+  __syncthreads();
+  const uint32_t G = block_excl_scan_lds<OP_TE, 1024>(s_gid, s_tmp);
+  if ((uint64_t)G > capacity) {
+    if (tid == 0) {
+      res[0] = OP_CAPACITY;
+      res[1] = G;
+    }
+    return;
+  }
+  if (present) {
+    const uint32_t g = s_gid[d];
+    K raw = o;
+    if (oc == ORD_SIGNED) raw ^= (K)1 << (sizeof(K) * 8 - 1);
+    if (out_keys) out_keys[g] = raw;
+    if (out.sum) out.sum[g] = st.sum();
+    if (out.count) out.count[g] = (int64_t)st.n;
+    if (out.mean) {
+      if constexpr (ValTraits<V>::is_float) out.mean[g] = (MT)st.sum() / (MT)st.n;
+      else out.mean[g] = (MT)((double)st.s / (double)st.n);
+    }
+  }
+  if (tid == 0) {
+    res[0] = OP_HIT;
+    res[1] = G;
+  }
+}
+
+}  // namespace jfx

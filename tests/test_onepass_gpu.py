@@ -1,0 +1,184 @@
+"""GPU parity of the one-pass groupby (integer keys whose range is below 1024;
+sum / count / mean of float32 or integer values) against the oracle, and of
+its misses (wider ranges), which must give the statistics-first path's
+result. Keys and counts are bit-exact, integer sums exact (wrapping in the
+value dtype), float32 sums within the FP64-oracle bound of the groupby tests.
+"""
+
+import ctypes
+import os
+import zlib
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from jaxframes_amd import _lib, grouping
+from jaxframes_amd._tensors import to_numpy
+from tests._util import cuda
+from tests.test_groupby_gpu import _check_float_sums, _wrapping_sums
+
+pytestmark = pytest.mark.gpu
+
+N = 1 << 18   # above the one-pass minimum (2^16 rows)
+
+
+def _groups(keys, vals):
+    sk, sv, _ = oracle.radix_sort(keys, vals)
+    starts = np.flatnonzero(oracle.head_flags(sk))
+    return sk, sv, starts
+
+
+def _check(keys, vals, uk, res_sum, res_count=None):
+    sk, sv, starts = _groups(keys, vals)
+    assert np.array_equal(to_numpy(uk), sk[starts])
+    if res_count is not None:
+        assert np.array_equal(to_numpy(res_count), np.diff(np.append(starts, len(sk))))
+    got = to_numpy(res_sum)
+    if np.dtype(vals.dtype).kind == "f":
+        _check_float_sums(got, sv, starts, vals.dtype)
+    else:
+        assert np.array_equal(got, _wrapping_sums(sv, starts))
+
+
+def _kernels_run(fn):
+    """fn's result and the names of the library kernels it launched."""
+    with _lib.KernelTimer() as kt:
+        out = fn()
+        torch.cuda.synchronize()
+    return out, set(kt.stats().keys())
+
+
+KEY_CASES = [
+    ("int32", np.int32, 0),
+    ("int32-neg", np.int32, -500),
+    ("int32-top", np.int32, 2**31 - 1000),
+    ("int32-bottom", np.int32, -2**31),
+    ("int64", np.int64, 10**12),
+    ("uint32-top", np.uint32, 2**32 - 1000),
+    ("uint64-top", np.uint64, 2**64 - 1000),
+]
+
+
+@pytest.mark.parametrize("name,kdt,base", KEY_CASES, ids=[c[0] for c in KEY_CASES])
+@pytest.mark.parametrize("vdt", [np.float32, np.int32, np.int64, np.uint64])
+def test_onepass_hit(name, kdt, base, vdt):
+    rng = np.random.default_rng(zlib.crc32(f"{name}/{np.dtype(vdt).name}".encode()))
+    off = rng.integers(0, 1000, N)   # 1000 distinct keys from base on
+    if np.dtype(kdt).itemsize == 4:
+        keys = (np.int64(base) + off).astype(kdt)
+    else:
+        keys = (np.uint64(base % 2**64) + off.astype(np.uint64)).view(kdt)
+    if np.dtype(vdt).kind == "f":
+        vals = rng.standard_normal(N).astype(vdt)
+    else:
+        info = np.iinfo(vdt)
+        vals = rng.integers(info.min, info.max, N, dtype=vdt, endpoint=True)
+    (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)},
+                                                                       {"v": "sum"}))
+    assert "onepass_table" in ran and "sort_stats" not in ran, ran
+    _check(keys, vals, uk, res["v"])
+
+
+def test_onepass_count_and_mean():
+    rng = np.random.default_rng(1)
+    keys = rng.integers(-300, 300, N).astype(np.int32)
+    vals = rng.standard_normal(N).astype(np.float32)
+    for fn in ("count", "mean"):
+        uk, res = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": fn})
+        sk, sv, starts = _groups(keys, vals)
+        assert np.array_equal(to_numpy(uk), sk[starts])
+        counts = np.diff(np.append(starts, N))
+        if fn == "count":
+            assert np.array_equal(to_numpy(res["v"]), counts)
+        else:
+            np.testing.assert_allclose(to_numpy(res["v"]),
+                                       oracle.segment_sums_f64(sv, starts) / counts,
+                                       rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("case", ["span1024", "outlier", "sorted_wide", "two_clusters"])
+def test_onepass_miss_falls_back(case):
+    rng = np.random.default_rng(7)
+    if case == "span1024":        # keys 0..1024: 1025 values, one too many
+        keys = rng.integers(0, 1025, N).astype(np.int32)
+        keys[:2] = [0, 1024]
+    elif case == "outlier":       # one key far away, late in the input
+        keys = rng.integers(0, 1000, N).astype(np.int32)
+        keys[N - 12345] = 10**6
+    elif case == "sorted_wide":   # every wave sees a narrow slice
+        keys = np.sort(rng.integers(0, 10**6, N)).astype(np.int64)
+    else:                         # residues of the two clusters collide
+        keys = np.where(rng.random(N) < 0.5, rng.integers(0, 100, N),
+                        rng.integers(4096, 4196, N)).astype(np.int32)
+    vals = rng.standard_normal(N).astype(np.float32)
+    (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)},
+                                                                       {"v": "sum"}))
+    assert "onepass_table" in ran and "sort_stats" in ran, ran
+    _check(keys, vals, uk, res["v"])
+
+
+def test_onepass_hit_after_miss_and_unaligned():
+    """A miss leaves its stop word in the workspace; the next call (same
+    workspace size, so the same cached buffer) must still hit. The unaligned
+    view takes the row-by-row loads."""
+    rng = np.random.default_rng(9)
+    wide = rng.integers(0, 10**6, N).astype(np.int32)
+    narrow = rng.integers(50, 900, N).astype(np.int32)
+    vals = rng.standard_normal(N).astype(np.float32)
+    for _ in range(2):
+        for keys in (wide, narrow):
+            uk, res = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "sum"})
+            _check(keys, vals, uk, res["v"])
+    (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(
+        cuda(narrow)[1:], {"v": cuda(vals)[1:]}, {"v": "sum"}))
+    assert "onepass_table" in ran and "sort_stats" not in ran
+    _check(narrow[1:], vals[1:], uk, res["v"])
+
+
+def test_onepass_matches_statistics_first_path():
+    rng = np.random.default_rng(11)
+    keys = rng.integers(0, 1000, 1_000_003).astype(np.int32)
+    vals = rng.standard_normal(keys.size).astype(np.float32)
+    uk1, r1 = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "sum"})
+    os.environ["JFX_ONEPASS"] = "0"
+    try:
+        (uk2, r2), ran = _kernels_run(lambda: grouping.groupby_aggregate(
+            cuda(keys), {"v": cuda(vals)}, {"v": "sum"}))
+    finally:
+        del os.environ["JFX_ONEPASS"]
+    assert "onepass_table" not in ran
+    assert torch.equal(uk1, uk2)
+    a, b = to_numpy(r1["v"]).astype(np.float64), to_numpy(r2["v"]).astype(np.float64)
+    sk, sv, starts = _groups(keys, vals)
+    nrm = np.add.reduceat(np.abs(sv.astype(np.float64)), starts)
+    assert (np.abs(a - b) / nrm).max() <= 4e-7
+
+
+def test_onepass_capacity_report():
+    """Too small an output capacity: the exact group count comes back and
+    nothing is written."""
+    rng = np.random.default_rng(13)
+    keys = torch.from_numpy(rng.integers(0, 700, N).astype(np.int32)).cuda()
+    vals = torch.from_numpy(rng.standard_normal(N).astype(np.float32)).cuda()
+    G = int(torch.unique(keys).numel())
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_groupby_workspace_size", _lib.JFX_I32, _lib.JFX_F32, N, ctypes.byref(nb))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    out_k = torch.full((16,), 7, dtype=torch.int32, device="cuda")
+    out_s = torch.full((16,), 7.0, dtype=torch.float32, device="cuda")
+    g = ctypes.c_uint64(0)
+    st = _lib.lib().jfx_groupby_ex(_lib.JFX_I32, _lib.ptr(keys), _lib.JFX_F32, _lib.ptr(vals), N,
+                                   _lib.ptr(out_k), _lib.ptr(out_s), None, None, None, None,
+                                   ctypes.byref(g), 16, 0, _lib.ptr(ws), ws.numel(), 0)
+    assert st == _lib.JFX_ERR_CAPACITY and g.value == G
+    torch.cuda.synchronize()
+    assert torch.all(out_k == 7) and torch.all(out_s == 7.0)
+    out_k = torch.empty(G, dtype=torch.int32, device="cuda")
+    out_s = torch.empty(G, dtype=torch.float32, device="cuda")
+    st = _lib.lib().jfx_groupby_ex(_lib.JFX_I32, _lib.ptr(keys), _lib.JFX_F32, _lib.ptr(vals), N,
+                                   _lib.ptr(out_k), _lib.ptr(out_s), None, None, None, None,
+                                   ctypes.byref(g), G, 0, _lib.ptr(ws), ws.numel(), 0)
+    assert st == 0 and g.value == G
+    _check(keys.cpu().numpy(), vals.cpu().numpy(), out_k, out_s)
