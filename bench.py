@@ -116,13 +116,18 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
     # the plain path is timed without the plan's buffers held by the allocator
     del plan
     torch.cuda.empty_cache()
+    # warmed over the whole sample (the allocator grows its cached blocks to
+    # every frame's workspace size once), then the median of 3 passes
     rows_plain = int(offs[plain_frames])
-    batched.plain_per_frame(keys, vals, offs, "sum", frames=range(4))
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
     batched.plain_per_frame(keys, vals, offs, "sum", frames=range(plain_frames))
     torch.cuda.synchronize()
-    plain_rps = rows_plain / (time.perf_counter() - t1)
+    plain_t = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        batched.plain_per_frame(keys, vals, offs, "sum", frames=range(plain_frames))
+        torch.cuda.synchronize()
+        plain_t.append(time.perf_counter() - t1)
+    plain_rps = rows_plain / float(np.median(plain_t))
     out = {"workload": f"4096 frames, {n} rows, int32 keys (per-frame cardinality log-uniform "
                        f"[10, 1M]), float32 values; sum + count per frame",
            "rows": n, "frames": int(offs.shape[0] - 1), "groups": groups,

@@ -118,18 +118,24 @@ int jfx_groupby_workspace_size(int key_type, int val_type, uint64_t n, uint64_t*
 /* One-column groupby: radix sort of the keys carrying the values as payload,
  * then jfx_segreduce. Replaces groupby_aggregate (parallel_algorithms.py:
  * 1117-1148) for a single aggregated column. Synchronises once for the digit
- * plan and once to read G into *out_ngroups. */
+ * plan and once to read G into *out_ngroups. Integer keys with at least 2^22
+ * rows and no min / max output first try one pass (wave tables indexed by
+ * key mod 1024; float32 or integer values): when max - min < 1024 that pass
+ * gives the groups with one synchronisation, otherwise the sort-based or
+ * small-range path runs as above (JFX_ONEPASS=0 skips the one-pass try,
+ * JFX_ONEPASS=always makes it at any size). */
 int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
                 void* out_keys, void* out_sum, int64_t* out_count, void* out_min, void* out_max,
                 void* out_mean, uint64_t* out_ngroups, int digit_bits, void* workspace,
                 uint64_t ws_bytes, void* stream);
 
 /* jfx_groupby with outputs of out_capacity entries (jfx_groupby: n). When
- * the path chosen after the key statistics may produce more groups than
- * that -- more than min(n, max - min + 1) for a small integer key range, else
- * more than n -- nothing is written, *out_ngroups is set to the capacity needed
- * and JFX_ERR_CAPACITY is returned; a caller that expects few groups passes
- * small outputs and retries with the capacity returned. */
+ * the groups may outnumber that -- on the one-pass path, more than the exact
+ * group count; after the key statistics, more than min(n, max - min + 1) for
+ * a small integer key range, else more than n -- nothing is written,
+ * *out_ngroups is set to the capacity needed and JFX_ERR_CAPACITY is
+ * returned; a caller that expects few groups passes small outputs and
+ * retries with the capacity returned. */
 int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
                    void* out_keys, void* out_sum, int64_t* out_count, void* out_min,
                    void* out_max, void* out_mean, uint64_t* out_ngroups, uint64_t out_capacity,

@@ -690,16 +690,21 @@ int tile_reduce_key(const SortCtx& c, const SegWs& w, int vt, const void* keys, 
 }
 
 // ---- one-pass path ------------------------------------------------------------
-// Integer keys, float32 or integer values, no min / max, at least 2^16 rows
-// (JFX_ONEPASS=0 turns it off). res receives the combine's result words.
+// Integer keys, float32 or integer values, no min / max, at least 2^22 rows:
+// a miss costs about 30 us (the aborted table launch, two small kernels and a
+// read-back) before the statistics-first path, under 1% of a groupby of that
+// size. JFX_ONEPASS=0 turns the path off, =always tries it at any size.
+// res receives the final kernel's result words.
 
 std::atomic<uint32_t> g_op_epoch{0};
-constexpr uint64_t OP_MIN_ROWS = 1ull << 16;
+constexpr uint64_t OP_MIN_ROWS = 1ull << 22;
 
 bool onepass_wanted(int kt, int vt, uint64_t n, const void* out_min, const void* out_max) {
-  if (!int_key(kt) || vt == JFX_F64 || out_min || out_max || n < OP_MIN_ROWS) return false;
+  if (!int_key(kt) || vt == JFX_F64 || out_min || out_max) return false;
   const char* e = getenv("JFX_ONEPASS");
-  return !(e && strcmp(e, "0") == 0);
+  if (e && strcmp(e, "0") == 0) return false;
+  if (e && strcmp(e, "always") == 0) return true;
+  return n >= OP_MIN_ROWS;
 }
 
 template <typename K, typename V, typename MT>

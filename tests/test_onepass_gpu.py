@@ -6,7 +6,6 @@ value dtype), float32 sums within the FP64-oracle bound of the groupby tests.
 """
 
 import ctypes
-import os
 import zlib
 
 import numpy as np
@@ -21,7 +20,14 @@ from tests.test_groupby_gpu import _check_float_sums, _wrapping_sums
 
 pytestmark = pytest.mark.gpu
 
-N = 1 << 18   # above the one-pass minimum (2^16 rows)
+N = 1 << 18
+
+
+@pytest.fixture(autouse=True)
+def _onepass_at_any_size(monkeypatch):
+    # by default the one-pass try starts at 2^22 rows; these tests take it at
+    # any size (test_onepass_default_threshold checks the default)
+    monkeypatch.setenv("JFX_ONEPASS", "always")
 
 
 def _groups(keys, vals):
@@ -137,17 +143,14 @@ def test_onepass_hit_after_miss_and_unaligned():
     _check(narrow[1:], vals[1:], uk, res["v"])
 
 
-def test_onepass_matches_statistics_first_path():
+def test_onepass_matches_statistics_first_path(monkeypatch):
     rng = np.random.default_rng(11)
     keys = rng.integers(0, 1000, 1_000_003).astype(np.int32)
     vals = rng.standard_normal(keys.size).astype(np.float32)
     uk1, r1 = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "sum"})
-    os.environ["JFX_ONEPASS"] = "0"
-    try:
-        (uk2, r2), ran = _kernels_run(lambda: grouping.groupby_aggregate(
-            cuda(keys), {"v": cuda(vals)}, {"v": "sum"}))
-    finally:
-        del os.environ["JFX_ONEPASS"]
+    monkeypatch.setenv("JFX_ONEPASS", "0")
+    (uk2, r2), ran = _kernels_run(lambda: grouping.groupby_aggregate(
+        cuda(keys), {"v": cuda(vals)}, {"v": "sum"}))
     assert "onepass_table" not in ran
     assert torch.equal(uk1, uk2)
     a, b = to_numpy(r1["v"]).astype(np.float64), to_numpy(r2["v"]).astype(np.float64)
@@ -182,3 +185,27 @@ def test_onepass_capacity_report():
                                    ctypes.byref(g), G, 0, _lib.ptr(ws), ws.numel(), 0)
     assert st == 0 and g.value == G
     _check(keys.cpu().numpy(), vals.cpu().numpy(), out_k, out_s)
+
+
+@pytest.mark.parametrize("n", [1, 5, 1000, 65537])
+def test_onepass_small_inputs(n):
+    """Inputs smaller than one wave step go row by row (no vector loop)."""
+    rng = np.random.default_rng(n)
+    keys = rng.integers(-7, 900, n).astype(np.int64)
+    vals = rng.standard_normal(n).astype(np.float32)
+    (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)},
+                                                                       {"v": "sum"}))
+    assert "onepass_table" in ran and "sort_stats" not in ran, ran
+    _check(keys, vals, uk, res["v"])
+
+
+@pytest.mark.parametrize("n,onepass", [((1 << 22) - 1, False), ((1 << 22) + 5, True)])
+def test_onepass_default_threshold(monkeypatch, n, onepass):
+    monkeypatch.delenv("JFX_ONEPASS")
+    rng = np.random.default_rng(17)
+    keys = rng.integers(0, 1000, n).astype(np.int32)
+    vals = rng.standard_normal(n).astype(np.float32)
+    (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)},
+                                                                       {"v": "sum"}))
+    assert ("onepass_table" in ran) == onepass and ("sort_stats" in ran) != onepass, ran
+    _check(keys, vals, uk, res["v"])
