@@ -712,19 +712,21 @@ int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const 
                   void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t* res,
                   hipStream_t s) {
   constexpr uint64_t STEP = 64u * WT_RPL;
-  constexpr uint64_t ESZ = sizeof(typename WaAcc<V>::T) + sizeof(uint32_t);
+  // LDS per block: a sum table per wave and one count table
+  constexpr uint64_t BLK = (uint64_t)OP_W * OP_TE * sizeof(typename WaAcc<V>::T) + OP_TE * 4u;
   OnePassArgs<K, V> a{};
   a.keys = static_cast<const K*>(keys);
   a.vals = static_cast<const V*>(vals);
   a.n = n;
   a.oc = oc;
   a.epoch = ++g_op_epoch;
-  a.part = static_cast<AggState<V, false>*>(w.part);
+  a.part = static_cast<OpPart<V>*>(w.part);
   a.mm = w.op_mm;
   a.ctl = w.op_ctl;
-  // as many blocks as fit in LDS at once, within the partial buffer
-  const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / ((uint64_t)OP_W * OP_TE * ESZ));
-  const uint64_t cap = std::min<uint64_t>(256 * per_cu, (uint64_t)SORT_MAXCHUNKS * 2048 / OP_TE);
+  // as many blocks as fit in LDS at once, at most OP_GROUP x OP_L1 (the
+  // final merge's reach)
+  const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / BLK);
+  const uint64_t cap = std::min<uint64_t>(256 * per_cu, (uint64_t)OP_GROUP * OP_L1);
   const uint64_t want = (n + (uint64_t)OP_W * STEP - 1) / ((uint64_t)OP_W * STEP);
   uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, want));
   a.rows_per_wave = ((n + (uint64_t)grid * OP_W - 1) / ((uint64_t)grid * OP_W) + STEP - 1) / STEP * STEP;
@@ -741,11 +743,12 @@ int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const 
   // level-1 merges go to the part buffer after the block tables (it holds
   // SORT_MAXCHUNKS x 2048 min / max states, more than both need)
   const uint32_t ng1 = (grid + OP_GROUP - 1) / OP_GROUP;
+  JFX_CHECK(ng1 <= OP_L1, JFX_ERR_UNSUPPORTED, "one-pass grid exceeds the final merge");
   const uint64_t part_bytes = (uint64_t)SORT_MAXCHUNKS * 2048 *
                               (sizeof(V) == 4 ? sizeof(AggState<float>) : sizeof(AggState<double>));
-  JFX_CHECK(((uint64_t)grid + ng1) * OP_TE * sizeof(AggState<V, false>) <= part_bytes,
+  JFX_CHECK(((uint64_t)grid + ng1) * OP_TE * sizeof(OpPart<V>) <= part_bytes,
             JFX_ERR_WORKSPACE, "one-pass partials exceed the partial buffer");
-  AggState<V, false>* l1 = a.part + (size_t)grid * OP_TE;
+  OpPart<V>* l1 = a.part + (size_t)grid * OP_TE;
   JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V><<<ng1, 1024, 0, s>>>(a.part, grid, a.ctl,
                                                                           a.epoch, l1)));
   JFX_HIP(hipGetLastError());

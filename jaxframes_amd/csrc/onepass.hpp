@@ -5,9 +5,12 @@
 // distinct residues mod 1024, so wave tables indexed by key & 1023 need no
 // minimum key in advance. k_onepass_table reads keys and values once: each
 // lane folds runs of equal residues in registers (as the wave-table kernel
-// does) and adds every finished run into its wave's table with two LDS
-// atomics (ds_add_f64 / integer add for the sum, ds_add_u32 for the row
-// count), and it tracks the ordered min / max of every key it reads.
+// does) and adds every finished run with two LDS atomics: the sum into its
+// wave's table (ds_add_f64 / integer add; wave-private so that the float add
+// order is fixed) and the row count into its block's table (ds_add_u32:
+// integer adds commute, so the 4 waves share one count table, 36 KB of LDS
+// per block and 4 blocks per CU). It also tracks the ordered min / max of
+// every key it reads.
 // k_onepass_merge and k_onepass_final merge the block tables per residue in
 // block order, take the global min / max, and when max - min < 1024 write the
 // groups in key order (digit d = ordered(key) - min, residue = (min + d) &
@@ -34,6 +37,36 @@ constexpr uint32_t OP_TE = 1024;  // table entries (residues) per wave
 constexpr int OP_W = 4;           // waves per block
 constexpr uint32_t OP_NO_RUN = 0xFFFFFFFFu;
 
+// One residue of a block table: the sum and the row count. float32 values
+// keep their double sum as a float pair s + c (c the rounding error of s, 0
+// when s is not finite; s + c converts back to the double exactly), 12 bytes;
+// integer values a wrapping unsigned sum.
+template <typename V> struct OpPart {
+  typename ValTraits<V>::U s;
+  uint32_t n;
+};
+template <> struct OpPart<float> {
+  float s, c;
+  uint32_t n;
+};
+template <typename V>
+__device__ __forceinline__ OpPart<V> op_pack(typename WaAcc<V>::T acc, uint32_t n) {
+  OpPart<V> p;
+  if constexpr (std::is_same<V, float>::value) {
+    p.s = (float)acc;
+    p.c = __builtin_isfinite(p.s) ? (float)(acc - (double)p.s) : 0.0f;
+  } else {
+    p.s = acc;
+  }
+  p.n = n;
+  return p;
+}
+template <typename V>
+__device__ __forceinline__ typename WaAcc<V>::T op_sum(const OpPart<V>& p) {
+  if constexpr (std::is_same<V, float>::value) return (double)p.s + (double)p.c;
+  else return p.s;
+}
+
 template <typename K, typename V>
 struct OnePassArgs {
   const K* keys;             // raw key bits (integer keys)
@@ -42,7 +75,7 @@ struct OnePassArgs {
   uint64_t rows_per_wave;    // multiple of the wave step
   int oc;                    // key order code
   uint32_t epoch;            // this call's tag for the stop word
-  AggState<V, false>* part;  // [gridDim.x][OP_TE] block tables (sum, count in n)
+  OpPart<V>* part;           // [gridDim.x][OP_TE] block tables
   uint64_t* mm;              // [gridDim.x][2] ordered min and max of the block's keys
   uint32_t* ctl;             // [0] stop word
 };
@@ -69,25 +102,21 @@ __device__ __forceinline__ K wave_max_k(K v) {
 template <typename K, typename V, bool VEC>
 __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a) {
   using T = typename WaAcc<V>::T;
-  using St = AggState<V, false>;
   constexpr uint64_t STEP = 64u * WT_RPL;
   constexpr uint32_t M = OP_TE - 1u;
   __shared__ T s_acc[OP_W * OP_TE];
-  __shared__ uint32_t s_cnt[OP_W * OP_TE];
+  __shared__ uint32_t s_cnt[OP_TE];
   __shared__ K s_mn[OP_W], s_mx[OP_W];
   __shared__ uint32_t s_stop;
   if (threadIdx.x == 0) s_stop = 0u;
-  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < OP_TE; i += OP_W * 64) s_cnt[i] = 0u;
 
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lane = lane_id();
   T* accs = s_acc + wid * OP_TE;
-  uint32_t* cnts = s_cnt + wid * OP_TE;
-  for (uint32_t i = lane; i < OP_TE; i += 64) {
-    accs[i] = T(0);
-    cnts[i] = 0u;
-  }
-  lds_compiler_fence();
+  uint32_t* cnts = s_cnt;
+  for (uint32_t i = lane; i < OP_TE; i += 64) accs[i] = T(0);
+  __syncthreads();
 
   K omin = ~K(0), omax = K(0);
   auto see = [&](K key) {
@@ -122,7 +151,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   auto flush = [&](uint32_t r, T v, uint32_t c, bool pend) {
     if (pend) {
       __hip_atomic_fetch_add(&accs[r], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-      __hip_atomic_fetch_add(&cnts[r], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      __hip_atomic_fetch_add(&cnts[r], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   };
   auto row = [&](K key, V x) {
@@ -216,20 +245,8 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   if (s_stop) return;  // block-uniform: the result will be a miss
   for (uint32_t r = threadIdx.x; r < OP_TE; r += OP_W * 64) {
     T acc = T(0);
-    uint32_t c = 0u;
-    for (int w = 0; w < OP_W; ++w) {
-      acc += s_acc[w * OP_TE + r];
-      c += s_cnt[w * OP_TE + r];
-    }
-    St st = St::identity();
-    if constexpr (std::is_same<V, float>::value) {
-      st.s = (float)acc;
-      st.c = __builtin_isfinite(st.s) ? (float)(acc - (double)st.s) : 0.0f;
-    } else {
-      st.s = (V)acc;
-    }
-    st.n = c;
-    a.part[(size_t)blockIdx.x * OP_TE + r] = st;
+    for (int w = 0; w < OP_W; ++w) acc += s_acc[w * OP_TE + r];
+    a.part[(size_t)blockIdx.x * OP_TE + r] = op_pack<V>(acc, s_cnt[r]);
   }
 }
 
@@ -238,64 +255,79 @@ constexpr uint32_t OP_MISS = 0u, OP_HIT = 1u, OP_CAPACITY = 2u;
 
 // Two-level merge, thread = residue (coalesced reads): block b of
 // k_onepass_merge merges block tables [OP_GROUP * b, OP_GROUP * (b + 1)) in
-// block order into l1[b]; k_onepass_final (one block) merges l1 in order,
-// checks the key range and writes the groups in key order. (A last-block
-// ticket in one kernel measured slower: its release fences write back L2.)
-constexpr uint32_t OP_GROUP = 16;
+// block order into l1[b]; k_onepass_final (one block) merges the at most
+// OP_L1 level-1 tables in order, checks the key range and writes the groups
+// in key order. Each merge issues all its loads before the first add. (A
+// last-block ticket in one kernel measured slower: its release fences write
+// back L2.)
+constexpr uint32_t OP_GROUP = 32;
+constexpr uint32_t OP_L1 = 32;  // up to 1024 block tables
 
-// st := st then p[0], p[stride], ... (count <= OP_GROUP entries, in order);
-// the loads are all issued before the first merge.
-template <typename St>
-__device__ __forceinline__ void merge_group(St& st, const St* p, size_t stride, uint32_t count) {
-  St v[OP_GROUP];
+// acc, n := acc, n plus p[0], p[stride], ... (count entries, added in order
+// in the accumulator type), 16 loads in flight at a time (more would spill
+// at 1024 threads).
+template <typename V>
+__device__ __forceinline__ void merge_seq(typename WaAcc<V>::T& acc, uint32_t& n,
+                                          const OpPart<V>* p, size_t stride, uint32_t count) {
+  constexpr uint32_t G = 16;
+  for (uint32_t b = 0; b < count; b += G, p += G * stride) {
+    const uint32_t m = (count - b) < G ? (count - b) : G;
+    OpPart<V> v[G];
 #pragma unroll
-  for (uint32_t i = 0; i < OP_GROUP; ++i)
-    if (i < count) v[i] = p[i * stride];
+    for (uint32_t i = 0; i < G; ++i)
+      if (i < m) v[i] = p[i * stride];
 #pragma unroll
-  for (uint32_t i = 0; i < OP_GROUP; ++i)
-    if (i < count) st.merge(v[i]);
+    for (uint32_t i = 0; i < G; ++i)
+      if (i < m) {
+        acc += op_sum<V>(v[i]);
+        n += v[i].n;
+      }
+  }
 }
 
 template <typename V>
 __global__ void __launch_bounds__(1024)
-k_onepass_merge(const AggState<V, false>* part, uint32_t nblocks, const uint32_t* ctl,
-                uint32_t epoch, AggState<V, false>* l1) {
-  using St = AggState<V, false>;
+k_onepass_merge(const OpPart<V>* part, uint32_t nblocks, const uint32_t* ctl, uint32_t epoch,
+                OpPart<V>* l1) {
   if (ctl[0] == epoch) return;  // stopped: the result is a miss
   const uint32_t c0 = blockIdx.x * OP_GROUP;
   const uint32_t c1 = (c0 + OP_GROUP) < nblocks ? (c0 + OP_GROUP) : nblocks;
-  St st = St::identity();
-  merge_group(st, part + (size_t)c0 * OP_TE + threadIdx.x, OP_TE, c1 - c0);
-  l1[(size_t)blockIdx.x * OP_TE + threadIdx.x] = st;
+  typename WaAcc<V>::T acc(0);
+  uint32_t n = 0u;
+  merge_seq<V>(acc, n, part + (size_t)c0 * OP_TE + threadIdx.x, OP_TE, c1 - c0);
+  l1[(size_t)blockIdx.x * OP_TE + threadIdx.x] = op_pack<V>(acc, n);
 }
 
 template <typename K, typename V, typename MT>
 __global__ void __launch_bounds__(1024)
-k_onepass_final(const AggState<V, false>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
+k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
                 const uint32_t* ctl, uint32_t epoch, int oc, K* out_keys, GroupOut<V, MT> out,
                 uint64_t capacity, uint32_t* res) {
-  using St = AggState<V, false>;
+  using T = typename WaAcc<V>::T;
   __shared__ K s_lo[16], s_hi[16];
-  __shared__ St s_st[OP_TE];
+  __shared__ T s_sum[OP_TE];
+  __shared__ uint32_t s_n[OP_TE];
   __shared__ uint32_t s_gid[OP_TE];
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
   const int tid = threadIdx.x;
   const int wid = tid >> 6;
   const uint32_t lane = lane_id();
   const bool stopped = ctl[0] == epoch;
-  if (!stopped) {
-    St st = St::identity();
-    for (uint32_t b = 0; b < nl1; b += OP_GROUP)
-      merge_group(st, l1 + (size_t)b * OP_TE + tid, OP_TE, (nl1 - b) < OP_GROUP ? (nl1 - b) : OP_GROUP);
-    s_st[tid] = st;
-  }
-
+  // block min / max loads first, so that they are in flight with the merge's
   K lo = ~K(0), hi = K(0);
   for (uint32_t b = tid; b < nblocks; b += 1024) {
     const K l = (K)mm[(size_t)b * 2], h = (K)mm[(size_t)b * 2 + 1];
     lo = l < lo ? l : lo;
     hi = h > hi ? h : hi;
   }
+  if (!stopped) {
+    T acc(0);
+    uint32_t n = 0u;
+    merge_seq<V>(acc, n, l1 + tid, OP_TE, nl1);
+    s_sum[tid] = acc;
+    s_n[tid] = n;
+  }
+
   lo = wave_min_k(lo);
   hi = wave_max_k(hi);
   if (lane == 0) {
@@ -322,9 +354,9 @@ k_onepass_final(const AggState<V, false>* l1, uint32_t nl1, const uint64_t* mm, 
   const uint32_t d = (uint32_t)tid;
   const K o = lo + (K)d;
   const uint32_t r = (uint32_t)o & (OP_TE - 1u);
-  St st = St::identity();
-  if (d <= span) st = s_st[r];
-  const bool present = d <= span && st.n != 0u;
+  const T sum = d <= span ? s_sum[r] : T(0);
+  const uint32_t cnt = d <= span ? s_n[r] : 0u;
+  const bool present = cnt != 0u;
   s_gid[d] = present ? 1u : 0u;
   __syncthreads();
   const uint32_t G = block_excl_scan_lds<OP_TE, 1024>(s_gid, s_tmp);
@@ -340,11 +372,11 @@ k_onepass_final(const AggState<V, false>* l1, uint32_t nl1, const uint64_t* mm, 
     K raw = o;
     if (oc == ORD_SIGNED) raw ^= (K)1 << (sizeof(K) * 8 - 1);
     if (out_keys) out_keys[g] = raw;
-    if (out.sum) out.sum[g] = st.sum();
-    if (out.count) out.count[g] = (int64_t)st.n;
+    if (out.sum) out.sum[g] = (V)sum;
+    if (out.count) out.count[g] = (int64_t)cnt;
     if (out.mean) {
-      if constexpr (ValTraits<V>::is_float) out.mean[g] = (MT)st.sum() / (MT)st.n;
-      else out.mean[g] = (MT)((double)st.s / (double)st.n);
+      if constexpr (ValTraits<V>::is_float) out.mean[g] = (MT)(sum / (double)cnt);
+      else out.mean[g] = (MT)((double)(V)sum / (double)cnt);
     }
   }
   if (tid == 0) {
