@@ -20,9 +20,9 @@
 // Early exit: a wave first reads 64 keys spread over the whole input, and
 // checks its own keys again after its first step of rows; when either spans
 // 1024 or more values the final check cannot pass, so the wave records that
-// span in its block's min / max, raises the call's stop word (epoch-tagged:
-// no clearing needed) and skips its rows; other waves poll the stop word once
-// per step.
+// span in its block's min / max and skips its rows, and the call's stop word
+// (epoch-tagged: no clearing needed) is raised once, for the other waves,
+// which poll it once per step, and for the merge kernels.
 #pragma once
 
 #include "blockscan.hpp"
@@ -136,10 +136,13 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
     return __hip_atomic_load(&a.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
   };
 
-  // 64 keys spread over the whole input (index lane * n / 64 < n)
+  // 64 keys spread over the whole input (index lane * n / 64 < n). Every
+  // wave reads the same keys and reaches the same verdict, so one wave raises
+  // the stop word: same-address stores from thousands of waves serialise at
+  // the coherence point (~30 ns each, measured 130 us for a miss).
   see(a.keys[(uint64_t)lane * a.n / 64u]);
   bool stop = too_wide();
-  if (stop) raise_stop();
+  if (stop && blockIdx.x == 0 && wid == 0) raise_stop();
 
   const uint64_t wave = (uint64_t)blockIdx.x * OP_W + wid;
   const uint64_t wbeg = wave * a.rows_per_wave;
@@ -186,7 +189,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
       if (first) {
         first = false;
         if (too_wide()) {
-          raise_stop();
+          if (!stopped()) raise_stop();
           return true;
         }
       }
