@@ -35,8 +35,6 @@ constexpr int SEG_IPT = 16;
 constexpr int SEG_TILE = SEG_BLOCK * SEG_IPT;
 constexpr int SEG_W = SEG_BLOCK / 64;
 
-__host__ __device__ inline uint32_t seg_pad(uint32_t e) { return e + (e >> 5); }
-
 template <typename K, typename V, typename MT, bool MM>
 struct SegArgs {
   const K* keys;
@@ -60,6 +58,7 @@ struct SegArgs {
   uint32_t num_tiles;
   const uint32_t* seg_start; // known-segment mode: G+1 starts (last = n)
   uint32_t nseg;             // known-segment mode: G (<= 2048)
+  int vec;                   // keys, values and perm are 16-byte aligned
 };
 
 // First index i in s[0..m) with s[i] >= x (s strictly increasing).
@@ -166,11 +165,18 @@ __device__ __forceinline__ void block_seg_scan(uint32_t f, const AggState<V, MM>
 
 // KNOWN: the groups are given as sorted row ranges (seg_start) instead of
 // being detected from the keys, so no keys are read and no lookback is needed.
+//
+// Each thread loads its own 16 consecutive rows straight into registers
+// (16-byte vector loads when the inputs are aligned); the key before its
+// first row comes from the previous lane (a shuffle) or, for a wave's first
+// lane, from the previous wave's last key in LDS. With no tile staged in LDS
+// the workgroup's LDS is a few KB and occupancy is set by registers (tiles
+// staged through LDS, 68-100 KB per workgroup, ran 1-2 tiles per CU and were
+// bound by each tile's load / barrier / lookback latency).
 template <typename K, typename V, typename MT, bool MM, bool KNOWN>
 __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT, MM> a) {
-  __shared__ K s_k[KNOWN ? 1 : SEG_TILE + SEG_TILE / 32];
   __shared__ uint32_t s_starts[KNOWN ? 2049 : 1];
-  __shared__ V s_v[SEG_TILE + SEG_TILE / 32];
+  __shared__ K s_wlast[SEG_W];
   __shared__ uint32_t s_cnt[SEG_BLOCK];
   __shared__ uint32_t s_tmp[SEG_W + 1];
   __shared__ AggState<V, MM> s_wst[SEG_W + 1];
@@ -189,27 +195,52 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT,
   const uint64_t tbase = (uint64_t)tile * SEG_TILE;
   const uint32_t tn = (uint32_t)((a.n - tbase) < (uint64_t)SEG_TILE ? (a.n - tbase) : (uint64_t)SEG_TILE);
 
-  // striped (coalesced) loads into LDS in row order
+  // blocked: thread t owns rows t*IPT .. t*IPT+IPT-1 of the tile
+  const uint32_t e0 = (uint32_t)tid * SEG_IPT;
+  const uint32_t lane = lane_id();
+  const int wid = tid >> 6;
+  K kj[SEG_IPT];
+  V vj[SEG_IPT];
+  K prevk = K(0);
+  if constexpr (!KNOWN) {
+    if (tid == 0 && tbase > 0) prevk = a.keys[tbase - 1];
+  }
+  if (a.vec && e0 + SEG_IPT <= tn) {
+    if constexpr (!KNOWN) wt_load_rows(a.keys + tbase + e0, kj);
+    if (a.perm) {
+      uint32_t pj[SEG_IPT];
+      wt_load_rows(a.perm + tbase + e0, pj);
 #pragma unroll
-  for (int i = 0; i < SEG_IPT; ++i) {
-    const uint32_t e = (uint32_t)i * SEG_BLOCK + tid;
-    if (e < tn) {
-      if constexpr (!KNOWN) s_k[seg_pad(e)] = a.keys[tbase + e];
-      s_v[seg_pad(e)] = a.perm ? a.vals[a.perm[tbase + e]] : a.vals[tbase + e];
+      for (int j = 0; j < SEG_IPT; ++j) vj[j] = a.vals[pj[j]];
+    } else {
+      wt_load_rows(a.vals + tbase + e0, vj);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < SEG_IPT; ++j) {
+      const uint32_t e = e0 + j;
+      kj[j] = K(0);
+      vj[j] = V(0);
+      if (e < tn) {
+        if constexpr (!KNOWN) kj[j] = a.keys[tbase + e];
+        vj[j] = a.perm ? a.vals[a.perm[tbase + e]] : a.vals[tbase + e];
+      }
     }
   }
-  K prevk = K(0);
   if constexpr (KNOWN) {
     for (uint32_t i = tid; i < a.nseg; i += SEG_BLOCK) s_starts[i] = a.seg_start[i];
   } else {
-    if (tid == 0 && tbase > 0) prevk = a.keys[tbase - 1];
+    // the key before the thread's first row (every thread before a thread
+    // with rows has all its rows, so the neighbour's last key is valid)
+    const K up = __shfl_up(kj[SEG_IPT - 1], 1, 64);
+    if (lane == 63) s_wlast[wid] = kj[SEG_IPT - 1];
+    if (tid > 0) prevk = up;
   }
   __syncthreads();
+  if constexpr (!KNOWN) {
+    if (lane == 0 && wid > 0) prevk = s_wlast[wid - 1];
+  }
 
-  // blocked: thread t owns rows t*IPT .. t*IPT+IPT-1 of the tile
-  const uint32_t e0 = (uint32_t)tid * SEG_IPT;
-  K kj[SEG_IPT];
-  V vj[SEG_IPT];
   uint32_t hmask = 0u;
   uint32_t T, toff, texcl;
   if constexpr (KNOWN) {
@@ -221,7 +252,6 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT,
     for (int j = 0; j < SEG_IPT; ++j) {
       const uint32_t e = e0 + j;
       if (e < tn) {
-        vj[j] = s_v[seg_pad(e)];
         if (nxt < a.nseg && (uint64_t)s_starts[nxt] == tbase + e) {
           hmask |= 1u << j;
           ++nxt;
@@ -229,13 +259,10 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT,
       }
     }
   } else {
-    if (tid > 0 && e0 - 1 < tn) prevk = s_k[seg_pad(e0 - 1)];
 #pragma unroll
     for (int j = 0; j < SEG_IPT; ++j) {
       const uint32_t e = e0 + j;
       if (e < tn) {
-        kj[j] = s_k[seg_pad(e)];
-        vj[j] = s_v[seg_pad(e)];
         const K pk = j ? kj[j - 1] : prevk;
         const bool h = (tbase + e == 0) || !key_equal(kj[j], pk, a.oc);
         hmask |= (uint32_t)h << j;
@@ -488,6 +515,8 @@ int segreduce_mm(const SegWs& w, const SegCall& c, hipStream_t s) {
   a.num_tiles = (uint32_t)seg_tiles(c.n);
   a.seg_start = w.seg_start;
   a.nseg = c.nseg;
+  a.vec = ((c.known || (uintptr_t)c.keys % 16 == 0) && (uintptr_t)c.vals % 16 == 0 &&
+           (uintptr_t)c.perm % 16 == 0) ? 1 : 0;
   JFX_CHECK(a.err != nullptr, JFX_ERR_HIP, "cannot resolve the device error word");
   JFX_LAUNCH("seg_zero", s, (k_seg_zero<<<64, 256, 0, s>>>(w.desc, a.num_tiles, w.counter)));
   JFX_HIP(hipGetLastError());
