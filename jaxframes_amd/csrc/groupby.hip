@@ -888,6 +888,8 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
   void* sort_ws = cv.take<char>(sws);
   JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "groupby workspace too small");
   hipStream_t s = jfx::as_stream(stream);
+  // a key span known to be at least 2048: the sort-based path will run
+  bool wide = false;
   if (jfx::onepass_wanted(key_type, val_type, n, out_min, out_max)) {
     // one read of keys and values when the key range turns out below 1024;
     // on a miss the statistics-first path below runs
@@ -908,12 +910,19 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
       return jfx::set_error(JFX_ERR_CAPACITY, "outputs hold fewer entries than the groups this "
                                               "call produces; *out_ngroups holds the capacity needed");
     }
+    wide = res[1] >= 2048u;
+    if (wide && out_capacity < n) {
+      *out_ngroups = n;
+      return jfx::set_error(JFX_ERR_CAPACITY, "outputs hold fewer entries than the groups this "
+                                              "call may produce; *out_ngroups holds the capacity needed");
+    }
   }
   jfx::SortCtx ctx;
-  // float keys never take the small-range path: their statistics pass also
-  // makes the layout histograms of the multi-pass sort
+  // float keys never take the small-range path, nor keys whose span is known
+  // to be at least 2048: their statistics pass also makes the layout
+  // histograms of the multi-pass sort
   int rc = jfx::sort_prepare(ctx, key_type, keys, n, vb, digit_bits, sort_ws, sws, s,
-                             !jfx::int_key(key_type));
+                             !jfx::int_key(key_type) || wide);
   if (rc != JFX_OK) return rc;
   const int oc = jfx::order_code(key_type);
   const bool small = jfx::int_key(key_type) && ctx.single_pass();

@@ -253,7 +253,8 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   }
 }
 
-// Result words written by k_onepass_final: [0] status, [1] groups.
+// Result words written by k_onepass_final: [0] status, [1] groups (on a
+// miss: a lower bound of the key span max - min).
 constexpr uint32_t OP_MISS = 0u, OP_HIT = 1u, OP_CAPACITY = 2u;
 
 // Two-level merge, thread = residue (coalesced reads): block b of
@@ -347,8 +348,11 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
   const bool hit = !stopped && hi >= lo && hi - lo < (K)OP_TE;
   if (!hit) {
     if (tid == 0) {
+      // a lower bound of max - min over all keys (the keys that were read),
+      // saturated to 32 bits
+      const K span = hi >= lo ? hi - lo : K(0);
       res[0] = OP_MISS;
-      res[1] = 0u;
+      res[1] = span > (K)0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)span;
     }
     return;
   }
