@@ -19,7 +19,6 @@ import torch
 
 from . import _lib
 from ._tensors import key_code, to_device, widen_key
-from .sorting import radix_sort
 
 AGGS = ("sum", "mean", "count", "min", "max")
 
@@ -91,8 +90,43 @@ def _segreduce_ws(vcode, n, dev):
     return _lib.workspace(nb.value, dev)
 
 
+def _aggregate_one(k, kc, v, vorig, fn, n, dev, stream, digit_bits, want_keys):
+    """One value column through jfx_groupby_ex: (G, out_keys or None, column)."""
+    vcode = _VAL_CODE[v.dtype]
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_groupby_workspace_size", kc, vcode, n, ctypes.byref(nb))
+    ws = _lib.workspace(nb.value, dev)
+    g = ctypes.c_uint64(0)
+    # large inputs first try outputs sized for a small key range; the call
+    # reports the capacity it needs when the keys span more
+    cap = _SMALL_CAP if n >= _SMALL_CAP_MIN_ROWS else n
+    while True:
+        out_keys = torch.empty(cap, dtype=k.dtype, device=dev) if want_keys else None
+        out = _alloc_out(fn, v, cap, dev)
+        st = _lib.lib().jfx_groupby_ex(kc, _lib.ptr(k), vcode, _lib.ptr(v), n,
+                                       _lib.ptr(out_keys), *_out_ptrs(fn, out),
+                                       ctypes.byref(g), cap, digit_bits, _lib.ptr(ws),
+                                       ws.numel(), stream)
+        if st == _lib.JFX_ERR_CAPACITY and cap < n:
+            cap = g.value
+            continue
+        _lib.check(st, "jfx_groupby_ex")
+        break
+    G = g.value
+    return G, out_keys, _finish_col(fn, out, vorig, G)
+
+
 def groupby_aggregate(keys, values: dict, agg_funcs: dict, *, digit_bits: int = 0):
-    """Single-key groupby; returns (unique_keys, {col: aggregated})."""
+    """Single-key groupby; returns (unique_keys, {col: aggregated}).
+
+    Each aggregated column is one jfx_groupby_ex call (the one-pass or
+    small-range table kernels when the key range allows, else a radix sort
+    carrying that column and a segmented reduction); every call yields the
+    same ascending unique keys. Sorting the keys once and gathering each
+    column through the row permutation measured slower at every key range
+    (100M rows, two columns: 5.5 ms against 0.39 ms for 1000 keys, 6.2 against
+    4.0 ms for 1e6 keys), the gathers being random reads.
+    """
     _check_aggs(agg_funcs)
     _lib.require_gpu()
     k = to_device(keys)
@@ -107,50 +141,18 @@ def groupby_aggregate(keys, values: dict, agg_funcs: dict, *, digit_bits: int = 
     if n == 0:
         return k[:0].to(korig), {c: v[:0] for c, (v, _) in cols.items()}
     stream = ctypes.c_void_p(_lib.stream_handle(dev))
-    g = ctypes.c_uint64(0)
     results = {}
-    if len(agg_funcs) == 1:
-        (col, fn), = agg_funcs.items()
+    uk = None
+    G0 = None
+    for col, fn in agg_funcs.items():
         v, vorig = cols[col]
-        vcode = _VAL_CODE[v.dtype]
-        nb = ctypes.c_uint64(0)
-        _lib.call("jfx_groupby_workspace_size", kc, vcode, n, ctypes.byref(nb))
-        ws = _lib.workspace(nb.value, dev)
-        # large inputs first try outputs sized for a small key range; the call
-        # reports the capacity it needs when the keys span more
-        cap = _SMALL_CAP if n >= _SMALL_CAP_MIN_ROWS else n
-        while True:
-            out_keys = torch.empty(cap, dtype=k.dtype, device=dev)
-            out = _alloc_out(fn, v, cap, dev)
-            st = _lib.lib().jfx_groupby_ex(kc, _lib.ptr(k), vcode, _lib.ptr(v), n,
-                                           _lib.ptr(out_keys), *_out_ptrs(fn, out),
-                                           ctypes.byref(g), cap, digit_bits, _lib.ptr(ws),
-                                           ws.numel(), stream)
-            if st == _lib.JFX_ERR_CAPACITY and cap < n:
-                cap = g.value
-                continue
-            _lib.check(st, "jfx_groupby_ex")
-            break
-        G = g.value
-        results[col] = _finish_col(fn, out, vorig, G)
-        return _finish_keys(out_keys, G, korig), results
-    out_keys = torch.empty(n, dtype=k.dtype, device=dev)
-    sk, _, perm = radix_sort(k, None, True, return_index=True, digit_bits=digit_bits)
-    perm32 = perm.to(torch.int32)
-    G = None
-    for i, (col, fn) in enumerate(agg_funcs.items()):
-        v, vorig = cols[col]
-        vcode = _VAL_CODE[v.dtype]
-        ws = _segreduce_ws(vcode, n, dev)
-        out = _alloc_out(fn, v, n, dev)
-        _lib.call("jfx_segreduce", kc, _lib.ptr(sk), vcode, _lib.ptr(v), _lib.ptr(perm32), n,
-                  _lib.ptr(out_keys if i == 0 else None), *_out_ptrs(fn, out),
-                  ctypes.byref(g) if i == 0 else None, _lib.ptr(ws), ws.numel(), stream)
-        if i == 0:
-            G = g.value
-        results[col] = (fn, out, vorig)
-    results = {c: _finish_col(fn, out, vorig, G) for c, (fn, out, vorig) in results.items()}
-    return _finish_keys(out_keys, G, korig), results
+        G, out_keys, results[col] = _aggregate_one(k, kc, v, vorig, fn, n, dev, stream,
+                                                   digit_bits, uk is None)
+        if uk is None:
+            uk, G0 = _finish_keys(out_keys, G, korig), G
+        elif G != G0:
+            raise RuntimeError(f"column {col}: {G} groups, the first column gave {G0}")
+    return uk, results
 
 
 def _finish_keys(out_keys, G, korig):
