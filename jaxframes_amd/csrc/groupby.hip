@@ -782,7 +782,7 @@ int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const 
                                                                           a.epoch, l1)));
   JFX_HIP(hipGetLastError());
   JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT><<<1, 1024, 0, s>>>(
-                                      l1, ng1, a.mm, grid, a.ctl, a.epoch, oc,
+                                      l1, ng1, a.mm, grid, a.ctl, a.epoch, a.keys, n, oc,
                                       static_cast<K*>(out_keys), o, capacity, w.op_ctl + 2)));
   JFX_HIP(hipGetLastError());
   return read_back(s, w.op_ctl + 2, 2 * sizeof(uint32_t), res);

@@ -99,11 +99,28 @@ __device__ __forceinline__ K wave_max_k(K v) {
   return v;
 }
 
+// Residues per lane-group table for a 64-key sample spanning `span` values
+// (2x margin): few distinct keys would make the lanes of a wave collide on
+// one entry (same-address LDS atomics serialise), so the 1024 entries are
+// split into OP_TE / rt tables and lane l uses table l % (OP_TE / rt). The
+// final check then requires max - min < rt. Both kernels derive rt from the
+// same 64 keys, so every wave agrees without communicating.
+template <typename K>
+__device__ __forceinline__ uint32_t op_residues(K span) {
+  return span < (K)8 ? 16u : span < (K)32 ? 64u : span < (K)128 ? 256u : OP_TE;
+}
+
+// Span (max - min of the ordered keys) of the 64 keys at rows lane * n / 64.
+template <typename K>
+__device__ __forceinline__ K op_sample_span(const K* keys, uint64_t n, int oc, uint32_t lane) {
+  const K o = ordered(keys[(uint64_t)lane * n / 64u], oc);
+  return wave_max_k(o) - wave_min_k(o);
+}
+
 template <typename K, typename V, bool VEC>
 __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a) {
   using T = typename WaAcc<V>::T;
   constexpr uint64_t STEP = 64u * WT_RPL;
-  constexpr uint32_t M = OP_TE - 1u;
   __shared__ T s_acc[OP_W * OP_TE];
   __shared__ uint32_t s_cnt[OP_TE];
   __shared__ K s_mn[OP_W], s_mx[OP_W];
@@ -113,9 +130,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
 
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lane = lane_id();
-  T* accs = s_acc + wid * OP_TE;
-  uint32_t* cnts = s_cnt;
-  for (uint32_t i = lane; i < OP_TE; i += 64) accs[i] = T(0);
+  for (uint32_t i = lane; i < OP_TE; i += 64) s_acc[wid * OP_TE + i] = T(0);
   __syncthreads();
 
   K omin = ~K(0), omax = K(0);
@@ -124,10 +139,21 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
     omin = o < omin ? o : omin;
     omax = o > omax ? o : omax;
   };
-  // the wave's keys so far span OP_TE or more values: the final check fails
+  // 64 keys spread over the whole input (index lane * n / 64 < n) choose the
+  // lane-group tables (every wave reads the same keys)
+  const K sspan = op_sample_span(a.keys, a.n, a.oc, lane);
+  const uint32_t rt = op_residues(sspan);
+  const uint32_t M = rt - 1u;
+  // lane group g holds residue r at entry (r ^ g) mod rt of its table: with
+  // tables rt entries apart, an unswizzled residue would put the lanes of a
+  // wave on a few LDS banks
+  const uint32_t gsw = lane & (OP_TE / rt - 1u);
+  T* accs = s_acc + wid * OP_TE + gsw * rt;
+  uint32_t* cnts = s_cnt + gsw * rt;
+  // the wave's keys so far span rt or more values: the final check fails
   auto too_wide = [&]() -> bool {
     const K lo = wave_min_k(omin), hi = wave_max_k(omax);
-    return hi >= lo && hi - lo >= (K)OP_TE;
+    return hi >= lo && hi - lo >= (K)rt;
   };
   auto raise_stop = [&]() {
     if (lane == 0) __hip_atomic_store(&a.ctl[0], a.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -136,12 +162,12 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
     return __hip_atomic_load(&a.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
   };
 
-  // 64 keys spread over the whole input (index lane * n / 64 < n). Every
-  // wave reads the same keys and reaches the same verdict, so one wave raises
-  // the stop word: same-address stores from thousands of waves serialise at
-  // the coherence point (~30 ns each, measured 130 us for a miss).
+  // The sample already spanning OP_TE or more values is a miss. Every wave
+  // reaches the same verdict, so one wave raises the stop word: same-address
+  // stores from thousands of waves serialise at the coherence point (~30 ns
+  // each, measured 130 us for a miss).
   see(a.keys[(uint64_t)lane * a.n / 64u]);
-  bool stop = too_wide();
+  bool stop = sspan >= (K)OP_TE;
   if (stop && blockIdx.x == 0 && wid == 0) raise_stop();
 
   const uint64_t wave = (uint64_t)blockIdx.x * OP_W + wid;
@@ -159,7 +185,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   };
   auto row = [&](K key, V x) {
     see(key);
-    const uint32_t r = (uint32_t)key & M;
+    const uint32_t r = ((uint32_t)key ^ gsw) & M;
     flush(run_r, run, run_n, r != run_r && run_r != OP_NO_RUN);
     const bool same = r == run_r;
     run = same ? run + (T)x : (T)x;
@@ -225,7 +251,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
         v = (T)a.vals[r];
         see(key);
       }
-      flush((uint32_t)key & M, v, 1u, valid);
+      flush(((uint32_t)key ^ gsw) & M, v, 1u, valid);
     }
   }
 
@@ -246,10 +272,18 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
     a.mm[(size_t)blockIdx.x * 2 + 1] = (uint64_t)bh;
   }
   if (s_stop) return;  // block-uniform: the result will be a miss
+  // residue r < rt: its lane-group tables in (wave, group) order; the
+  // entries from rt on are unused
+  const uint32_t lg = OP_TE / rt;
   for (uint32_t r = threadIdx.x; r < OP_TE; r += OP_W * 64) {
     T acc = T(0);
-    for (int w = 0; w < OP_W; ++w) acc += s_acc[w * OP_TE + r];
-    a.part[(size_t)blockIdx.x * OP_TE + r] = op_pack<V>(acc, s_cnt[r]);
+    uint32_t c = 0u;
+    if (r < rt) {
+      for (int w = 0; w < OP_W; ++w)
+        for (uint32_t g = 0; g < lg; ++g) acc += s_acc[w * OP_TE + g * rt + ((r ^ g) & (rt - 1u))];
+      for (uint32_t g = 0; g < lg; ++g) c += s_cnt[g * rt + ((r ^ g) & (rt - 1u))];
+    }
+    a.part[(size_t)blockIdx.x * OP_TE + r] = op_pack<V>(acc, c);
   }
 }
 
@@ -305,18 +339,24 @@ k_onepass_merge(const OpPart<V>* part, uint32_t nblocks, const uint32_t* ctl, ui
 template <typename K, typename V, typename MT>
 __global__ void __launch_bounds__(1024)
 k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
-                const uint32_t* ctl, uint32_t epoch, int oc, K* out_keys, GroupOut<V, MT> out,
-                uint64_t capacity, uint32_t* res) {
+                const uint32_t* ctl, uint32_t epoch, const K* keys, uint64_t n, int oc,
+                K* out_keys, GroupOut<V, MT> out, uint64_t capacity, uint32_t* res) {
   using T = typename WaAcc<V>::T;
   __shared__ K s_lo[16], s_hi[16];
   __shared__ T s_sum[OP_TE];
   __shared__ uint32_t s_n[OP_TE];
   __shared__ uint32_t s_gid[OP_TE];
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
+  __shared__ uint32_t s_rt;
   const int tid = threadIdx.x;
   const int wid = tid >> 6;
   const uint32_t lane = lane_id();
   const bool stopped = ctl[0] == epoch;
+  // the table kernel's residues per table, from the same 64 keys
+  if (wid == 0) {
+    const K sspan = op_sample_span(keys, n, oc, lane);
+    if (lane == 0) s_rt = op_residues(sspan);
+  }
   // block min / max loads first, so that they are in flight with the merge's
   K lo = ~K(0), hi = K(0);
   for (uint32_t b = tid; b < nblocks; b += 1024) {
@@ -345,7 +385,8 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
     lo = s_lo[w] < lo ? s_lo[w] : lo;
     hi = s_hi[w] > hi ? s_hi[w] : hi;
   }
-  const bool hit = !stopped && hi >= lo && hi - lo < (K)OP_TE;
+  const uint32_t rt = s_rt;
+  const bool hit = !stopped && hi >= lo && hi - lo < (K)rt;
   if (!hit) {
     if (tid == 0) {
       // a lower bound of max - min over all keys (the keys that were read),
@@ -360,7 +401,7 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
   const uint32_t span = (uint32_t)(hi - lo);
   const uint32_t d = (uint32_t)tid;
   const K o = lo + (K)d;
-  const uint32_t r = (uint32_t)o & (OP_TE - 1u);
+  const uint32_t r = (uint32_t)o & (rt - 1u);
   const T sum = d <= span ? s_sum[r] : T(0);
   const uint32_t cnt = d <= span ? s_n[r] : 0u;
   const bool present = cnt != 0u;

@@ -213,3 +213,35 @@ def test_onepass_default_threshold(monkeypatch, n, onepass):
                                                                        {"v": "sum"}))
     assert ("onepass_table" in ran) == onepass and ("sort_stats" in ran) != onepass, ran
     _check(keys, vals, uk, res["v"])
+
+
+@pytest.mark.parametrize("R", [1, 2, 5, 10, 40, 100, 200])
+def test_onepass_lane_group_tables(R):
+    """Few distinct keys: the sample picks lane-group tables (16, 64 or 256
+    residues each) so that lanes do not collide on one entry."""
+    rng = np.random.default_rng(R)
+    keys = (rng.integers(0, R, N) + 1_000_000).astype(np.int64)
+    vals = rng.standard_normal(N).astype(np.float32)
+    (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)},
+                                                                       {"v": "sum"}))
+    assert "onepass_table" in ran and "sort_stats" not in ran, ran
+    _check(keys, vals, uk, res["v"])
+    (uk, res) = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "count"})
+    assert np.array_equal(to_numpy(res["v"]), np.unique(keys, return_counts=True)[1])
+
+
+@pytest.mark.parametrize("rare", [50, 900, 3000])
+def test_onepass_narrow_sample_wider_keys(rare):
+    """The 64 sampled rows (i * n / 64) hold 4 distinct keys, so the tables
+    are 16 residues wide; a few other rows hold a wider key: the range check
+    must miss, and the statistics-first path gives the groups."""
+    rng = np.random.default_rng(rare)
+    keys = rng.integers(0, 4, N).astype(np.int32)
+    sampled = {i * N // 64 for i in range(64)}
+    rows = [r for r in (N // 3 + 7, N // 2 + 11, N - 5) if r not in sampled]
+    keys[rows] = rare
+    vals = rng.standard_normal(N).astype(np.float32)
+    (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)},
+                                                                       {"v": "sum"}))
+    assert "onepass_table" in ran and any(k.startswith("sort_stats") for k in ran), ran
+    _check(keys, vals, uk, res["v"])
