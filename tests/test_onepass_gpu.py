@@ -245,3 +245,23 @@ def test_onepass_narrow_sample_wider_keys(rare):
                                                                        {"v": "sum"}))
     assert "onepass_table" in ran and any(k.startswith("sort_stats") for k in ran), ran
     _check(keys, vals, uk, res["v"])
+
+
+@pytest.mark.parametrize("hot,R", [(0.5, 1000), (0.8, 3), (0.3, 200), ("zipf", 1000)])
+def test_onepass_frequent_key(hot, R):
+    """A frequent key makes many lanes of a wave add to one table entry in
+    the same instruction (serialised LDS atomics): exact counts, float sums
+    to the FP64 bound, exact integer sums."""
+    rng = np.random.default_rng(R)
+    if hot == "zipf":
+        keys = np.minimum(rng.zipf(1.3, N) - 1, R - 1).astype(np.int32)
+    else:
+        keys = np.where(rng.random(N) < hot, 1, rng.integers(0, R, N)).astype(np.int32)
+    for vals in (rng.standard_normal(N).astype(np.float32),
+                 rng.integers(-2**31, 2**31 - 1, N, dtype=np.int32)):
+        (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(
+            cuda(keys), {"v": cuda(vals)}, {"v": "sum"}))
+        assert "onepass_table" in ran and "sort_stats" not in ran, ran
+        _check(keys, vals, uk, res["v"])
+    uk, res = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "count"})
+    assert np.array_equal(to_numpy(res["v"]), np.unique(keys, return_counts=True)[1])
