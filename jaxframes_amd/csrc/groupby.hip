@@ -736,19 +736,20 @@ bool onepass_wanted(int kt, int vt, uint64_t n, const void* out_min, const void*
   return n >= OP_MIN_ROWS;
 }
 
-template <typename K, typename V, typename MT>
-int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const SegWs& w,
-                  void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t* res,
-                  hipStream_t s) {
+template <typename K, typename V, typename MT, uint32_t TE>
+int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const SegWs& w,
+                void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t rt_min,
+                uint32_t* res, hipStream_t s) {
   constexpr uint64_t STEP = 64u * WT_RPL;
   // LDS per block: a sum table per wave and one count table
-  constexpr uint64_t BLK = (uint64_t)OP_W * OP_TE * sizeof(typename WaAcc<V>::T) + OP_TE * 4u;
+  constexpr uint64_t BLK = (uint64_t)OP_W * TE * sizeof(typename WaAcc<V>::T) + TE * 4u;
   OnePassArgs<K, V> a{};
   a.keys = static_cast<const K*>(keys);
   a.vals = static_cast<const V*>(vals);
   a.n = n;
   a.oc = oc;
   a.epoch = ++g_op_epoch;
+  a.rt_min = rt_min;
   a.part = static_cast<OpPart<V>*>(w.part);
   a.mm = w.op_mm;
   a.ctl = w.op_ctl;
@@ -761,10 +762,11 @@ int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const 
   a.rows_per_wave = ((n + (uint64_t)grid * OP_W - 1) / ((uint64_t)grid * OP_W) + STEP - 1) / STEP * STEP;
   grid = (uint32_t)((n + a.rows_per_wave * OP_W - 1) / (a.rows_per_wave * OP_W));
   const bool vec = (uintptr_t)keys % 16 == 0 && (uintptr_t)vals % 16 == 0;
+  const char* nm = TE == OP_TE ? "onepass_table" : "onepass_table_2k";
   if (vec) {
-    JFX_LAUNCH("onepass_table", s, (k_onepass_table<K, V, true><<<grid, OP_W * 64, 0, s>>>(a)));
+    JFX_LAUNCH(nm, s, (k_onepass_table<K, V, true, TE><<<grid, OP_W * 64, 0, s>>>(a)));
   } else {
-    JFX_LAUNCH("onepass_table", s, (k_onepass_table<K, V, false><<<grid, OP_W * 64, 0, s>>>(a)));
+    JFX_LAUNCH(nm, s, (k_onepass_table<K, V, false, TE><<<grid, OP_W * 64, 0, s>>>(a)));
   }
   JFX_HIP(hipGetLastError());
   GroupOut<V, MT> o{static_cast<V*>(sc.out_sum), sc.out_count, nullptr, nullptr,
@@ -775,28 +777,40 @@ int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const 
   JFX_CHECK(ng1 <= OP_L1, JFX_ERR_UNSUPPORTED, "one-pass grid exceeds the final merge");
   const uint64_t part_bytes = (uint64_t)SORT_MAXCHUNKS * 2048 *
                               (sizeof(V) == 4 ? sizeof(AggState<float>) : sizeof(AggState<double>));
-  JFX_CHECK(((uint64_t)grid + ng1) * OP_TE * sizeof(OpPart<V>) <= part_bytes,
+  JFX_CHECK(((uint64_t)grid + ng1) * TE * sizeof(OpPart<V>) <= part_bytes,
             JFX_ERR_WORKSPACE, "one-pass partials exceed the partial buffer");
-  OpPart<V>* l1 = a.part + (size_t)grid * OP_TE;
-  JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V><<<ng1, 1024, 0, s>>>(a.part, grid, a.ctl,
-                                                                          a.epoch, l1)));
+  OpPart<V>* l1 = a.part + (size_t)grid * TE;
+  JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V, TE><<<ng1, 1024, 0, s>>>(a.part, grid, a.ctl,
+                                                                              a.epoch, l1)));
   JFX_HIP(hipGetLastError());
-  JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT><<<1, 1024, 0, s>>>(
+  JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT, TE><<<1, 1024, 0, s>>>(
                                       l1, ng1, a.mm, grid, a.ctl, a.epoch, a.keys, n, oc,
-                                      static_cast<K*>(out_keys), o, capacity, w.op_ctl + 2)));
+                                      rt_min, static_cast<K*>(out_keys), o, capacity,
+                                      w.op_ctl + 2)));
   JFX_HIP(hipGetLastError());
   return read_back(s, w.op_ctl + 2, 2 * sizeof(uint32_t), res);
 }
 
+// te = 1024 or 2048 residues; rt_min = te turns the lane-group tables off.
+template <typename K, typename V, typename MT>
+int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const SegWs& w,
+                  void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t te,
+                  uint32_t rt_min, uint32_t* res, hipStream_t s) {
+  if (te == 2 * OP_TE)
+    return onepass_run<K, V, MT, 2 * OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
+  return onepass_run<K, V, MT, OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
+}
+
 template <typename K>
 int onepass_key(int oc, int vt, const void* keys, const void* vals, uint64_t n, const SegWs& w,
-                void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t* res, hipStream_t s) {
+                void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t te,
+                uint32_t rt_min, uint32_t* res, hipStream_t s) {
   switch (vt) {
-    case JFX_F32: return onepass_typed<K, float, float>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
-    case JFX_I32: return onepass_typed<K, int32_t, float>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
-    case JFX_I64: return onepass_typed<K, int64_t, double>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
-    case JFX_U32: return onepass_typed<K, uint32_t, float>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
-    case JFX_U64: return onepass_typed<K, uint64_t, double>(oc, keys, vals, n, w, out_keys, sc, capacity, res, s);
+    case JFX_F32: return onepass_typed<K, float, float>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
+    case JFX_I32: return onepass_typed<K, int32_t, float>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
+    case JFX_I64: return onepass_typed<K, int64_t, double>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
+    case JFX_U32: return onepass_typed<K, uint32_t, float>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
+    case JFX_U64: return onepass_typed<K, uint64_t, double>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
   }
   return set_error(JFX_ERR_INVALID, "bad value type for the one-pass groupby");
 }
@@ -896,11 +910,22 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
     jfx::SegCall oc{nullptr, jfx::order_code(key_type), vals, nullptr, n, out_keys, out_sum,
                     out_count, nullptr, nullptr, out_mean, true, 0u};
     uint32_t res[2] = {0u, 0u};
-    int rc = kb == 4 ? jfx::onepass_key<uint32_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc,
-                                                  out_capacity, res, s)
+    auto attempt = [&](uint32_t te, uint32_t rt_min) -> int {
+      return kb == 4 ? jfx::onepass_key<uint32_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc,
+                                                  out_capacity, te, rt_min, res, s)
                      : jfx::onepass_key<uint64_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc,
-                                                  out_capacity, res, s);
+                                                  out_capacity, te, rt_min, res, s);
+    };
+    int rc = attempt(jfx::OP_TE, 16u);
     if (rc != JFX_OK) return rc;
+    // one retry on a miss whose span seen so far still fits: 1024-residue
+    // tables without lane groups (the sample chose too narrow ones), or
+    // 2048-residue tables
+    if (res[0] == jfx::OP_MISS && res[1] < 2 * jfx::OP_TE) {
+      const uint32_t te = res[1] < jfx::OP_TE ? jfx::OP_TE : 2 * jfx::OP_TE;
+      rc = attempt(te, te);
+      if (rc != JFX_OK) return rc;
+    }
     if (res[0] == jfx::OP_HIT) {
       *out_ngroups = res[1];
       return JFX_OK;

@@ -33,7 +33,7 @@
 
 namespace jfx {
 
-constexpr uint32_t OP_TE = 1024;  // table entries (residues) per wave
+constexpr uint32_t OP_TE = 1024;  // table entries (residues) per wave; 2048 for the wider retry
 constexpr int OP_W = 4;           // waves per block
 constexpr uint32_t OP_NO_RUN = 0xFFFFFFFFu;
 
@@ -75,7 +75,8 @@ struct OnePassArgs {
   uint64_t rows_per_wave;    // multiple of the wave step
   int oc;                    // key order code
   uint32_t epoch;            // this call's tag for the stop word
-  OpPart<V>* part;           // [gridDim.x][OP_TE] block tables
+  uint32_t rt_min;           // residues per lane-group table at least (TE: no lane groups)
+  OpPart<V>* part;           // [gridDim.x][TE] block tables
   uint64_t* mm;              // [gridDim.x][2] ordered min and max of the block's keys
   uint32_t* ctl;             // [0] stop word
 };
@@ -102,12 +103,12 @@ __device__ __forceinline__ K wave_max_k(K v) {
 // Residues per lane-group table for a 64-key sample spanning `span` values
 // (2x margin): few distinct keys would make the lanes of a wave collide on
 // one entry (same-address LDS atomics serialise), so the 1024 entries are
-// split into OP_TE / rt tables and lane l uses table l % (OP_TE / rt). The
+// split into TE / rt tables and lane l uses table l % (TE / rt). The
 // final check then requires max - min < rt. Both kernels derive rt from the
 // same 64 keys, so every wave agrees without communicating.
-template <typename K>
+template <uint32_t TE, typename K>
 __device__ __forceinline__ uint32_t op_residues(K span) {
-  return span < (K)8 ? 16u : span < (K)32 ? 64u : span < (K)128 ? 256u : OP_TE;
+  return span < (K)8 ? 16u : span < (K)32 ? 64u : span < (K)128 ? 256u : TE;
 }
 
 // Span (max - min of the ordered keys) of the 64 keys at rows lane * n / 64.
@@ -117,20 +118,20 @@ __device__ __forceinline__ K op_sample_span(const K* keys, uint64_t n, int oc, u
   return wave_max_k(o) - wave_min_k(o);
 }
 
-template <typename K, typename V, bool VEC>
+template <typename K, typename V, bool VEC, uint32_t TE>
 __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a) {
   using T = typename WaAcc<V>::T;
   constexpr uint64_t STEP = 64u * WT_RPL;
-  __shared__ T s_acc[OP_W * OP_TE];
-  __shared__ uint32_t s_cnt[OP_TE];
+  __shared__ T s_acc[OP_W * TE];
+  __shared__ uint32_t s_cnt[TE];
   __shared__ K s_mn[OP_W], s_mx[OP_W];
   __shared__ uint32_t s_stop;
   if (threadIdx.x == 0) s_stop = 0u;
-  for (uint32_t i = threadIdx.x; i < OP_TE; i += OP_W * 64) s_cnt[i] = 0u;
+  for (uint32_t i = threadIdx.x; i < TE; i += OP_W * 64) s_cnt[i] = 0u;
 
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lane = lane_id();
-  for (uint32_t i = lane; i < OP_TE; i += 64) s_acc[wid * OP_TE + i] = T(0);
+  for (uint32_t i = lane; i < TE; i += 64) s_acc[wid * TE + i] = T(0);
   __syncthreads();
 
   K omin = ~K(0), omax = K(0);
@@ -142,13 +143,14 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   // 64 keys spread over the whole input (index lane * n / 64 < n) choose the
   // lane-group tables (every wave reads the same keys)
   const K sspan = op_sample_span(a.keys, a.n, a.oc, lane);
-  const uint32_t rt = op_residues(sspan);
+  const uint32_t rt0 = op_residues<TE>(sspan);
+  const uint32_t rt = rt0 > a.rt_min ? rt0 : a.rt_min;
   const uint32_t M = rt - 1u;
   // lane group g holds residue r at entry (r ^ g) mod rt of its table: with
   // tables rt entries apart, an unswizzled residue would put the lanes of a
   // wave on a few LDS banks
-  const uint32_t gsw = lane & (OP_TE / rt - 1u);
-  T* accs = s_acc + wid * OP_TE + gsw * rt;
+  const uint32_t gsw = lane & (TE / rt - 1u);
+  T* accs = s_acc + wid * TE + gsw * rt;
   uint32_t* cnts = s_cnt + gsw * rt;
   // the wave's keys so far span rt or more values: the final check fails
   auto too_wide = [&]() -> bool {
@@ -162,12 +164,12 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
     return __hip_atomic_load(&a.ctl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == a.epoch;
   };
 
-  // The sample already spanning OP_TE or more values is a miss. Every wave
+  // The sample already spanning TE or more values is a miss. Every wave
   // reaches the same verdict, so one wave raises the stop word: same-address
   // stores from thousands of waves serialise at the coherence point (~30 ns
   // each, measured 130 us for a miss).
   see(a.keys[(uint64_t)lane * a.n / 64u]);
-  bool stop = sspan >= (K)OP_TE;
+  bool stop = sspan >= (K)TE;
   if (stop && blockIdx.x == 0 && wid == 0) raise_stop();
 
   const uint64_t wave = (uint64_t)blockIdx.x * OP_W + wid;
@@ -274,16 +276,16 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   if (s_stop) return;  // block-uniform: the result will be a miss
   // residue r < rt: its lane-group tables in (wave, group) order; the
   // entries from rt on are unused
-  const uint32_t lg = OP_TE / rt;
-  for (uint32_t r = threadIdx.x; r < OP_TE; r += OP_W * 64) {
+  const uint32_t lg = TE / rt;
+  for (uint32_t r = threadIdx.x; r < TE; r += OP_W * 64) {
     T acc = T(0);
     uint32_t c = 0u;
     if (r < rt) {
       for (int w = 0; w < OP_W; ++w)
-        for (uint32_t g = 0; g < lg; ++g) acc += s_acc[w * OP_TE + g * rt + ((r ^ g) & (rt - 1u))];
+        for (uint32_t g = 0; g < lg; ++g) acc += s_acc[w * TE + g * rt + ((r ^ g) & (rt - 1u))];
       for (uint32_t g = 0; g < lg; ++g) c += s_cnt[g * rt + ((r ^ g) & (rt - 1u))];
     }
-    a.part[(size_t)blockIdx.x * OP_TE + r] = op_pack<V>(acc, c);
+    a.part[(size_t)blockIdx.x * TE + r] = op_pack<V>(acc, c);
   }
 }
 
@@ -323,29 +325,32 @@ __device__ __forceinline__ void merge_seq(typename WaAcc<V>::T& acc, uint32_t& n
   }
 }
 
-template <typename V>
+template <typename V, uint32_t TE>
 __global__ void __launch_bounds__(1024)
 k_onepass_merge(const OpPart<V>* part, uint32_t nblocks, const uint32_t* ctl, uint32_t epoch,
                 OpPart<V>* l1) {
   if (ctl[0] == epoch) return;  // stopped: the result is a miss
   const uint32_t c0 = blockIdx.x * OP_GROUP;
   const uint32_t c1 = (c0 + OP_GROUP) < nblocks ? (c0 + OP_GROUP) : nblocks;
-  typename WaAcc<V>::T acc(0);
-  uint32_t n = 0u;
-  merge_seq<V>(acc, n, part + (size_t)c0 * OP_TE + threadIdx.x, OP_TE, c1 - c0);
-  l1[(size_t)blockIdx.x * OP_TE + threadIdx.x] = op_pack<V>(acc, n);
+  for (uint32_t r = threadIdx.x; r < TE; r += 1024) {
+    typename WaAcc<V>::T acc(0);
+    uint32_t n = 0u;
+    merge_seq<V>(acc, n, part + (size_t)c0 * TE + r, TE, c1 - c0);
+    l1[(size_t)blockIdx.x * TE + r] = op_pack<V>(acc, n);
+  }
 }
 
-template <typename K, typename V, typename MT>
+template <typename K, typename V, typename MT, uint32_t TE>
 __global__ void __launch_bounds__(1024)
 k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
                 const uint32_t* ctl, uint32_t epoch, const K* keys, uint64_t n, int oc,
-                K* out_keys, GroupOut<V, MT> out, uint64_t capacity, uint32_t* res) {
+                uint32_t rt_min, K* out_keys, GroupOut<V, MT> out, uint64_t capacity,
+                uint32_t* res) {
   using T = typename WaAcc<V>::T;
   __shared__ K s_lo[16], s_hi[16];
-  __shared__ T s_sum[OP_TE];
-  __shared__ uint32_t s_n[OP_TE];
-  __shared__ uint32_t s_gid[OP_TE];
+  __shared__ T s_sum[TE];
+  __shared__ uint32_t s_n[TE];
+  __shared__ uint32_t s_gid[TE];
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
   __shared__ uint32_t s_rt;
   const int tid = threadIdx.x;
@@ -355,7 +360,8 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
   // the table kernel's residues per table, from the same 64 keys
   if (wid == 0) {
     const K sspan = op_sample_span(keys, n, oc, lane);
-    if (lane == 0) s_rt = op_residues(sspan);
+    const uint32_t rt0 = op_residues<TE>(sspan);
+    if (lane == 0) s_rt = rt0 > rt_min ? rt0 : rt_min;
   }
   // block min / max loads first, so that they are in flight with the merge's
   K lo = ~K(0), hi = K(0);
@@ -365,11 +371,13 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
     hi = h > hi ? h : hi;
   }
   if (!stopped) {
-    T acc(0);
-    uint32_t n = 0u;
-    merge_seq<V>(acc, n, l1 + tid, OP_TE, nl1);
-    s_sum[tid] = acc;
-    s_n[tid] = n;
+    for (uint32_t r = tid; r < TE; r += 1024) {
+      T acc(0);
+      uint32_t n = 0u;
+      merge_seq<V>(acc, n, l1 + r, TE, nl1);
+      s_sum[r] = acc;
+      s_n[r] = n;
+    }
   }
 
   lo = wave_min_k(lo);
@@ -398,16 +406,14 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
     return;
   }
   // digit d = ordered key - lo, in key order; its residue holds its group
+  // (TE / 1024 digits per thread)
   const uint32_t span = (uint32_t)(hi - lo);
-  const uint32_t d = (uint32_t)tid;
-  const K o = lo + (K)d;
-  const uint32_t r = (uint32_t)o & (rt - 1u);
-  const T sum = d <= span ? s_sum[r] : T(0);
-  const uint32_t cnt = d <= span ? s_n[r] : 0u;
-  const bool present = cnt != 0u;
-  s_gid[d] = present ? 1u : 0u;
+  auto count_of = [&](uint32_t d) -> uint32_t {
+    return d <= span ? s_n[(uint32_t)(lo + (K)d) & (rt - 1u)] : 0u;
+  };
+  for (uint32_t d = tid; d < TE; d += 1024) s_gid[d] = count_of(d) != 0u ? 1u : 0u;
   __syncthreads();
-  const uint32_t G = block_excl_scan_lds<OP_TE, 1024>(s_gid, s_tmp);
+  const uint32_t G = block_excl_scan_lds<TE, 1024>(s_gid, s_tmp);
   if ((uint64_t)G > capacity) {
     if (tid == 0) {
       res[0] = OP_CAPACITY;
@@ -415,7 +421,11 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
     }
     return;
   }
-  if (present) {
+  for (uint32_t d = tid; d < TE; d += 1024) {
+    const uint32_t cnt = count_of(d);
+    if (cnt == 0u) continue;
+    const K o = lo + (K)d;
+    const T sum = s_sum[(uint32_t)o & (rt - 1u)];
     const uint32_t g = s_gid[d];
     K raw = o;
     if (oc == ORD_SIGNED) raw ^= (K)1 << (sizeof(K) * 8 - 1);

@@ -104,13 +104,10 @@ def test_onepass_count_and_mean():
                                        rtol=1e-5, atol=1e-6)
 
 
-@pytest.mark.parametrize("case", ["span1024", "outlier", "sorted_wide", "two_clusters"])
+@pytest.mark.parametrize("case", ["outlier", "sorted_wide", "two_clusters"])
 def test_onepass_miss_falls_back(case):
     rng = np.random.default_rng(7)
-    if case == "span1024":        # keys 0..1024: 1025 values, one too many
-        keys = rng.integers(0, 1025, N).astype(np.int32)
-        keys[:2] = [0, 1024]
-    elif case == "outlier":       # one key far away, late in the input
+    if case == "outlier":         # one key far away, late in the input
         keys = rng.integers(0, 1000, N).astype(np.int32)
         keys[N - 12345] = 10**6
     elif case == "sorted_wide":   # every wave sees a narrow slice
@@ -234,17 +231,40 @@ def test_onepass_lane_group_tables(R):
 def test_onepass_narrow_sample_wider_keys(rare):
     """The 64 sampled rows (i * n / 64) hold 4 distinct keys, so the tables
     are 16 residues wide; a few other rows hold a wider key: the range check
-    must miss, and the statistics-first path gives the groups."""
+    must miss. A span still below 1024 is retried with 1024-residue tables
+    (a hit), a wider one goes to the statistics-first path."""
     rng = np.random.default_rng(rare)
     keys = rng.integers(0, 4, N).astype(np.int32)
     sampled = {i * N // 64 for i in range(64)}
     rows = [r for r in (N // 3 + 7, N // 2 + 11, N - 5) if r not in sampled]
     keys[rows] = rare
     vals = rng.standard_normal(N).astype(np.float32)
+    with _lib.KernelTimer() as kt:
+        uk, res = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "sum"})
+        torch.cuda.synchronize()
+    ran = kt.stats()
+    assert ran["onepass_table"][1] == (2 if rare < 1024 else 1), ran
+    assert any(k.startswith("sort_stats") for k in ran) == (rare >= 1024), ran
+    _check(keys, vals, uk, res["v"])
+
+
+@pytest.mark.parametrize("lo,hi", [(0, 1025), (-700, 800), (5, 2052)])
+def test_onepass_wider_tables(lo, hi):
+    """Key spans of 1024..2047 values: the first try misses and the retry
+    with 2048-residue tables hits (no statistics pass)."""
+    rng = np.random.default_rng(hi)
+    keys = rng.integers(lo, hi, N).astype(np.int64)
+    keys[:2] = [lo, hi - 1]
+    vals = rng.standard_normal(N).astype(np.float32)
     (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)},
                                                                        {"v": "sum"}))
-    assert "onepass_table" in ran and any(k.startswith("sort_stats") for k in ran), ran
+    assert "onepass_table_2k" in ran and not any(k.startswith("sort_stats") for k in ran), ran
     _check(keys, vals, uk, res["v"])
+    uk, res = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "mean"})
+    sk, sv, starts = _groups(keys, vals)
+    np.testing.assert_allclose(to_numpy(res["v"]),
+                               oracle.segment_sums_f64(sv, starts) / np.diff(np.append(starts, N)),
+                               rtol=1e-5, atol=1e-6)
 
 
 @pytest.mark.parametrize("hot,R", [(0.5, 1000), (0.8, 3), (0.3, 200), ("zipf", 1000)])
