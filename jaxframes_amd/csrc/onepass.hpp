@@ -1,28 +1,31 @@
-// One-pass groupby for integer keys whose range turns out to be below 1024:
-// sums, counts and means (no min / max) of float32 or integer values.
+// One-pass groupby for integer keys whose range turns out to be below the
+// table width TE (1024, or 2048 for a retry): sums, counts and means (no min
+// / max) of float32 or integer values.
 //
-// Distinct integers inside any window of 1024 consecutive values have
-// distinct residues mod 1024, so wave tables indexed by key & 1023 need no
-// minimum key in advance. k_onepass_table reads keys and values once: each
-// lane folds runs of equal residues in registers (as the wave-table kernel
-// does) and adds every finished run with two LDS atomics: the sum into its
-// wave's table (ds_add_f64 / integer add; wave-private so that the float add
-// order is fixed) and the row count into its block's table (ds_add_u32:
-// integer adds commute, so the 4 waves share one count table, 36 KB of LDS
-// per block and 4 blocks per CU). It also tracks the ordered min / max of
-// every key it reads.
+// Distinct integers inside any window of TE consecutive values have distinct
+// residues mod TE, so wave tables indexed by key mod TE need no minimum key in
+// advance. k_onepass_table reads keys and values once: each lane folds runs
+// of equal residues in registers (as the wave-table kernel does) and adds
+// every finished run with two LDS atomics: the sum into its wave's table
+// (ds_add_f64 / integer add; wave-private so that the float add order is
+// fixed) and the row count into its block's table (ds_add_u32: integer adds
+// commute, so the 4 waves share one count table, 36 KB of LDS per block and 4
+// blocks per CU at TE = 1024). It also tracks the ordered min / max of every
+// key it reads. When 64 keys sampled over the whole input span few values,
+// the table is split into lane-group tables of rt < TE residues (op_residues).
 // k_onepass_merge and k_onepass_final merge the block tables per residue in
-// block order, take the global min / max, and when max - min < 1024 write the
-// groups in key order (digit d = ordered(key) - min, residue = (min + d) &
-// 1023). Otherwise it reports a miss and the caller runs the statistics-first
+// block order, take the global min / max, and when max - min < rt write the
+// groups in key order (digit d = ordered(key) - min, residue = (min + d) mod
+// rt). Otherwise they report a miss with a lower bound of the key span, and
+// the caller retries once with wider tables or runs the statistics-first
 // path. The result of a hit is a fixed function of the row layout.
 //
-// Early exit: a wave first reads 64 keys spread over the whole input, and
-// checks its own keys again after its first step of rows; when either spans
-// 1024 or more values the final check cannot pass, so the wave records that
-// span in its block's min / max and skips its rows, and the call's stop word
-// (epoch-tagged: no clearing needed) is raised once, for the other waves,
-// which poll it once per step, and for the merge kernels.
+// Early exit: a wave first reads the 64 sampled keys, and checks its own keys
+// again after its first step of rows; when either spans rt or more values the
+// final check cannot pass, so the wave records that span in its block's min /
+// max and skips its rows, and the call's stop word (epoch-tagged: no clearing
+// needed) is raised once, for the other waves, which poll it once per step,
+// and for the merge kernels.
 #pragma once
 
 #include "blockscan.hpp"
@@ -102,7 +105,7 @@ __device__ __forceinline__ K wave_max_k(K v) {
 
 // Residues per lane-group table for a 64-key sample spanning `span` values
 // (2x margin): few distinct keys would make the lanes of a wave collide on
-// one entry (same-address LDS atomics serialise), so the 1024 entries are
+// one entry (same-address LDS atomics serialise), so the TE entries are
 // split into TE / rt tables and lane l uses table l % (TE / rt). The
 // final check then requires max - min < rt. Both kernels derive rt from the
 // same 64 keys, so every wave agrees without communicating.
