@@ -719,7 +719,8 @@ int tile_reduce_key(const SortCtx& c, const SegWs& w, int vt, const void* keys, 
 }
 
 // ---- one-pass path ------------------------------------------------------------
-// Integer keys, float32 or integer values, no min / max, at least 2^22 rows:
+// Integer keys, float32 or integer values (sum / count / mean / min / max), at
+// least 2^22 rows:
 // a miss costs about 30 us (the aborted table launch, two small kernels and a
 // read-back) before the statistics-first path, under 1% of a groupby of that
 // size. JFX_ONEPASS=0 turns the path off, =always tries it at any size.
@@ -728,29 +729,31 @@ int tile_reduce_key(const SortCtx& c, const SegWs& w, int vt, const void* keys, 
 std::atomic<uint32_t> g_op_epoch{0};
 constexpr uint64_t OP_MIN_ROWS = 1ull << 22;
 
-bool onepass_wanted(int kt, int vt, uint64_t n, const void* out_min, const void* out_max) {
-  if (!int_key(kt) || vt == JFX_F64 || out_min || out_max) return false;
+bool onepass_wanted(int kt, int vt, uint64_t n) {
+  if (!int_key(kt) || vt == JFX_F64) return false;
   const char* e = getenv("JFX_ONEPASS");
   if (e && strcmp(e, "0") == 0) return false;
   if (e && strcmp(e, "always") == 0) return true;
   return n >= OP_MIN_ROWS;
 }
 
-template <typename K, typename V, typename MT, uint32_t TE>
+template <typename K, typename V, typename MT, bool MM, uint32_t TE>
 int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const SegWs& w,
                 void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t rt_min,
                 uint32_t* res, hipStream_t s) {
   constexpr uint64_t STEP = 64u * WT_RPL;
-  // LDS per block: a sum table per wave and one count table
-  constexpr uint64_t BLK = (uint64_t)OP_W * TE * sizeof(typename WaAcc<V>::T) + TE * 4u;
-  OnePassArgs<K, V> a{};
+  // LDS per block: a sum table per wave, one count table and (MM) one min and
+  // one max table
+  constexpr uint64_t BLK = (uint64_t)OP_W * TE * sizeof(typename WaAcc<V>::T) + TE * 4u +
+                           (MM ? 2u * TE * sizeof(typename ValTraits<V>::U) : 0u);
+  OnePassArgs<K, V, MM> a{};
   a.keys = static_cast<const K*>(keys);
   a.vals = static_cast<const V*>(vals);
   a.n = n;
   a.oc = oc;
   a.epoch = ++g_op_epoch;
   a.rt_min = rt_min;
-  a.part = static_cast<OpPart<V>*>(w.part);
+  a.part = static_cast<OpPart<V, MM>*>(w.part);
   a.mm = w.op_mm;
   a.ctl = w.op_ctl;
   // as many blocks as fit in LDS at once, at most OP_GROUP x OP_L1 (the
@@ -764,26 +767,26 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   const bool vec = (uintptr_t)keys % 16 == 0 && (uintptr_t)vals % 16 == 0;
   const char* nm = TE == OP_TE ? "onepass_table" : "onepass_table_2k";
   if (vec) {
-    JFX_LAUNCH(nm, s, (k_onepass_table<K, V, true, TE><<<grid, OP_W * 64, 0, s>>>(a)));
+    JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, true, TE><<<grid, OP_W * 64, 0, s>>>(a)));
   } else {
-    JFX_LAUNCH(nm, s, (k_onepass_table<K, V, false, TE><<<grid, OP_W * 64, 0, s>>>(a)));
+    JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, false, TE><<<grid, OP_W * 64, 0, s>>>(a)));
   }
   JFX_HIP(hipGetLastError());
-  GroupOut<V, MT> o{static_cast<V*>(sc.out_sum), sc.out_count, nullptr, nullptr,
-                    static_cast<MT*>(sc.out_mean)};
+  GroupOut<V, MT> o{static_cast<V*>(sc.out_sum), sc.out_count, static_cast<V*>(sc.out_min),
+                    static_cast<V*>(sc.out_max), static_cast<MT*>(sc.out_mean)};
   // level-1 merges go to the part buffer after the block tables (it holds
   // SORT_MAXCHUNKS x 2048 min / max states, more than both need)
   const uint32_t ng1 = (grid + OP_GROUP - 1) / OP_GROUP;
   JFX_CHECK(ng1 <= OP_L1, JFX_ERR_UNSUPPORTED, "one-pass grid exceeds the final merge");
   const uint64_t part_bytes = (uint64_t)SORT_MAXCHUNKS * 2048 *
                               (sizeof(V) == 4 ? sizeof(AggState<float>) : sizeof(AggState<double>));
-  JFX_CHECK(((uint64_t)grid + ng1) * TE * sizeof(OpPart<V>) <= part_bytes,
+  JFX_CHECK(((uint64_t)grid + ng1) * TE * sizeof(OpPart<V, MM>) <= part_bytes,
             JFX_ERR_WORKSPACE, "one-pass partials exceed the partial buffer");
-  OpPart<V>* l1 = a.part + (size_t)grid * TE;
-  JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V, TE><<<ng1, 1024, 0, s>>>(a.part, grid, a.ctl,
-                                                                              a.epoch, l1)));
+  OpPart<V, MM>* l1 = a.part + (size_t)grid * TE;
+  JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V, MM, TE><<<ng1, 1024, 0, s>>>(
+                                      a.part, grid, a.ctl, a.epoch, l1)));
   JFX_HIP(hipGetLastError());
-  JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT, TE><<<1, 1024, 0, s>>>(
+  JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT, MM, TE><<<1, 1024, 0, s>>>(
                                       l1, ng1, a.mm, grid, a.ctl, a.epoch, a.keys, n, oc,
                                       rt_min, static_cast<K*>(out_keys), o, capacity,
                                       w.op_ctl + 2)));
@@ -791,14 +794,19 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   return read_back(s, w.op_ctl + 2, 2 * sizeof(uint32_t), res);
 }
 
-// te = 1024 or 2048 residues; rt_min = te turns the lane-group tables off.
+// te = 1024 or 2048 residues; rt_min = te turns the lane-group tables off;
+// min / max tables when a min or max output is asked for.
 template <typename K, typename V, typename MT>
 int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const SegWs& w,
                   void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t te,
                   uint32_t rt_min, uint32_t* res, hipStream_t s) {
-  if (te == 2 * OP_TE)
-    return onepass_run<K, V, MT, 2 * OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
-  return onepass_run<K, V, MT, OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
+  const bool mm = sc.out_min || sc.out_max;
+  if (te == 2 * OP_TE) {
+    if (mm) return onepass_run<K, V, MT, true, 2 * OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
+    return onepass_run<K, V, MT, false, 2 * OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
+  }
+  if (mm) return onepass_run<K, V, MT, true, OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
+  return onepass_run<K, V, MT, false, OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
 }
 
 template <typename K>
@@ -904,11 +912,11 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
   hipStream_t s = jfx::as_stream(stream);
   // a key span known to be at least 2048: the sort-based path will run
   bool wide = false;
-  if (jfx::onepass_wanted(key_type, val_type, n, out_min, out_max)) {
+  if (jfx::onepass_wanted(key_type, val_type, n)) {
     // one read of keys and values when the key range turns out below 1024;
     // on a miss the statistics-first path below runs
     jfx::SegCall oc{nullptr, jfx::order_code(key_type), vals, nullptr, n, out_keys, out_sum,
-                    out_count, nullptr, nullptr, out_mean, true, 0u};
+                    out_count, out_min, out_max, out_mean, true, 0u};
     uint32_t res[2] = {0u, 0u};
     auto attempt = [&](uint32_t te, uint32_t rt_min) -> int {
       return kb == 4 ? jfx::onepass_key<uint32_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc,

@@ -40,37 +40,109 @@ constexpr uint32_t OP_TE = 1024;  // table entries (residues) per wave; 2048 for
 constexpr int OP_W = 4;           // waves per block
 constexpr uint32_t OP_NO_RUN = 0xFFFFFFFFu;
 
-// One residue of a block table: the sum and the row count. float32 values
-// keep their double sum as a float pair s + c (c the rounding error of s, 0
-// when s is not finite; s + c converts back to the double exactly), 12 bytes;
-// integer values a wrapping unsigned sum.
-template <typename V> struct OpPart {
+// Order-preserving unsigned codes of a value for the min / max tables (LDS
+// integer atomics; float LDS atomics serialise their lanes). Float NaN codes
+// below every value in the min table and above every value in the max table,
+// so a NaN anywhere in a group makes its min and max NaN (numpy's minimum /
+// maximum reductions); -0.0 and +0.0 share one code (decoded as +0.0).
+template <typename V> struct MmCode {
+  using U = typename ValTraits<V>::U;
+  static __device__ __forceinline__ U lo(V x) {
+    if constexpr (std::is_same<V, float>::value) {
+      const uint32_t b = __float_as_uint(x);
+      return (b & 0x7FFFFFFFu) > 0x7F800000u ? 0u : ordered(b, ORD_FLOAT);
+    } else if constexpr (std::is_signed<V>::value) {
+      return (U)x ^ ((U)1 << (sizeof(U) * 8 - 1));
+    } else {
+      return (U)x;
+    }
+  }
+  static __device__ __forceinline__ U hi(V x) {
+    if constexpr (std::is_same<V, float>::value) {
+      const uint32_t b = __float_as_uint(x);
+      return (b & 0x7FFFFFFFu) > 0x7F800000u ? 0xFFFFFFFFu : ordered(b, ORD_FLOAT);
+    } else {
+      return lo(x);
+    }
+  }
+  static __device__ __forceinline__ V decode(U u) {
+    if constexpr (std::is_same<V, float>::value) {
+      if (u == 0u || u == 0xFFFFFFFFu) return __uint_as_float(0x7FC00000u);
+      return __uint_as_float((u & 0x80000000u) ? (u ^ 0x80000000u) : ~u);
+    } else if constexpr (std::is_signed<V>::value) {
+      return (V)(u ^ ((U)1 << (sizeof(U) * 8 - 1)));
+    } else {
+      return (V)u;
+    }
+  }
+};
+
+// One residue of a block table: the sum, the row count and (MM) the min /
+// max codes. float32 values keep their double sum as a float pair s + c (c
+// the rounding error of s, 0 when s is not finite; s + c converts back to the
+// double exactly); integer values a wrapping unsigned sum.
+// (Plain structs: with a base struct the merge loop's 16 loads were no
+// longer kept in flight together.)
+template <typename V, bool MM> struct OpPart {
   typename ValTraits<V>::U s;
   uint32_t n;
 };
-template <> struct OpPart<float> {
+template <typename V> struct OpPart<V, true> {
+  typename ValTraits<V>::U s;
+  uint32_t n;
+  typename ValTraits<V>::U mn, mx;
+};
+template <> struct OpPart<float, false> {
   float s, c;
   uint32_t n;
 };
-template <typename V>
-__device__ __forceinline__ OpPart<V> op_pack(typename WaAcc<V>::T acc, uint32_t n) {
-  OpPart<V> p;
-  if constexpr (std::is_same<V, float>::value) {
-    p.s = (float)acc;
-    p.c = __builtin_isfinite(p.s) ? (float)(acc - (double)p.s) : 0.0f;
-  } else {
-    p.s = acc;
-  }
-  p.n = n;
-  return p;
-}
-template <typename V>
-__device__ __forceinline__ typename WaAcc<V>::T op_sum(const OpPart<V>& p) {
-  if constexpr (std::is_same<V, float>::value) return (double)p.s + (double)p.c;
-  else return p.s;
-}
+template <> struct OpPart<float, true> {
+  float s, c;
+  uint32_t n;
+  uint32_t mn, mx;
+};
 
-template <typename K, typename V>
+// Running state of one residue: the sum in the accumulator type, the count
+// and (MM) the min / max codes.
+template <typename V, bool MM> struct OpAcc {
+  using T = typename WaAcc<V>::T;
+  using U = typename ValTraits<V>::U;
+  T s;
+  uint32_t n;
+  U mn, mx;
+  __device__ __forceinline__ void init() {
+    s = T(0);
+    n = 0u;
+    mn = ~U(0);
+    mx = U(0);
+  }
+  __device__ __forceinline__ void add(const OpPart<V, MM>& p) {
+    if constexpr (std::is_same<V, float>::value) s += (double)p.s + (double)p.c;
+    else s += p.s;
+    n += p.n;
+    if constexpr (MM) {
+      mn = p.mn < mn ? p.mn : mn;
+      mx = p.mx > mx ? p.mx : mx;
+    }
+  }
+  __device__ __forceinline__ OpPart<V, MM> pack() const {
+    OpPart<V, MM> p;
+    if constexpr (std::is_same<V, float>::value) {
+      p.s = (float)s;
+      p.c = __builtin_isfinite(p.s) ? (float)(s - (double)p.s) : 0.0f;
+    } else {
+      p.s = s;
+    }
+    p.n = n;
+    if constexpr (MM) {
+      p.mn = mn;
+      p.mx = mx;
+    }
+    return p;
+  }
+};
+
+template <typename K, typename V, bool MM>
 struct OnePassArgs {
   const K* keys;             // raw key bits (integer keys)
   const V* vals;
@@ -79,7 +151,7 @@ struct OnePassArgs {
   int oc;                    // key order code
   uint32_t epoch;            // this call's tag for the stop word
   uint32_t rt_min;           // residues per lane-group table at least (TE: no lane groups)
-  OpPart<V>* part;           // [gridDim.x][TE] block tables
+  OpPart<V, MM>* part;       // [gridDim.x][TE] block tables
   uint64_t* mm;              // [gridDim.x][2] ordered min and max of the block's keys
   uint32_t* ctl;             // [0] stop word
 };
@@ -121,16 +193,26 @@ __device__ __forceinline__ K op_sample_span(const K* keys, uint64_t n, int oc, u
   return wave_max_k(o) - wave_min_k(o);
 }
 
-template <typename K, typename V, bool VEC, uint32_t TE>
-__global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a) {
+template <typename K, typename V, bool MM, bool VEC, uint32_t TE>
+__global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, MM> a) {
   using T = typename WaAcc<V>::T;
+  using U = typename ValTraits<V>::U;
+  using C = MmCode<V>;
   constexpr uint64_t STEP = 64u * WT_RPL;
   __shared__ T s_acc[OP_W * TE];
   __shared__ uint32_t s_cnt[TE];
+  // min / max codes: order-independent like the counts, one table per block
+  __shared__ U s_min[MM ? TE : 1], s_max[MM ? TE : 1];
   __shared__ K s_mn[OP_W], s_mx[OP_W];
   __shared__ uint32_t s_stop;
   if (threadIdx.x == 0) s_stop = 0u;
-  for (uint32_t i = threadIdx.x; i < TE; i += OP_W * 64) s_cnt[i] = 0u;
+  for (uint32_t i = threadIdx.x; i < TE; i += OP_W * 64) {
+    s_cnt[i] = 0u;
+    if constexpr (MM) {
+      s_min[i] = ~U(0);
+      s_max[i] = U(0);
+    }
+  }
 
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lane = lane_id();
@@ -155,6 +237,8 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   const uint32_t gsw = lane & (TE / rt - 1u);
   T* accs = s_acc + wid * TE + gsw * rt;
   uint32_t* cnts = s_cnt + gsw * rt;
+  U* mins = s_min + (MM ? gsw * rt : 0u);
+  U* maxs = s_max + (MM ? gsw * rt : 0u);
   // the wave's keys so far span rt or more values: the final check fails
   auto too_wide = [&]() -> bool {
     const K lo = wave_min_k(omin), hi = wave_max_k(omax);
@@ -182,19 +266,29 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   uint32_t run_r = OP_NO_RUN;
   T run = T(0);
   uint32_t run_n = 0u;
-  auto flush = [&](uint32_t r, T v, uint32_t c, bool pend) {
+  U run_lo = ~U(0), run_hi = U(0);
+  auto flush = [&](uint32_t r, T v, uint32_t c, U lo, U hi, bool pend) {
     if (pend) {
       __hip_atomic_fetch_add(&accs[r], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
       __hip_atomic_fetch_add(&cnts[r], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if constexpr (MM) {
+        __hip_atomic_fetch_min(&mins[r], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_max(&maxs[r], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
     }
   };
   auto row = [&](K key, V x) {
     see(key);
     const uint32_t r = ((uint32_t)key ^ gsw) & M;
-    flush(run_r, run, run_n, r != run_r && run_r != OP_NO_RUN);
+    flush(run_r, run, run_n, run_lo, run_hi, r != run_r && run_r != OP_NO_RUN);
     const bool same = r == run_r;
     run = same ? run + (T)x : (T)x;
     run_n = same ? run_n + 1u : 1u;
+    if constexpr (MM) {
+      const U lo = C::lo(x), hi = C::hi(x);
+      run_lo = same && run_lo < lo ? run_lo : lo;
+      run_hi = same && run_hi > hi ? run_hi : hi;
+    }
     run_r = r;
   };
 
@@ -245,18 +339,18 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   }
   if (!stop) {
     // the lane's last run, then rows [base, wend) one row per lane
-    flush(run_r, run, run_n, run_r != OP_NO_RUN);
+    flush(run_r, run, run_n, run_lo, run_hi, run_r != OP_NO_RUN);
     for (uint64_t r0 = base; r0 < wend; r0 += 64) {
       const uint64_t r = r0 + lane;
       const bool valid = r < wend;
       K key = K(0);
-      T v = T(0);
+      V x = V(0);
       if (valid) {
         key = a.keys[r];
-        v = (T)a.vals[r];
+        x = a.vals[r];
         see(key);
       }
-      flush(((uint32_t)key ^ gsw) & M, v, 1u, valid);
+      flush(((uint32_t)key ^ gsw) & M, (T)x, 1u, C::lo(x), C::hi(x), valid);
     }
   }
 
@@ -281,14 +375,21 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V> a
   // entries from rt on are unused
   const uint32_t lg = TE / rt;
   for (uint32_t r = threadIdx.x; r < TE; r += OP_W * 64) {
-    T acc = T(0);
-    uint32_t c = 0u;
+    OpAcc<V, MM> st;
+    st.init();
     if (r < rt) {
       for (int w = 0; w < OP_W; ++w)
-        for (uint32_t g = 0; g < lg; ++g) acc += s_acc[w * TE + g * rt + ((r ^ g) & (rt - 1u))];
-      for (uint32_t g = 0; g < lg; ++g) c += s_cnt[g * rt + ((r ^ g) & (rt - 1u))];
+        for (uint32_t g = 0; g < lg; ++g) st.s += s_acc[w * TE + g * rt + ((r ^ g) & (rt - 1u))];
+      for (uint32_t g = 0; g < lg; ++g) {
+        const uint32_t e = g * rt + ((r ^ g) & (rt - 1u));
+        st.n += s_cnt[e];
+        if constexpr (MM) {
+          st.mn = s_min[e] < st.mn ? s_min[e] : st.mn;
+          st.mx = s_max[e] > st.mx ? s_max[e] : st.mx;
+        }
+      }
     }
-    a.part[(size_t)blockIdx.x * TE + r] = op_pack<V>(acc, c);
+    a.part[(size_t)blockIdx.x * TE + r] = st.pack();
   }
 }
 
@@ -309,50 +410,52 @@ constexpr uint32_t OP_L1 = 32;  // up to 1024 block tables
 // acc, n := acc, n plus p[0], p[stride], ... (count entries, added in order
 // in the accumulator type), 16 loads in flight at a time (more would spill
 // at 1024 threads).
-template <typename V>
-__device__ __forceinline__ void merge_seq(typename WaAcc<V>::T& acc, uint32_t& n,
-                                          const OpPart<V>* p, size_t stride, uint32_t count) {
+template <typename V, bool MM>
+__device__ __forceinline__ void merge_seq(OpAcc<V, MM>& st, const OpPart<V, MM>* p, size_t stride,
+                                          uint32_t count) {
   constexpr uint32_t G = 16;
+  OpAcc<V, MM> x = st;  // a local copy: accumulating through the reference kept
+                        // the compiler from leaving the 16 loads in flight
   for (uint32_t b = 0; b < count; b += G, p += G * stride) {
     const uint32_t m = (count - b) < G ? (count - b) : G;
-    OpPart<V> v[G];
+    OpPart<V, MM> v[G];
 #pragma unroll
     for (uint32_t i = 0; i < G; ++i)
       if (i < m) v[i] = p[i * stride];
 #pragma unroll
     for (uint32_t i = 0; i < G; ++i)
-      if (i < m) {
-        acc += op_sum<V>(v[i]);
-        n += v[i].n;
-      }
+      if (i < m) x.add(v[i]);
   }
+  st = x;
 }
 
-template <typename V, uint32_t TE>
+template <typename V, bool MM, uint32_t TE>
 __global__ void __launch_bounds__(1024)
-k_onepass_merge(const OpPart<V>* part, uint32_t nblocks, const uint32_t* ctl, uint32_t epoch,
-                OpPart<V>* l1) {
+k_onepass_merge(const OpPart<V, MM>* part, uint32_t nblocks, const uint32_t* ctl, uint32_t epoch,
+                OpPart<V, MM>* l1) {
   if (ctl[0] == epoch) return;  // stopped: the result is a miss
   const uint32_t c0 = blockIdx.x * OP_GROUP;
   const uint32_t c1 = (c0 + OP_GROUP) < nblocks ? (c0 + OP_GROUP) : nblocks;
   for (uint32_t r = threadIdx.x; r < TE; r += 1024) {
-    typename WaAcc<V>::T acc(0);
-    uint32_t n = 0u;
-    merge_seq<V>(acc, n, part + (size_t)c0 * TE + r, TE, c1 - c0);
-    l1[(size_t)blockIdx.x * TE + r] = op_pack<V>(acc, n);
+    OpAcc<V, MM> st;
+    st.init();
+    merge_seq<V, MM>(st, part + (size_t)c0 * TE + r, TE, c1 - c0);
+    l1[(size_t)blockIdx.x * TE + r] = st.pack();
   }
 }
 
-template <typename K, typename V, typename MT, uint32_t TE>
+template <typename K, typename V, typename MT, bool MM, uint32_t TE>
 __global__ void __launch_bounds__(1024)
-k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
+k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
                 const uint32_t* ctl, uint32_t epoch, const K* keys, uint64_t n, int oc,
                 uint32_t rt_min, K* out_keys, GroupOut<V, MT> out, uint64_t capacity,
                 uint32_t* res) {
   using T = typename WaAcc<V>::T;
+  using U = typename ValTraits<V>::U;
   __shared__ K s_lo[16], s_hi[16];
   __shared__ T s_sum[TE];
   __shared__ uint32_t s_n[TE];
+  __shared__ U s_min[MM ? TE : 1], s_max[MM ? TE : 1];
   __shared__ uint32_t s_gid[TE];
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
   __shared__ uint32_t s_rt;
@@ -375,11 +478,15 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
   }
   if (!stopped) {
     for (uint32_t r = tid; r < TE; r += 1024) {
-      T acc(0);
-      uint32_t n = 0u;
-      merge_seq<V>(acc, n, l1 + r, TE, nl1);
-      s_sum[r] = acc;
-      s_n[r] = n;
+      OpAcc<V, MM> st;
+      st.init();
+      merge_seq<V, MM>(st, l1 + r, TE, nl1);
+      s_sum[r] = st.s;
+      s_n[r] = st.n;
+      if constexpr (MM) {
+        s_min[r] = st.mn;
+        s_max[r] = st.mx;
+      }
     }
   }
 
@@ -428,13 +535,18 @@ k_onepass_final(const OpPart<V>* l1, uint32_t nl1, const uint64_t* mm, uint32_t 
     const uint32_t cnt = count_of(d);
     if (cnt == 0u) continue;
     const K o = lo + (K)d;
-    const T sum = s_sum[(uint32_t)o & (rt - 1u)];
+    const uint32_t r = (uint32_t)o & (rt - 1u);
+    const T sum = s_sum[r];
     const uint32_t g = s_gid[d];
     K raw = o;
     if (oc == ORD_SIGNED) raw ^= (K)1 << (sizeof(K) * 8 - 1);
     if (out_keys) out_keys[g] = raw;
     if (out.sum) out.sum[g] = (V)sum;
     if (out.count) out.count[g] = (int64_t)cnt;
+    if constexpr (MM) {
+      if (out.mn) out.mn[g] = MmCode<V>::decode(s_min[r]);
+      if (out.mx) out.mx[g] = MmCode<V>::decode(s_max[r]);
+    }
     if (out.mean) {
       if constexpr (ValTraits<V>::is_float) out.mean[g] = (MT)(sum / (double)cnt);
       else out.mean[g] = (MT)((double)(V)sum / (double)cnt);

@@ -285,3 +285,41 @@ def test_onepass_frequent_key(hot, R):
         _check(keys, vals, uk, res["v"])
     uk, res = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": "count"})
     assert np.array_equal(to_numpy(res["v"]), np.unique(keys, return_counts=True)[1])
+
+
+@pytest.mark.parametrize("vdt", [np.float32, np.int32, np.int64, np.uint32, np.uint64])
+@pytest.mark.parametrize("R", [3, 1000, 1500])
+def test_onepass_min_max(vdt, R):
+    """Min / max in the one-pass tables (order-preserving integer codes,
+    float NaN winning both): bit-exact against numpy's reductions, with the
+    other aggregations of groupby_stats; lane-group tables (R = 3) and the
+    2048-residue retry (R = 1500) included."""
+    rng = np.random.default_rng(R * 11 + np.dtype(vdt).itemsize)
+    keys = (rng.integers(0, R, N) - 40).astype(np.int32)
+    if np.dtype(vdt).kind == "f":
+        vals = (rng.standard_normal(N) * 100).astype(vdt)
+        vals[rng.integers(0, N, 40)] = np.inf
+        vals[rng.integers(0, N, 40)] = -np.inf
+        vals[rng.integers(0, N, 40)] = -0.0
+        vals[np.flatnonzero(keys == keys[7])[:3]] = np.nan    # one group with NaN
+    else:
+        info = np.iinfo(vdt)
+        vals = rng.integers(info.min, info.max, N, dtype=vdt, endpoint=True)
+        vals[:2] = [info.min, info.max]
+    with _lib.KernelTimer() as kt:
+        uk, res = grouping.groupby_stats(cuda(keys), cuda(vals))
+        torch.cuda.synchronize()
+    ran = kt.stats()
+    assert any(k.startswith("onepass_table") for k in ran) and \
+        not any(k.startswith("sort_stats") for k in ran), ran
+    sk, sv, starts = _groups(keys, vals)
+    assert np.array_equal(to_numpy(uk), sk[starts])
+    assert np.array_equal(to_numpy(res["count"]), np.diff(np.append(starts, N)))
+    assert np.array_equal(to_numpy(res["min"]), np.minimum.reduceat(sv, starts), equal_nan=True)
+    assert np.array_equal(to_numpy(res["max"]), np.maximum.reduceat(sv, starts), equal_nan=True)
+    if np.dtype(vdt).kind != "f":
+        assert np.array_equal(to_numpy(res["sum"]), _wrapping_sums(sv, starts))
+    for fn, ref in (("min", np.minimum), ("max", np.maximum)):
+        uk1, r1 = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": fn})
+        assert np.array_equal(to_numpy(uk1), sk[starts])
+        assert np.array_equal(to_numpy(r1["v"]), ref.reduceat(sv, starts), equal_nan=True)
