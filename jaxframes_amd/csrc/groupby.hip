@@ -730,7 +730,8 @@ std::atomic<uint32_t> g_op_epoch{0};
 constexpr uint64_t OP_MIN_ROWS = 1ull << 22;
 
 bool onepass_wanted(int kt, int vt, uint64_t n) {
-  if (!int_key(kt) || vt == JFX_F64) return false;
+  (void)vt;
+  if (!int_key(kt)) return false;
   const char* e = getenv("JFX_ONEPASS");
   if (e && strcmp(e, "0") == 0) return false;
   if (e && strcmp(e, "always") == 0) return true;
@@ -744,8 +745,8 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   constexpr uint64_t STEP = 64u * WT_RPL;
   // LDS per block: a sum table per wave, one count table and (MM) one min and
   // one max table
-  constexpr uint64_t BLK = (uint64_t)OP_W * TE * sizeof(typename WaAcc<V>::T) + TE * 4u +
-                           (MM ? 2u * TE * sizeof(typename ValTraits<V>::U) : 0u);
+  constexpr uint64_t BLK = (uint64_t)OP_W * TE * sizeof(typename OpT<V>::T) * (op_dd<V> ? 2u : 1u) +
+                           TE * 4u + (MM ? 2u * TE * sizeof(typename ValTraits<V>::U) : 0u);
   OnePassArgs<K, V, MM> a{};
   a.keys = static_cast<const K*>(keys);
   a.vals = static_cast<const V*>(vals);
@@ -802,7 +803,12 @@ int onepass_typed(int oc, const void* keys, const void* vals, uint64_t n, const 
                   uint32_t rt_min, uint32_t* res, hipStream_t s) {
   const bool mm = sc.out_min || sc.out_max;
   if (te == 2 * OP_TE) {
-    if (mm) return onepass_run<K, V, MT, true, 2 * OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
+    if (mm) {
+      // float64 sums with compensation and 64-bit min / max codes at 2048
+      // residues exceed the 160 KB of LDS: no retry (res keeps the miss)
+      if constexpr (op_dd<V>) return JFX_OK;
+      else return onepass_run<K, V, MT, true, 2 * OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
+    }
     return onepass_run<K, V, MT, false, 2 * OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
   }
   if (mm) return onepass_run<K, V, MT, true, OP_TE>(oc, keys, vals, n, w, out_keys, sc, capacity, rt_min, res, s);
@@ -815,6 +821,7 @@ int onepass_key(int oc, int vt, const void* keys, const void* vals, uint64_t n, 
                 uint32_t rt_min, uint32_t* res, hipStream_t s) {
   switch (vt) {
     case JFX_F32: return onepass_typed<K, float, float>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
+    case JFX_F64: return onepass_typed<K, double, double>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
     case JFX_I32: return onepass_typed<K, int32_t, float>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
     case JFX_I64: return onepass_typed<K, int64_t, double>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);
     case JFX_U32: return onepass_typed<K, uint32_t, float>(oc, keys, vals, n, w, out_keys, sc, capacity, te, rt_min, res, s);

@@ -40,6 +40,18 @@ constexpr uint32_t OP_TE = 1024;  // table entries (residues) per wave; 2048 for
 constexpr int OP_W = 4;           // waves per block
 constexpr uint32_t OP_NO_RUN = 0xFFFFFFFFu;
 
+// Accumulator of a residue's sum: double for float32 and float64 values
+// (float64 with a compensation term, see k_onepass_table), the value type's
+// unsigned integer for integer values (wrapping sums).
+template <typename V> struct OpT {
+  using T = typename WaAcc<V>::T;
+};
+template <> struct OpT<double> {
+  using T = double;
+};
+template <typename V>
+constexpr bool op_dd = std::is_same<V, double>::value;
+
 // Order-preserving unsigned codes of a value for the min / max tables (LDS
 // integer atomics; float LDS atomics serialise their lanes). Float NaN codes
 // below every value in the min table and above every value in the max table,
@@ -51,6 +63,9 @@ template <typename V> struct MmCode {
     if constexpr (std::is_same<V, float>::value) {
       const uint32_t b = __float_as_uint(x);
       return (b & 0x7FFFFFFFu) > 0x7F800000u ? 0u : ordered(b, ORD_FLOAT);
+    } else if constexpr (std::is_same<V, double>::value) {
+      const uint64_t b = (uint64_t)__double_as_longlong(x);
+      return (b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull ? 0ull : ordered(b, ORD_FLOAT);
     } else if constexpr (std::is_signed<V>::value) {
       return (U)x ^ ((U)1 << (sizeof(U) * 8 - 1));
     } else {
@@ -61,6 +76,9 @@ template <typename V> struct MmCode {
     if constexpr (std::is_same<V, float>::value) {
       const uint32_t b = __float_as_uint(x);
       return (b & 0x7FFFFFFFu) > 0x7F800000u ? 0xFFFFFFFFu : ordered(b, ORD_FLOAT);
+    } else if constexpr (std::is_same<V, double>::value) {
+      const uint64_t b = (uint64_t)__double_as_longlong(x);
+      return (b & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull ? ~0ull : ordered(b, ORD_FLOAT);
     } else {
       return lo(x);
     }
@@ -69,6 +87,10 @@ template <typename V> struct MmCode {
     if constexpr (std::is_same<V, float>::value) {
       if (u == 0u || u == 0xFFFFFFFFu) return __uint_as_float(0x7FC00000u);
       return __uint_as_float((u & 0x80000000u) ? (u ^ 0x80000000u) : ~u);
+    } else if constexpr (std::is_same<V, double>::value) {
+      if (u == 0ull || u == ~0ull) return __longlong_as_double(0x7FF8000000000000ll);
+      const uint64_t sgn = 0x8000000000000000ull;
+      return __longlong_as_double((long long)((u & sgn) ? (u ^ sgn) : ~u));
     } else if constexpr (std::is_signed<V>::value) {
       return (V)(u ^ ((U)1 << (sizeof(U) * 8 - 1)));
     } else {
@@ -101,23 +123,43 @@ template <> struct OpPart<float, true> {
   uint32_t n;
   uint32_t mn, mx;
 };
+// float64: the compensated pair s + c itself
+template <> struct OpPart<double, false> {
+  double s, c;
+  uint32_t n;
+};
+template <> struct OpPart<double, true> {
+  double s, c;
+  uint32_t n;
+  uint64_t mn, mx;
+};
 
 // Running state of one residue: the sum in the accumulator type, the count
 // and (MM) the min / max codes.
 template <typename V, bool MM> struct OpAcc {
-  using T = typename WaAcc<V>::T;
+  using T = typename OpT<V>::T;
   using U = typename ValTraits<V>::U;
   T s;
+  T c;  // float64 values: compensation of s (TwoSum errors)
   uint32_t n;
   U mn, mx;
   __device__ __forceinline__ void init() {
     s = T(0);
+    c = T(0);
     n = 0u;
     mn = ~U(0);
     mx = U(0);
   }
+  // s, c := s, c then (ps, pc)
+  __device__ __forceinline__ void add_pair(T ps, T pc) {
+    T e;
+    const T t = two_sum(s, ps, e);
+    c = c + pc + e;
+    s = t;
+  }
   __device__ __forceinline__ void add(const OpPart<V, MM>& p) {
     if constexpr (std::is_same<V, float>::value) s += (double)p.s + (double)p.c;
+    else if constexpr (op_dd<V>) add_pair(p.s, p.c);
     else s += p.s;
     n += p.n;
     if constexpr (MM) {
@@ -130,6 +172,9 @@ template <typename V, bool MM> struct OpAcc {
     if constexpr (std::is_same<V, float>::value) {
       p.s = (float)s;
       p.c = __builtin_isfinite(p.s) ? (float)(s - (double)p.s) : 0.0f;
+    } else if constexpr (op_dd<V>) {
+      p.s = s;
+      p.c = c;
     } else {
       p.s = s;
     }
@@ -195,11 +240,14 @@ __device__ __forceinline__ K op_sample_span(const K* keys, uint64_t n, int oc, u
 
 template <typename K, typename V, bool MM, bool VEC, uint32_t TE>
 __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, MM> a) {
-  using T = typename WaAcc<V>::T;
+  using T = typename OpT<V>::T;
   using U = typename ValTraits<V>::U;
   using C = MmCode<V>;
+  constexpr bool DD = op_dd<V>;
   constexpr uint64_t STEP = 64u * WT_RPL;
   __shared__ T s_acc[OP_W * TE];
+  // float64 values: wave-private compensation of each sum entry
+  __shared__ T s_comp[DD ? OP_W * TE : 1];
   __shared__ uint32_t s_cnt[TE];
   // min / max codes: order-independent like the counts, one table per block
   __shared__ U s_min[MM ? TE : 1], s_max[MM ? TE : 1];
@@ -216,7 +264,10 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
 
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t lane = lane_id();
-  for (uint32_t i = lane; i < TE; i += 64) s_acc[wid * TE + i] = T(0);
+  for (uint32_t i = lane; i < TE; i += 64) {
+    s_acc[wid * TE + i] = T(0);
+    if constexpr (DD) s_comp[wid * TE + i] = T(0);
+  }
   __syncthreads();
 
   K omin = ~K(0), omax = K(0);
@@ -236,6 +287,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
   // wave on a few LDS banks
   const uint32_t gsw = lane & (TE / rt - 1u);
   T* accs = s_acc + wid * TE + gsw * rt;
+  T* comps = s_comp + (DD ? wid * TE + gsw * rt : 0u);
   uint32_t* cnts = s_cnt + gsw * rt;
   U* mins = s_min + (MM ? gsw * rt : 0u);
   U* maxs = s_max + (MM ? gsw * rt : 0u);
@@ -265,11 +317,23 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
 
   uint32_t run_r = OP_NO_RUN;
   T run = T(0);
+  T run_c = T(0);  // float64: the run's compensation
   uint32_t run_n = 0u;
   U run_lo = ~U(0), run_hi = U(0);
-  auto flush = [&](uint32_t r, T v, uint32_t c, U lo, U hi, bool pend) {
+  // float64 values add the run with ds_add_rtn_f64, which returns the sum it
+  // was added to, so TwoSum(old, v) is the exact rounding error of that add;
+  // it goes, with the run's own compensation vc, into the compensation entry
+  auto flush = [&](uint32_t r, T v, T vc, uint32_t c, U lo, U hi, bool pend) {
     if (pend) {
-      __hip_atomic_fetch_add(&accs[r], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      if constexpr (DD) {
+        const T old = __hip_atomic_fetch_add(&accs[r], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        T e;
+        (void)two_sum(old, v, e);
+        __hip_atomic_fetch_add(&comps[r], vc + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      } else {
+        (void)vc;
+        __hip_atomic_fetch_add(&accs[r], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+      }
       __hip_atomic_fetch_add(&cnts[r], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       if constexpr (MM) {
         __hip_atomic_fetch_min(&mins[r], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -280,9 +344,16 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
   auto row = [&](K key, V x) {
     see(key);
     const uint32_t r = ((uint32_t)key ^ gsw) & M;
-    flush(run_r, run, run_n, run_lo, run_hi, r != run_r && run_r != OP_NO_RUN);
+    flush(run_r, run, run_c, run_n, run_lo, run_hi, r != run_r && run_r != OP_NO_RUN);
     const bool same = r == run_r;
-    run = same ? run + (T)x : (T)x;
+    if constexpr (DD) {
+      T e;
+      const T t = two_sum(run, (T)x, e);
+      run = same ? t : (T)x;
+      run_c = same ? run_c + e : T(0);
+    } else {
+      run = same ? run + (T)x : (T)x;
+    }
     run_n = same ? run_n + 1u : 1u;
     if constexpr (MM) {
       const U lo = C::lo(x), hi = C::hi(x);
@@ -339,7 +410,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
   }
   if (!stop) {
     // the lane's last run, then rows [base, wend) one row per lane
-    flush(run_r, run, run_n, run_lo, run_hi, run_r != OP_NO_RUN);
+    flush(run_r, run, run_c, run_n, run_lo, run_hi, run_r != OP_NO_RUN);
     for (uint64_t r0 = base; r0 < wend; r0 += 64) {
       const uint64_t r = r0 + lane;
       const bool valid = r < wend;
@@ -350,7 +421,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
         x = a.vals[r];
         see(key);
       }
-      flush(((uint32_t)key ^ gsw) & M, (T)x, 1u, C::lo(x), C::hi(x), valid);
+      flush(((uint32_t)key ^ gsw) & M, (T)x, T(0), 1u, C::lo(x), C::hi(x), valid);
     }
   }
 
@@ -379,7 +450,11 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
     st.init();
     if (r < rt) {
       for (int w = 0; w < OP_W; ++w)
-        for (uint32_t g = 0; g < lg; ++g) st.s += s_acc[w * TE + g * rt + ((r ^ g) & (rt - 1u))];
+        for (uint32_t g = 0; g < lg; ++g) {
+          const uint32_t e = w * TE + g * rt + ((r ^ g) & (rt - 1u));
+          if constexpr (DD) st.add_pair(s_acc[e], s_comp[e]);
+          else st.s += s_acc[e];
+        }
       for (uint32_t g = 0; g < lg; ++g) {
         const uint32_t e = g * rt + ((r ^ g) & (rt - 1u));
         st.n += s_cnt[e];
@@ -450,10 +525,12 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
                 const uint32_t* ctl, uint32_t epoch, const K* keys, uint64_t n, int oc,
                 uint32_t rt_min, K* out_keys, GroupOut<V, MT> out, uint64_t capacity,
                 uint32_t* res) {
-  using T = typename WaAcc<V>::T;
+  using T = typename OpT<V>::T;
   using U = typename ValTraits<V>::U;
   __shared__ K s_lo[16], s_hi[16];
+  constexpr bool DD = op_dd<V>;
   __shared__ T s_sum[TE];
+  __shared__ T s_c[DD ? TE : 1];
   __shared__ uint32_t s_n[TE];
   __shared__ U s_min[MM ? TE : 1], s_max[MM ? TE : 1];
   __shared__ uint32_t s_gid[TE];
@@ -482,6 +559,7 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
       st.init();
       merge_seq<V, MM>(st, l1 + r, TE, nl1);
       s_sum[r] = st.s;
+      if constexpr (DD) s_c[r] = st.c;
       s_n[r] = st.n;
       if constexpr (MM) {
         s_min[r] = st.mn;
@@ -536,7 +614,8 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
     if (cnt == 0u) continue;
     const K o = lo + (K)d;
     const uint32_t r = (uint32_t)o & (rt - 1u);
-    const T sum = s_sum[r];
+    T sum = s_sum[r];
+    if constexpr (DD) sum = sum + s_c[r];
     const uint32_t g = s_gid[d];
     K raw = o;
     if (oc == ORD_SIGNED) raw ^= (K)1 << (sizeof(K) * 8 - 1);

@@ -6,6 +6,7 @@ value dtype), float32 sums within the FP64-oracle bound of the groupby tests.
 """
 
 import ctypes
+import math
 import zlib
 
 import numpy as np
@@ -42,7 +43,16 @@ def _check(keys, vals, uk, res_sum, res_count=None):
     if res_count is not None:
         assert np.array_equal(to_numpy(res_count), np.diff(np.append(starts, len(sk))))
     got = to_numpy(res_sum)
-    if np.dtype(vals.dtype).kind == "f":
+    if vals.dtype == np.float64:
+        # compensated float64 sums against correctly rounded ones (math.fsum):
+        # numpy's own float64 reduceat is the less accurate side here
+        ends = np.append(starts[1:], len(sv))
+        exact = np.array([math.fsum(sv[a:b]) for a, b in zip(starts, ends)])
+        absx = np.add.reduceat(np.abs(sv), starts)
+        fin = np.isfinite(exact)
+        assert np.array_equal(got[~fin], exact[~fin], equal_nan=True)
+        assert (np.abs(got[fin] - exact[fin]) <= 1e-15 * absx[fin]).all()
+    elif np.dtype(vals.dtype).kind == "f":
         _check_float_sums(got, sv, starts, vals.dtype)
     else:
         assert np.array_equal(got, _wrapping_sums(sv, starts))
@@ -68,7 +78,7 @@ KEY_CASES = [
 
 
 @pytest.mark.parametrize("name,kdt,base", KEY_CASES, ids=[c[0] for c in KEY_CASES])
-@pytest.mark.parametrize("vdt", [np.float32, np.int32, np.int64, np.uint64])
+@pytest.mark.parametrize("vdt", [np.float32, np.float64, np.int32, np.int64, np.uint64])
 def test_onepass_hit(name, kdt, base, vdt):
     rng = np.random.default_rng(zlib.crc32(f"{name}/{np.dtype(vdt).name}".encode()))
     off = rng.integers(0, 1000, N)   # 1000 distinct keys from base on
@@ -287,7 +297,7 @@ def test_onepass_frequent_key(hot, R):
     assert np.array_equal(to_numpy(res["v"]), np.unique(keys, return_counts=True)[1])
 
 
-@pytest.mark.parametrize("vdt", [np.float32, np.int32, np.int64, np.uint32, np.uint64])
+@pytest.mark.parametrize("vdt", [np.float32, np.float64, np.int32, np.int64, np.uint32, np.uint64])
 @pytest.mark.parametrize("R", [3, 1000, 1500])
 def test_onepass_min_max(vdt, R):
     """Min / max in the one-pass tables (order-preserving integer codes,
@@ -310,8 +320,11 @@ def test_onepass_min_max(vdt, R):
         uk, res = grouping.groupby_stats(cuda(keys), cuda(vals))
         torch.cuda.synchronize()
     ran = kt.stats()
-    assert any(k.startswith("onepass_table") for k in ran) and \
-        not any(k.startswith("sort_stats") for k in ran), ran
+    # float64 with min / max has no 2048-residue tables (LDS): R = 1500 then
+    # runs the statistics-first path after the miss
+    stats_first = vdt == np.float64 and R == 1500
+    assert any(k.startswith("onepass_table") for k in ran), ran
+    assert any(k.startswith("sort_stats") for k in ran) == stats_first, ran
     sk, sv, starts = _groups(keys, vals)
     assert np.array_equal(to_numpy(uk), sk[starts])
     assert np.array_equal(to_numpy(res["count"]), np.diff(np.append(starts, N)))
@@ -323,3 +336,23 @@ def test_onepass_min_max(vdt, R):
         uk1, r1 = grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)}, {"v": fn})
         assert np.array_equal(to_numpy(uk1), sk[starts])
         assert np.array_equal(to_numpy(r1["v"]), ref.reduceat(sv, starts), equal_nan=True)
+
+
+@pytest.mark.parametrize("R", [2, 1000, 1700])
+def test_onepass_float64_sums(R):
+    """float64 values: sums compensated through ds_add_rtn_f64 (TwoSum of the
+    returned entry), against math.fsum; large magnitudes with cancellation."""
+    rng = np.random.default_rng(R)
+    keys = rng.integers(0, R, N).astype(np.int32)
+    vals = rng.standard_normal(N) * np.where(rng.random(N) < 0.1, 1e12, 1.0)
+    for agg in ("sum", "mean"):
+        (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(
+            cuda(keys), {"v": cuda(vals)}, {"v": agg}))
+        assert any(k.startswith("onepass_table") for k in ran), ran
+        if agg == "sum":
+            _check(keys, vals, uk, res["v"])
+        else:
+            sk, sv, starts = _groups(keys, vals)
+            ends = np.append(starts[1:], N)
+            exact = np.array([math.fsum(sv[a:b]) for a, b in zip(starts, ends)]) / (ends - starts)
+            np.testing.assert_allclose(to_numpy(res["v"]), exact, rtol=1e-12)
