@@ -787,12 +787,16 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V, MM, TE><<<ng1, 1024, 0, s>>>(
                                       a.part, grid, a.ctl, a.epoch, l1)));
   JFX_HIP(hipGetLastError());
+  // the final kernel publishes its result words to the host itself (no
+  // separate read-back kernel)
+  ReadBackSlot rb{};
+  const int rc = read_back_slot(rb);
+  if (rc != JFX_OK) return rc;
   JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT, MM, TE><<<1, 1024, 0, s>>>(
                                       l1, ng1, a.mm, grid, a.ctl, a.epoch, a.keys, n, oc,
-                                      rt_min, static_cast<K*>(out_keys), o, capacity,
-                                      w.op_ctl + 2)));
+                                      rt_min, static_cast<K*>(out_keys), o, capacity, rb)));
   JFX_HIP(hipGetLastError());
-  return read_back(s, w.op_ctl + 2, 2 * sizeof(uint32_t), res);
+  return read_back_wait(s, rb, 2 * sizeof(uint32_t), res);
 }
 
 // te = 1024 or 2048 residues; rt_min = te turns the lane-group tables off;

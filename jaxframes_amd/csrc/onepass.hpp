@@ -30,6 +30,7 @@
 
 #include "blockscan.hpp"
 #include "common.hpp"
+#include "runtime.hpp"
 #include "segreduce.hpp"
 #include "tilereduce.hpp"
 #include "wavetable.hpp"
@@ -468,9 +469,16 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
   }
 }
 
-// Result words written by k_onepass_final: [0] status, [1] groups (on a
-// miss: a lower bound of the key span max - min).
+// Result words that k_onepass_final publishes to the host (ReadBackSlot):
+// [0] status, [1] groups (on a miss: a lower bound of the key span max - min).
 constexpr uint32_t OP_MISS = 0u, OP_HIT = 1u, OP_CAPACITY = 2u;
+
+__device__ __forceinline__ void op_publish(const ReadBackSlot& rb, uint32_t status, uint32_t value) {
+  rb.dst[0] = status;
+  rb.dst[1] = value;
+  __threadfence_system();
+  __hip_atomic_store(rb.seq_slot, rb.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // Two-level merge, thread = residue (coalesced reads): block b of
 // k_onepass_merge merges block tables [OP_GROUP * b, OP_GROUP * (b + 1)) in
@@ -524,7 +532,7 @@ __global__ void __launch_bounds__(1024)
 k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
                 const uint32_t* ctl, uint32_t epoch, const K* keys, uint64_t n, int oc,
                 uint32_t rt_min, K* out_keys, GroupOut<V, MT> out, uint64_t capacity,
-                uint32_t* res) {
+                ReadBackSlot rb) {
   using T = typename OpT<V>::T;
   using U = typename ValTraits<V>::U;
   __shared__ K s_lo[16], s_hi[16];
@@ -588,8 +596,7 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
       // a lower bound of max - min over all keys (the keys that were read),
       // saturated to 32 bits
       const K span = hi >= lo ? hi - lo : K(0);
-      res[0] = OP_MISS;
-      res[1] = span > (K)0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)span;
+      op_publish(rb, OP_MISS, span > (K)0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)span);
     }
     return;
   }
@@ -603,10 +610,7 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
   __syncthreads();
   const uint32_t G = block_excl_scan_lds<TE, 1024>(s_gid, s_tmp);
   if ((uint64_t)G > capacity) {
-    if (tid == 0) {
-      res[0] = OP_CAPACITY;
-      res[1] = G;
-    }
+    if (tid == 0) op_publish(rb, OP_CAPACITY, G);
     return;
   }
   for (uint32_t d = tid; d < TE; d += 1024) {
@@ -631,10 +635,9 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
       else out.mean[g] = (MT)((double)(V)sum / (double)cnt);
     }
   }
-  if (tid == 0) {
-    res[0] = OP_HIT;
-    res[1] = G;
-  }
+  // every thread's outputs are written before the hit is published
+  __syncthreads();
+  if (tid == 0) op_publish(rb, OP_HIT, G);
 }
 
 }  // namespace jfx

@@ -68,8 +68,7 @@ k_read_back(const uint32_t* src, uint32_t nwords, uint32_t* dst, uint64_t* seq_s
   }
 }
 
-int read_back(hipStream_t s, const void* dev_src, size_t bytes, void* host_out) {
-  JFX_CHECK(bytes % 4 == 0 && bytes <= READBACK_MAX, JFX_ERR_INVALID, "bad read-back size");
+int read_back_slot(ReadBackSlot& out) {
   if (!t_coh.h) {
     void* p = nullptr;
     JFX_HIP(hipHostMalloc(&p, RB_BYTES, hipHostMallocCoherent | hipHostMallocMapped));
@@ -82,13 +81,16 @@ int read_back(hipStream_t s, const void* dev_src, size_t bytes, void* host_out) 
     t_coh.h = static_cast<uint32_t*>(p);
     t_coh.d = static_cast<uint32_t*>(dp);
   }
+  out.dst = t_coh.d;
+  out.seq_slot = reinterpret_cast<uint64_t*>(t_coh.d) + RB_SEQ;
+  out.seq = ++t_coh.seq;
+  return JFX_OK;
+}
+
+int read_back_wait(hipStream_t s, const ReadBackSlot& slot, size_t bytes, void* host_out) {
+  JFX_CHECK(bytes <= READBACK_MAX, JFX_ERR_INVALID, "bad read-back size");
   uint64_t* hseq = reinterpret_cast<uint64_t*>(t_coh.h) + RB_SEQ;
-  uint64_t* dseq = reinterpret_cast<uint64_t*>(t_coh.d) + RB_SEQ;
-  const uint64_t seq = ++t_coh.seq;
-  JFX_LAUNCH("read_back", s, (k_read_back<<<1, 256, 0, s>>>(static_cast<const uint32_t*>(dev_src),
-                                                           (uint32_t)(bytes / 4), t_coh.d, dseq,
-                                                           seq)));
-  JFX_HIP(hipGetLastError());
+  const uint64_t seq = slot.seq;
   const auto t0 = std::chrono::steady_clock::now();
   uint32_t polls = 0;
   while (__atomic_load_n(hseq, __ATOMIC_ACQUIRE) != seq) {
@@ -102,6 +104,18 @@ int read_back(hipStream_t s, const void* dev_src, size_t bytes, void* host_out) 
   }
   std::memcpy(host_out, t_coh.h, bytes);
   return JFX_OK;
+}
+
+int read_back(hipStream_t s, const void* dev_src, size_t bytes, void* host_out) {
+  JFX_CHECK(bytes % 4 == 0 && bytes <= READBACK_MAX, JFX_ERR_INVALID, "bad read-back size");
+  ReadBackSlot slot{};
+  int rc = read_back_slot(slot);
+  if (rc != JFX_OK) return rc;
+  JFX_LAUNCH("read_back", s, (k_read_back<<<1, 256, 0, s>>>(static_cast<const uint32_t*>(dev_src),
+                                                           (uint32_t)(bytes / 4), slot.dst,
+                                                           slot.seq_slot, slot.seq)));
+  JFX_HIP(hipGetLastError());
+  return read_back_wait(s, slot, bytes, host_out);
 }
 
 __device__ uint32_t g_device_error;

@@ -54,6 +54,19 @@ void* pinned_scratch(size_t bytes);
 constexpr size_t READBACK_MAX = 16384 - 64;
 int read_back(hipStream_t s, const void* dev_src, size_t bytes, void* host_out);
 
+// The same exchange with the words written by a kernel of the caller: the
+// kernel stores them to slot.dst, then __threadfence_system() and a
+// system-scope release store of slot.seq to *slot.seq_slot; read_back_wait
+// spins for that sequence number (falling back to a stream sync after 2 ms)
+// and copies the words out. One slot per host thread, one exchange at a time.
+struct ReadBackSlot {
+  uint32_t* dst;        // device view of the coherent host buffer
+  uint64_t* seq_slot;   // device view of its sequence word
+  uint64_t seq;         // the value to publish
+};
+int read_back_slot(ReadBackSlot& out);
+int read_back_wait(hipStream_t s, const ReadBackSlot& slot, size_t bytes, void* host_out);
+
 // Device error word (one per device, in device memory owned by the library
 // image as a __device__ variable).
 uint32_t* device_error_word();
