@@ -118,12 +118,12 @@ int jfx_groupby_workspace_size(int key_type, int val_type, uint64_t n, uint64_t*
 /* One-column groupby: radix sort of the keys carrying the values as payload,
  * then jfx_segreduce. Replaces groupby_aggregate (parallel_algorithms.py:
  * 1117-1148) for a single aggregated column. Synchronises once for the digit
- * plan and once to read G into *out_ngroups. Integer keys with at least 2^22
- * rows and no min / max output first try one pass (wave tables indexed by
- * key mod 1024; float32 or integer values): when max - min < 1024 that pass
- * gives the groups with one synchronisation, otherwise the sort-based or
- * small-range path runs as above (JFX_ONEPASS=0 skips the one-pass try,
- * JFX_ONEPASS=always makes it at any size). */
+ * plan and once to read G into *out_ngroups. Integer keys with at least 2^20
+ * rows first try one pass (wave tables indexed by key mod 1024, retried once
+ * at 2048 when the span seen allows): when max - min is below the table width
+ * that pass gives the groups with one synchronisation, otherwise the
+ * sort-based or small-range path runs as above (JFX_ONEPASS=0 skips the
+ * one-pass try, JFX_ONEPASS=always makes it at any size). */
 int jfx_groupby(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
                 void* out_keys, void* out_sum, int64_t* out_count, void* out_min, void* out_max,
                 void* out_mean, uint64_t* out_ngroups, int digit_bits, void* workspace,

@@ -719,15 +719,15 @@ int tile_reduce_key(const SortCtx& c, const SegWs& w, int vt, const void* keys, 
 }
 
 // ---- one-pass path ------------------------------------------------------------
-// Integer keys, float32 or integer values (sum / count / mean / min / max), at
-// least 2^22 rows:
-// a miss costs about 30 us (the aborted table launch, two small kernels and a
-// read-back) before the statistics-first path, under 1% of a groupby of that
-// size. JFX_ONEPASS=0 turns the path off, =always tries it at any size.
+// Integer keys, float32 / float64 / integer values (sum / count / mean / min /
+// max), at least 2^20 rows: a miss costs about 10 us (the aborted table
+// launch, two small kernels and the result exchange) before the
+// statistics-first path, measured against 25 us saved by a hit at 1M and 2M
+// rows. JFX_ONEPASS=0 turns the path off, =always tries it at any size.
 // res receives the final kernel's result words.
 
 std::atomic<uint32_t> g_op_epoch{0};
-constexpr uint64_t OP_MIN_ROWS = 1ull << 22;
+constexpr uint64_t OP_MIN_ROWS = 1ull << 20;
 
 bool onepass_wanted(int kt, int vt, uint64_t n) {
   (void)vt;

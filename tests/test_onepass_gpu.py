@@ -26,7 +26,7 @@ N = 1 << 18
 
 @pytest.fixture(autouse=True)
 def _onepass_at_any_size(monkeypatch):
-    # by default the one-pass try starts at 2^22 rows; these tests take it at
+    # by default the one-pass try starts at 2^20 rows; these tests take it at
     # any size (test_onepass_default_threshold checks the default)
     monkeypatch.setenv("JFX_ONEPASS", "always")
 
@@ -210,7 +210,7 @@ def test_onepass_small_inputs(n):
     _check(keys, vals, uk, res["v"])
 
 
-@pytest.mark.parametrize("n,onepass", [((1 << 22) - 1, False), ((1 << 22) + 5, True)])
+@pytest.mark.parametrize("n,onepass", [((1 << 20) - 1, False), ((1 << 20) + 5, True)])
 def test_onepass_default_threshold(monkeypatch, n, onepass):
     monkeypatch.delenv("JFX_ONEPASS")
     rng = np.random.default_rng(17)
