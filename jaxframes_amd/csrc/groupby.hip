@@ -446,7 +446,7 @@ struct SegWs {
   uint32_t* dgid;   // [2048] group index of each digit (small-range path)
   void* part;       // [chunks][2048] chunk accumulators (small-range path)
   uint64_t* op_mm;  // [SORT_MAXCHUNKS * 2][2] one-pass: block min / max of the ordered keys
-  uint32_t* op_ctl; // [4] one-pass: stop word, (unused), result words
+  uint32_t* op_ctl; // [4] one-pass: [0] stop word
 };
 
 uint64_t seg_tiles(uint64_t n) { return (n + SEG_TILE - 1) / SEG_TILE; }
