@@ -199,15 +199,23 @@ def test_onepass_capacity_report():
 
 
 @pytest.mark.parametrize("n", [1, 5, 1000, 65537])
-def test_onepass_small_inputs(n):
-    """Inputs smaller than one wave step go row by row (no vector loop)."""
-    rng = np.random.default_rng(n)
-    keys = rng.integers(-7, 900, n).astype(np.int64)
+@pytest.mark.parametrize("R", [907, 3])
+def test_onepass_small_inputs(n, R):
+    """Inputs smaller than one wave step go row by row (no vector loop),
+    with 1024-residue and (R = 3) lane-group tables; all five aggregations."""
+    rng = np.random.default_rng(n + R)
+    keys = rng.integers(-7, R - 7, n).astype(np.int64)
     vals = rng.standard_normal(n).astype(np.float32)
     (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(cuda(keys), {"v": cuda(vals)},
                                                                        {"v": "sum"}))
     assert "onepass_table" in ran and "sort_stats" not in ran, ran
     _check(keys, vals, uk, res["v"])
+    uk, st = grouping.groupby_stats(cuda(keys), cuda(vals))
+    sk, sv, starts = _groups(keys, vals)
+    assert np.array_equal(to_numpy(uk), sk[starts])
+    assert np.array_equal(to_numpy(st["count"]), np.diff(np.append(starts, n)))
+    assert np.array_equal(to_numpy(st["min"]), np.minimum.reduceat(sv, starts))
+    assert np.array_equal(to_numpy(st["max"]), np.maximum.reduceat(sv, starts))
 
 
 @pytest.mark.parametrize("n,onepass", [((1 << 20) - 1, False), ((1 << 20) + 5, True)])
