@@ -118,7 +118,7 @@ int jfx_groupby_workspace_size(int key_type, int val_type, uint64_t n, uint64_t*
 /* One-column groupby: radix sort of the keys carrying the values as payload,
  * then jfx_segreduce. Replaces groupby_aggregate (parallel_algorithms.py:
  * 1117-1148) for a single aggregated column. Synchronises once for the digit
- * plan and once to read G into *out_ngroups. Integer keys with at least 2^20
+ * plan and once to read G into *out_ngroups. Integer keys with at least 2^16
  * rows first try one pass (wave tables indexed by key mod 1024, retried once
  * at 2048 when the span seen allows): when max - min is below the table width
  * that pass gives the groups with one synchronisation, otherwise the

@@ -720,14 +720,14 @@ int tile_reduce_key(const SortCtx& c, const SegWs& w, int vt, const void* keys, 
 
 // ---- one-pass path ------------------------------------------------------------
 // Integer keys, float32 / float64 / integer values (sum / count / mean / min /
-// max), at least 2^20 rows: a miss costs about 10 us (the aborted table
-// launch, two small kernels and the result exchange) before the
-// statistics-first path, measured against 25 us saved by a hit at 1M and 2M
-// rows. JFX_ONEPASS=0 turns the path off, =always tries it at any size.
+// max), at least 2^16 rows: a miss costs 10-14 us (the aborted table launch,
+// two small kernels and the result exchange) before the statistics-first
+// path, against 22-25 us saved by a hit (measured at 64K to 2M rows).
+// JFX_ONEPASS=0 turns the path off, =always tries it at any size.
 // res receives the final kernel's result words.
 
 std::atomic<uint32_t> g_op_epoch{0};
-constexpr uint64_t OP_MIN_ROWS = 1ull << 20;
+constexpr uint64_t OP_MIN_ROWS = 1ull << 16;
 
 bool onepass_wanted(int kt, int vt, uint64_t n) {
   (void)vt;

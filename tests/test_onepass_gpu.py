@@ -26,7 +26,7 @@ N = 1 << 18
 
 @pytest.fixture(autouse=True)
 def _onepass_at_any_size(monkeypatch):
-    # by default the one-pass try starts at 2^20 rows; these tests take it at
+    # by default the one-pass try starts at 2^16 rows; these tests take it at
     # any size (test_onepass_default_threshold checks the default)
     monkeypatch.setenv("JFX_ONEPASS", "always")
 
@@ -218,7 +218,7 @@ def test_onepass_small_inputs(n, R):
     assert np.array_equal(to_numpy(st["max"]), np.maximum.reduceat(sv, starts))
 
 
-@pytest.mark.parametrize("n,onepass", [((1 << 20) - 1, False), ((1 << 20) + 5, True)])
+@pytest.mark.parametrize("n,onepass", [((1 << 16) - 1, False), ((1 << 16) + 5, True)])
 def test_onepass_default_threshold(monkeypatch, n, onepass):
     monkeypatch.delenv("JFX_ONEPASS")
     rng = np.random.default_rng(17)
