@@ -364,6 +364,10 @@ int jfx_prof_reset(void);
 /* Time only launches whose kernel name starts with name_prefix ("" or NULL:
  * all), so a timed region can carry events around one kernel only. */
 int jfx_prof_filter(const char* name_prefix);
+/* Time every `every`-th launch that passes the filter (1, the default: all),
+ * counted from the last jfx_prof_filter / jfx_prof_reset / jfx_prof_sample
+ * call, so that a long timed region carries few events. */
+int jfx_prof_sample(uint32_t every);
 int jfx_prof_count(int* out_n);
 int jfx_prof_get(int i, char* name_buf, uint64_t buf_len, double* total_ms, uint64_t* count);
 

@@ -52,6 +52,7 @@ _SIGNATURES = {
     "jfx_prof_enable": [c_int],
     "jfx_prof_reset": [],
     "jfx_prof_filter": [ctypes.c_char_p],
+    "jfx_prof_sample": [ctypes.c_uint32],
     "jfx_prof_count": [ctypes.POINTER(c_int)],
     "jfx_prof_get": [c_int, ctypes.c_char_p, c_u64, ctypes.POINTER(ctypes.c_double), c_u64p],
     "jfx_hash_rows": [ctypes.POINTER(c_vp), ctypes.POINTER(c_int), c_int, c_u64, c_vp, c_vp],
@@ -158,18 +159,26 @@ def stream_handle(device=None) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
+_gpu_seen = False
+
+
 def require_gpu():
+    global _gpu_seen
+    if _gpu_seen:
+        return
     if not torch.cuda.is_available():
         raise JfxError("no ROCm GPU is visible; jaxframes_amd runs on MI355X only")
+    _gpu_seen = True
 
 
 _ws_cache: dict = {}
 
 
-def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
-    """A cached device byte buffer of at least nbytes for (device, stream)."""
+def workspace(nbytes: int, device: torch.device, stream: int | None = None) -> torch.Tensor:
+    """A cached device byte buffer of at least nbytes for (device, stream);
+    stream is the current stream's handle when the caller already holds it."""
     key = (device.index if device.index is not None else torch.cuda.current_device(),
-           stream_handle(device))
+           stream_handle(device) if stream is None else stream)
     buf = _ws_cache.get(key)
     if buf is None or buf.numel() < nbytes:
         cap = max(int(nbytes * 1.25), 1 << 20)
@@ -191,20 +200,25 @@ def ptr(t: torch.Tensor | None) -> c_vp:
 class KernelTimer:
     """Per-kernel HIP-event timing inside libjfx (events recorded on the
     stream each kernel is launched on). Use as a context manager around the
-    timed region, then read .stats() -> {name: (total_ms, launches)}."""
+    timed region, then read .stats() -> {name: (total_ms, launches)}.
+    prefix limits the timing to kernels whose name starts with it, every to
+    one in `every` of those launches (launches counts the timed ones)."""
 
-    def __init__(self, prefix: str = ""):
+    def __init__(self, prefix: str = "", every: int = 1):
         self.prefix = prefix
+        self.every = every
 
     def __enter__(self):
         call("jfx_prof_reset")
         call("jfx_prof_filter", self.prefix.encode())
+        call("jfx_prof_sample", self.every)
         call("jfx_prof_enable", 1)
         return self
 
     def __exit__(self, *exc):
         call("jfx_prof_enable", 0)
         call("jfx_prof_filter", b"")
+        call("jfx_prof_sample", 1)
 
     @staticmethod
     def stats() -> dict:

@@ -784,7 +784,8 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   JFX_CHECK(((uint64_t)grid + ng1) * TE * sizeof(OpPart<V, MM>) <= part_bytes,
             JFX_ERR_WORKSPACE, "one-pass partials exceed the partial buffer");
   OpPart<V, MM>* l1 = a.part + (size_t)grid * TE;
-  JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V, MM, TE><<<ng1, 1024, 0, s>>>(
+  static_assert(TE % OP_MT == 0, "level-1 merge blocks cover whole tables");
+  JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V, MM, TE><<<dim3(ng1, TE / OP_MT), OP_MT, 0, s>>>(
                                       a.part, grid, a.ctl, a.epoch, l1)));
   JFX_HIP(hipGetLastError());
   // the final kernel publishes its result words to the host itself (no

@@ -480,23 +480,24 @@ __device__ __forceinline__ void op_publish(const ReadBackSlot& rb, uint32_t stat
   __hip_atomic_store(rb.seq_slot, rb.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Two-level merge, thread = residue (coalesced reads): block b of
-// k_onepass_merge merges block tables [OP_GROUP * b, OP_GROUP * (b + 1)) in
-// block order into l1[b]; k_onepass_final (one block) merges the at most
-// OP_L1 level-1 tables in order, checks the key range and writes the groups
-// in key order. Each merge issues all its loads before the first add. (A
-// last-block ticket in one kernel measured slower: its release fences write
-// back L2.)
-constexpr uint32_t OP_GROUP = 32;
-constexpr uint32_t OP_L1 = 32;  // up to 1024 block tables
+// Two-level merge, thread = residue (coalesced reads): block (g, c) of
+// k_onepass_merge merges residues [OP_MT * c, OP_MT * (c + 1)) of block
+// tables [OP_GROUP * g, OP_GROUP * (g + 1)) in block order into l1[g];
+// k_onepass_final (one block) merges the at most OP_L1 level-1 tables in
+// order, checks the key range and writes the groups in key order. Each merge
+// issues all its loads before the first add: the level-1 blocks are small
+// enough to hold 64 states per thread (32 of the 32- and 40-byte states that
+// carry 64-bit min / max codes), the final block of 1024 threads holds its 16. (A last-block ticket
+// in one kernel measured slower: its release fences write back L2.)
+constexpr uint32_t OP_GROUP = 64;
+constexpr uint32_t OP_L1 = 16;  // up to 1024 block tables
+constexpr uint32_t OP_MT = 128;  // level-1 merge threads per block
 
 // acc, n := acc, n plus p[0], p[stride], ... (count entries, added in order
-// in the accumulator type), 16 loads in flight at a time (more would spill
-// at 1024 threads).
-template <typename V, bool MM>
+// in the accumulator type), G loads in flight at a time.
+template <uint32_t G, typename V, bool MM>
 __device__ __forceinline__ void merge_seq(OpAcc<V, MM>& st, const OpPart<V, MM>* p, size_t stride,
                                           uint32_t count) {
-  constexpr uint32_t G = 16;
   OpAcc<V, MM> x = st;  // a local copy: accumulating through the reference kept
                         // the compiler from leaving the 16 loads in flight
   for (uint32_t b = 0; b < count; b += G, p += G * stride) {
@@ -512,19 +513,20 @@ __device__ __forceinline__ void merge_seq(OpAcc<V, MM>& st, const OpPart<V, MM>*
   st = x;
 }
 
+// grid (groups, TE / OP_MT)
 template <typename V, bool MM, uint32_t TE>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(OP_MT)
 k_onepass_merge(const OpPart<V, MM>* part, uint32_t nblocks, const uint32_t* ctl, uint32_t epoch,
                 OpPart<V, MM>* l1) {
   if (ctl[0] == epoch) return;  // stopped: the result is a miss
+  constexpr uint32_t G = sizeof(OpPart<V, MM>) <= 24 ? OP_GROUP : OP_GROUP / 2;
   const uint32_t c0 = blockIdx.x * OP_GROUP;
   const uint32_t c1 = (c0 + OP_GROUP) < nblocks ? (c0 + OP_GROUP) : nblocks;
-  for (uint32_t r = threadIdx.x; r < TE; r += 1024) {
-    OpAcc<V, MM> st;
-    st.init();
-    merge_seq<V, MM>(st, part + (size_t)c0 * TE + r, TE, c1 - c0);
-    l1[(size_t)blockIdx.x * TE + r] = st.pack();
-  }
+  const uint32_t r = blockIdx.y * OP_MT + threadIdx.x;
+  OpAcc<V, MM> st;
+  st.init();
+  merge_seq<G>(st, part + (size_t)c0 * TE + r, TE, c1 - c0);
+  l1[(size_t)blockIdx.x * TE + r] = st.pack();
 }
 
 template <typename K, typename V, typename MT, bool MM, uint32_t TE>
@@ -561,18 +563,18 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
     lo = l < lo ? l : lo;
     hi = h > hi ? h : hi;
   }
-  if (!stopped) {
-    for (uint32_t r = tid; r < TE; r += 1024) {
-      OpAcc<V, MM> st;
-      st.init();
-      merge_seq<V, MM>(st, l1 + r, TE, nl1);
-      s_sum[r] = st.s;
-      if constexpr (DD) s_c[r] = st.c;
-      s_n[r] = st.n;
-      if constexpr (MM) {
-        s_min[r] = st.mn;
-        s_max[r] = st.mx;
-      }
+  // merged without waiting for the stop word (after a stop l1 holds stale
+  // states, which a miss never reads out)
+  for (uint32_t r = tid; r < TE; r += 1024) {
+    OpAcc<V, MM> st;
+    st.init();
+    merge_seq<OP_L1>(st, l1 + r, TE, nl1);
+    s_sum[r] = st.s;
+    if constexpr (DD) s_c[r] = st.c;
+    s_n[r] = st.n;
+    if constexpr (MM) {
+      s_min[r] = st.mn;
+      s_max[r] = st.mx;
     }
   }
 

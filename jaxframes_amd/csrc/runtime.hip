@@ -147,6 +147,8 @@ struct ProfState {
   };
   std::vector<Pending> pending;
   std::string filter;   // time only kernels whose name starts with this
+  uint32_t every = 1;   // ... and of those every `every`-th launch
+  uint64_t seen = 0;    // launches that passed the filter since the count restarted
   struct Acc {
     double ms = 0.0;
     uint64_t count = 0;
@@ -161,6 +163,15 @@ struct ProfState {
     hipEvent_t e = nullptr;
     (void)hipEventCreate(&e);
     return e;
+  }
+  // events are created up front, so that a timed launch does not pay for
+  // hipEventCreate
+  void reserve(size_t k) {
+    for (size_t have = pool.size() + 2 * pending.size(); have < k; ++have) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      pool.push_back(e);
+    }
   }
   Acc& slot(const std::string& n) {
     for (auto& p : acc)
@@ -190,8 +201,10 @@ thread_local hipEvent_t t_prof_start = nullptr;  // start event of this thread's
 bool prof_wants(const char* name) {
   if (!g_prof.on) return false;
   std::lock_guard<std::mutex> lk(g_prof.mu);
-  return g_prof.filter.empty() ||
-         std::strncmp(name, g_prof.filter.c_str(), g_prof.filter.size()) == 0;
+  if (!g_prof.filter.empty() &&
+      std::strncmp(name, g_prof.filter.c_str(), g_prof.filter.size()) != 0)
+    return false;
+  return g_prof.seen++ % g_prof.every == 0;
 }
 
 void prof_begin(hipStream_t s) {
@@ -215,12 +228,22 @@ extern "C" {
 int jfx_prof_enable(int on) {
   std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
   jfx::g_prof.on = on != 0;
+  if (jfx::g_prof.on) jfx::g_prof.reserve(512);
   return JFX_OK;
 }
 
 int jfx_prof_filter(const char* name_prefix) {
   std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
   jfx::g_prof.filter = name_prefix ? name_prefix : "";
+  jfx::g_prof.seen = 0;
+  return JFX_OK;
+}
+
+int jfx_prof_sample(uint32_t every) {
+  JFX_CHECK(every >= 1, JFX_ERR_INVALID, "every must be at least 1");
+  std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
+  jfx::g_prof.every = every;
+  jfx::g_prof.seen = 0;
   return JFX_OK;
 }
 
@@ -228,6 +251,7 @@ int jfx_prof_reset(void) {
   std::lock_guard<std::mutex> lk(jfx::g_prof.mu);
   int rc = jfx::g_prof.collect();
   jfx::g_prof.acc.clear();
+  jfx::g_prof.seen = 0;
   return rc;
 }
 

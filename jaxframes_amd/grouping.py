@@ -90,12 +90,25 @@ def _segreduce_ws(vcode, n, dev):
     return _lib.workspace(nb.value, dev)
 
 
+_ws_sizes: dict = {}  # (key code, value code, rows) -> workspace bytes
+
+
+def _groupby_ws(kc, vcode, n, dev, stream):
+    nb = _ws_sizes.get((kc, vcode, n))
+    if nb is None:
+        b = ctypes.c_uint64(0)
+        _lib.call("jfx_groupby_workspace_size", kc, vcode, n, ctypes.byref(b))
+        nb = b.value
+        if len(_ws_sizes) >= 256:
+            _ws_sizes.clear()
+        _ws_sizes[(kc, vcode, n)] = nb
+    return _lib.workspace(nb, dev, stream.value)
+
+
 def _aggregate_one(k, kc, v, vorig, fn, n, dev, stream, digit_bits, want_keys):
     """One value column through jfx_groupby_ex: (G, out_keys or None, column)."""
     vcode = _VAL_CODE[v.dtype]
-    nb = ctypes.c_uint64(0)
-    _lib.call("jfx_groupby_workspace_size", kc, vcode, n, ctypes.byref(nb))
-    ws = _lib.workspace(nb.value, dev)
+    ws = _groupby_ws(kc, vcode, n, dev, stream)
     g = ctypes.c_uint64(0)
     # large inputs first try outputs sized for a small key range; the call
     # reports the capacity it needs when the keys span more

@@ -364,3 +364,24 @@ def test_onepass_float64_sums(R):
             ends = np.append(starts[1:], N)
             exact = np.array([math.fsum(sv[a:b]) for a, b in zip(starts, ends)]) / (ends - starts)
             np.testing.assert_allclose(to_numpy(res["v"]), exact, rtol=1e-12)
+
+
+def test_kernel_timer_samples_every_kth_launch():
+    """KernelTimer(prefix, every=k) times launches 0, k, 2k, ... of the
+    kernels passing the prefix filter, and nothing else."""
+    rng = np.random.default_rng(31)
+    keys = cuda(rng.integers(0, 100, N).astype(np.int32))
+    vals = cuda(rng.standard_normal(N).astype(np.float32))
+    with _lib.KernelTimer(prefix="onepass_table", every=3) as kt:
+        for _ in range(7):
+            grouping.groupby_aggregate(keys, {"v": vals}, {"v": "sum"})
+        torch.cuda.synchronize()
+    st = kt.stats()
+    assert set(st) == {"onepass_table"}
+    assert st["onepass_table"][1] == 3
+    with _lib.KernelTimer(prefix="onepass_table") as kt:
+        grouping.groupby_aggregate(keys, {"v": vals}, {"v": "sum"})
+        torch.cuda.synchronize()
+    assert kt.stats()["onepass_table"][1] == 1
+    with pytest.raises(_lib.JfxError):
+        _lib.call("jfx_prof_sample", 0)
