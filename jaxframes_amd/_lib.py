@@ -155,8 +155,24 @@ def call(name: str, *args):
     check(getattr(lib(), name)(*args), name)
 
 
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def stream_handle(device=None) -> int:
-    return torch.cuda.current_stream(device).cuda_stream
+    """The current HIP stream of device (None: the current device) as an
+    integer handle (torch's raw-stream query, a tenth of the cost of
+    torch.cuda.current_stream, where this torch has it)."""
+    if _raw_stream is None:
+        return torch.cuda.current_stream(device).cuda_stream
+    if device is None:
+        idx = torch.cuda.current_device()
+    elif isinstance(device, int):
+        idx = device
+    else:
+        idx = torch.device(device).index
+        if idx is None:
+            idx = torch.cuda.current_device()
+    return _raw_stream(idx)
 
 
 _gpu_seen = False

@@ -385,3 +385,14 @@ def test_kernel_timer_samples_every_kth_launch():
     assert kt.stats()["onepass_table"][1] == 1
     with pytest.raises(_lib.JfxError):
         _lib.call("jfx_prof_sample", 0)
+
+
+def test_stream_handle_is_the_current_stream():
+    assert _lib.stream_handle() == torch.cuda.current_stream().cuda_stream
+    assert _lib.stream_handle(torch.device("cuda")) == torch.cuda.current_stream().cuda_stream
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert _lib.stream_handle() == s.cuda_stream
+        assert _lib.stream_handle(0) == s.cuda_stream
+        assert _lib.stream_handle(torch.device("cuda", 0)) == s.cuda_stream
+    assert _lib.stream_handle() == torch.cuda.current_stream().cuda_stream
