@@ -324,7 +324,12 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT,
 // One workgroup; thread t owns a contiguous run of tiles. Pass 1 folds the
 // run into one (flag, state) element, a workgroup segmented scan gives each
 // thread the state of the group open at its first tile, pass 2 walks the run
-// again and finalises every group that ends at a tile's first head.
+// again and finalises every group that ends at a tile's first head. A tile's
+// info, lead and trail are loaded together (trail is left unwritten by a tile
+// without heads and then not used), so that each batch of B tiles is one
+// round trip rather than two dependent ones. States are chosen with plain
+// branches: a conditional expression over two states selects an address,
+// which kept the arrays in scratch memory.
 template <typename K, typename V, typename MT, bool MM>
 __global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT, MM> a) {
   __shared__ AggState<V, MM> s_wst[16 + 1];
@@ -334,21 +339,30 @@ __global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT, MM> a)
   const uint32_t per = (nt + 1023u) / 1024u;
   const uint32_t u0 = tid * per;
   const uint32_t u1 = (u0 + per) < nt ? (u0 + per) : nt;
-  constexpr uint32_t B = 4;  // tiles fetched per round trip
+  // tiles fetched per round trip (two states and the info per tile in flight)
+  constexpr uint32_t B = sizeof(AggState<V, MM>) <= 16 ? 8 : 4;
   uint32_t lf = 0u;
   AggState<V, MM> lst = AggState<V, MM>::identity();
   for (uint32_t u = u0; u < u1; u += B) {
     uint32_t hh[B];
-    AggState<V, MM> el[B];
+    AggState<V, MM> ld[B], tr[B];
 #pragma unroll
     for (uint32_t q = 0; q < B; ++q) {
-      const bool ok = u + q < u1;
-      hh[q] = ok && a.tile_info[u + q].y > 0u ? 1u : 0u;
-      el[q] = ok ? (hh[q] ? a.tile_trail[u + q] : a.tile_lead[u + q]) : AggState<V, MM>::identity();
+      hh[q] = 0u;
+      ld[q] = AggState<V, MM>::identity();
+      tr[q] = AggState<V, MM>::identity();
+      if (u + q < u1) {
+        hh[q] = a.tile_info[u + q].y;
+        ld[q] = a.tile_lead[u + q];
+        tr[q] = a.tile_trail[u + q];
+      }
     }
 #pragma unroll
-    for (uint32_t q = 0; q < B; ++q)
-      if (u + q < u1) seg_combine(lf, lst, hh[q], el[q]);
+    for (uint32_t q = 0; q < B; ++q) {
+      if (u + q >= u1) continue;
+      if (hh[q] > 0u) seg_combine(lf, lst, 1u, tr[q]);
+      else seg_combine(lf, lst, 0u, ld[q]);
+    }
   }
   uint32_t ef, tf;
   AggState<V, MM> est, tst;
@@ -360,10 +374,14 @@ __global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT, MM> a)
     AggState<V, MM> lead[B], trail[B];
 #pragma unroll
     for (uint32_t q = 0; q < B; ++q) {
-      const bool ok = u + q < u1;
-      info[q] = ok ? a.tile_info[u + q] : make_uint2(0u, 0u);
-      lead[q] = ok ? a.tile_lead[u + q] : AggState<V, MM>::identity();
-      trail[q] = (ok && info[q].y > 0u) ? a.tile_trail[u + q] : AggState<V, MM>::identity();
+      info[q] = make_uint2(0u, 0u);
+      lead[q] = AggState<V, MM>::identity();
+      trail[q] = AggState<V, MM>::identity();
+      if (u + q < u1) {
+        info[q] = a.tile_info[u + q];
+        lead[q] = a.tile_lead[u + q];
+        trail[q] = a.tile_trail[u + q];
+      }
     }
 #pragma unroll
     for (uint32_t q = 0; q < B; ++q) {
@@ -374,7 +392,8 @@ __global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT, MM> a)
         full.merge(lead[q]);
         finalize(a, info[q].x - 1u, full);
       }
-      seg_combine(cf, carry, hq, hq ? trail[q] : lead[q]);
+      if (hq) seg_combine(cf, carry, 1u, trail[q]);
+      else seg_combine(cf, carry, 0u, lead[q]);
     }
   }
   if (tid == 0) {
