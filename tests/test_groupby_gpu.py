@@ -286,3 +286,54 @@ def test_float_sums_with_infinities(vdt, span, stats):
     absx = np.add.reduceat(np.abs(sv.astype(np.float64)), starts)
     r = np.abs(got[fin].astype(np.float64) - exact[fin]) / absx[fin]
     assert r.max() <= (SUM_NORM_TOL if vdt == np.float32 else SUM64_NORM_TOL), r.max()
+
+
+def _dense_keys(pattern, n, rng):
+    """int64 keys spread over 2^62 (no small-range or few-groups shortcut) whose
+    sorted order has the given group sizes."""
+    if pattern == "unique":
+        sizes = np.ones(n, dtype=np.int64)
+    elif pattern == "pairs":
+        sizes = np.full((n + 1) // 2, 2, dtype=np.int64)
+    else:  # stretches of single rows and of runs up to 40 rows, mixed
+        parts = []
+        while sum(int(p.sum()) for p in parts) < n:
+            parts.append(np.ones(rng.integers(1, 3000), dtype=np.int64))
+            parts.append(rng.integers(1, 41, rng.integers(1, 300)))
+        sizes = np.concatenate(parts)
+        sizes = sizes[np.cumsum(sizes) <= n]
+        sizes = np.append(sizes, n - sizes.sum()) if sizes.sum() < n else sizes
+    g = sizes.size
+    distinct = np.unique(rng.integers(-2**62, 2**62, g + 64, dtype=np.int64))[:g]
+    keys = np.repeat(np.sort(distinct), sizes)[:n]
+    return keys[rng.permutation(n)]
+
+
+@pytest.mark.parametrize("pattern", ["unique", "pairs", "mixed"])
+@pytest.mark.parametrize("n", [8193, 100003, 300001])
+@pytest.mark.parametrize("vdt", [np.float32, np.int64])
+def test_many_groups_staged_writes(pattern, n, vdt):
+    """Nearly one group per row: keys, sums, counts and means go out through
+    the per-wave staging of the segmented reduction."""
+    rng = np.random.default_rng(n + len(pattern))
+    keys = _dense_keys(pattern, n, rng)
+    if vdt == np.float32:
+        vals = rng.standard_normal(n).astype(np.float32)
+    else:
+        vals = rng.integers(-2**62, 2**62, n, dtype=np.int64)
+    sk, sv, starts = _oracle_groups(keys, vals)
+    counts = np.diff(np.append(starts, n))
+    aggs = {"s": "sum", "c": "count", "m": "mean"}
+    uk, res = grouping.groupby_aggregate(cuda(keys), {c: cuda(vals) for c in aggs}, aggs)
+    assert np.array_equal(to_numpy(uk), sk[starts])
+    assert np.array_equal(to_numpy(res["c"]), counts)
+    got = to_numpy(res["s"])
+    if vdt == np.float32:
+        _check_float_sums(got, sv, starts, np.float32)
+        ref_mean = oracle.segment_sums_f64(sv, starts) / counts
+        nrm = np.add.reduceat(np.abs(sv.astype(np.float64)), starts) / counts
+        assert (np.abs(to_numpy(res["m"]) - ref_mean) <= 4e-7 * nrm + 1e-30).all()
+    else:
+        ref = _wrapping_sums(sv, starts)
+        assert np.array_equal(got, ref)
+        assert np.array_equal(to_numpy(res["m"]), ref.astype(np.float64) / counts)
