@@ -89,20 +89,33 @@ struct Stats4 {
 // pass into g_hist[pass][256], so a multi-pass sort reads its keys once
 // before the scatter passes instead of twice: digit 0 is folded from the
 // low-11-bit histogram, the higher digits take one LDS atomic each (one per
-// wave when the whole wave agrees on the digit).
+// wave when the whole wave agrees on the digit). The top digit of keys with a
+// narrow range often takes two or three values in a wave, and atomics of one
+// instruction to one address serialise: it is counted in TOP_REP copies, lane
+// l adding to copy l % TOP_REP (float keys in [0, 14285], 1e8 keys: 596 us
+// with one table, 385 us with 8 copies, 345 us with 16; uniform int64 keys
+// 231-238 us either way).
+constexpr int TOP_REP = 16;
+constexpr int TOP_STRIDE = 257;  // copies of one bin fall in different banks
+
 template <typename K, bool HIST>
 __global__ void __launch_bounds__(STATS_BLOCK)
 k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uint32_t* counts,
              unsigned long long* g_stats, uint32_t* g_maxbin, uint32_t* g_hist) {
   constexpr int NP = (int)sizeof(K);  // 8-bit layout digits
-  constexpr int NH = HIST ? (NP - 1) * 256 : 1;
+  constexpr int NH = HIST ? (NP - 2) * 256 : 1;      // digits 1 .. NP-2
+  constexpr int NT = HIST ? TOP_REP * TOP_STRIDE : 1;  // digit NP-1
   __shared__ uint32_t s_low[2048];
   __shared__ uint32_t s_h[NH];
+  __shared__ uint32_t s_top[NT];
   for (int i = threadIdx.x; i < 2048; i += STATS_BLOCK) s_low[i] = 0u;
-  if (HIST)
+  if (HIST) {
     for (int i = threadIdx.x; i < NH; i += STATS_BLOCK) s_h[i] = 0u;
+    for (int i = threadIdx.x; i < NT; i += STATS_BLOCK) s_top[i] = 0u;
+  }
   __syncthreads();
   const uint32_t lane = lane_id();
+  uint32_t* const my_top = s_top + (lane % TOP_REP) * TOP_STRIDE;
   auto digits = [&](K o) {
     if (!HIST) return;
     const uint64_t act = ballot64(true);
@@ -110,10 +123,11 @@ k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uin
     for (int p = 1; p < NP; ++p) {
       const uint32_t bin = (uint32_t)((o >> (p * 8)) & (K)255);
       const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
+      uint32_t* const tab = p == NP - 1 ? s_top : s_h + (p - 1) * 256;
       if (ballot64(bin == b0) == act) {
-        if (lane == (uint32_t)__builtin_ctzll(act)) atomicAdd(&s_h[(p - 1) * 256 + b0], (uint32_t)__popcll(act));
+        if (lane == (uint32_t)__builtin_ctzll(act)) atomicAdd(&tab[b0], (uint32_t)__popcll(act));
       } else {
-        atomicAdd(&s_h[(p - 1) * 256 + bin], 1u);
+        atomicAdd(p == NP - 1 ? &my_top[bin] : &tab[bin], 1u);
       }
     }
   };
@@ -224,6 +238,12 @@ k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uin
     for (int t = threadIdx.x; t < NH; t += STATS_BLOCK) {
       const uint32_t v = s_h[t];
       if (v) atomicAdd(&g_hist[256 + t], v);
+    }
+    if (threadIdx.x < 256) {
+      uint32_t v = 0u;
+#pragma unroll
+      for (int c = 0; c < TOP_REP; ++c) v += s_top[c * TOP_STRIDE + threadIdx.x];
+      if (v) atomicAdd(&g_hist[256 * (NP - 1) + threadIdx.x], v);
     }
   }
 }
