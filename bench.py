@@ -145,7 +145,7 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
 
 
 def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
-                    budget_bytes: int = 64 << 30, reps: int = 3):
+                    budget_bytes: int = 64 << 30, reps: int = 5):
     """The served stream: 8 resident frames (1M..50M rows, int64 keys of
     cardinality 10..1M, float32 values), 70% filter -> groupby(sum, count,
     mean) and 30% inner joins with 1e5..1e6-row dimension tables, Poisson
