@@ -396,25 +396,24 @@ __global__ void __launch_bounds__(SEG_BLOCK) k_segreduce_tiles(SegArgs<K, V, MT,
     auto agg_id = [&](int j) { return toff + before(j) - 1u; };
     // rows of the group closed at head j (j in omask: a head lies below j)
     auto rows_j = [&](int j) { return (uint32_t)j - (31u - (uint32_t)__clz(hmask & ((1u << j) - 1u))); };
-    const uint32_t wlane = lane;
     if (a.out_keys) {
       const uint32_t klo = __shfl(toff, 0, 64);
       const uint32_t khi = __shfl(toff + nh, 63, 64);
       wave_stage_store<K>(a.out_keys + texcl, klo, khi, hmask, [&](int j) { return toff + before(j); },
                           [&](int j) { return kj[j]; }, false, 0u, K(0),
-                          reinterpret_cast<K*>(s_stage + wid * SEG_WIN), wlane);
+                          reinterpret_cast<K*>(s_stage + wid * SEG_WIN), lane);
     }
     if (a.out_sum)
       wave_stage_store<V>(a.out_sum + texcl, alo, ahi, omask, agg_id, [&](int j) { return fs[j]; },
-                          has_x, xid, x.sum(), reinterpret_cast<V*>(s_stage + wid * SEG_WIN), wlane);
+                          has_x, xid, x.sum(), reinterpret_cast<V*>(s_stage + wid * SEG_WIN), lane);
     if (a.out_count)
       wave_stage_store<int64_t>(a.out_count + texcl, alo, ahi, omask, agg_id,
                                 [&](int j) { return (int64_t)rows_j(j); }, has_x, xid, (int64_t)x.n,
-                                reinterpret_cast<int64_t*>(s_stage + wid * SEG_WIN), wlane);
+                                reinterpret_cast<int64_t*>(s_stage + wid * SEG_WIN), lane);
     if (a.out_mean) {
       auto mean_j = [&](int j) { return mean_parts<V, MT>(fs[j], rows_j(j)); };
       wave_stage_store<MT>(a.out_mean + texcl, alo, ahi, omask, agg_id, mean_j, has_x, xid,
-                           mean_of<V, MT, MM>(x), reinterpret_cast<MT*>(s_stage + wid * SEG_WIN), wlane);
+                           mean_of<V, MT, MM>(x), reinterpret_cast<MT*>(s_stage + wid * SEG_WIN), lane);
     }
   }
   if (tid == 0) {
