@@ -60,10 +60,15 @@ def _alloc_out(fn, v, cap, dev):
     return torch.empty(cap, dtype=_mean_dtype(v), device=dev)
 
 
+_OUT_SLOT = {"sum": 0, "count": 1, "min": 2, "max": 3, "mean": 4}
+
+
 def _out_ptrs(fn, out):
-    p = {"sum": None, "count": None, "min": None, "max": None, "mean": None}
-    p[fn] = out
-    return [_lib.ptr(p[k]) for k in ("sum", "count", "min", "max", "mean")]
+    """The five output pointers of a single-aggregation call (ctypes turns the
+    ints and None of c_void_p parameters into pointers)."""
+    p = [None] * 5
+    p[_OUT_SLOT[fn]] = out.data_ptr()
+    return p
 
 
 # inputs of at least this many rows try outputs of _SMALL_CAP entries first
@@ -116,10 +121,10 @@ def _aggregate_one(k, kc, v, vorig, fn, n, dev, stream, digit_bits, want_keys):
     while True:
         out_keys = torch.empty(cap, dtype=k.dtype, device=dev) if want_keys else None
         out = _alloc_out(fn, v, cap, dev)
-        st = _lib.lib().jfx_groupby_ex(kc, _lib.ptr(k), vcode, _lib.ptr(v), n,
-                                       _lib.ptr(out_keys), *_out_ptrs(fn, out),
-                                       ctypes.byref(g), cap, digit_bits, _lib.ptr(ws),
-                                       ws.numel(), stream)
+        st = _lib.lib().jfx_groupby_ex(kc, k.data_ptr(), vcode, v.data_ptr(), n,
+                                       None if out_keys is None else out_keys.data_ptr(),
+                                       *_out_ptrs(fn, out), ctypes.byref(g), cap, digit_bits,
+                                       ws.data_ptr(), ws.numel(), stream)
         if st == _lib.JFX_ERR_CAPACITY and cap < n:
             cap = g.value
             continue
