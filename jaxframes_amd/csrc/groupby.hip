@@ -550,7 +550,7 @@ struct SegWs {
   uint32_t* dgid;   // [2048] group index of each digit (small-range path)
   void* part;       // [chunks][2048] chunk accumulators (small-range path)
   uint64_t* op_mm;  // [SORT_MAXCHUNKS * 2][2] one-pass: block min / max of the ordered keys
-  uint32_t* op_ctl; // [4] one-pass: [0] stop word
+  uint32_t* op_ctl; // [4] one-pass: [0] stop word, [1] residues per table
 };
 
 uint64_t seg_tiles(uint64_t n) { return (n + SEG_TILE - 1) / SEG_TILE; }
@@ -903,8 +903,8 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   const int rc = read_back_slot(rb);
   if (rc != JFX_OK) return rc;
   JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT, MM, TE><<<1, 1024, 0, s>>>(
-                                      l1, ng1, a.mm, grid, a.ctl, a.epoch, a.keys, n, oc,
-                                      rt_min, static_cast<K*>(out_keys), o, capacity, rb)));
+                                      l1, ng1, a.mm, grid, a.ctl, a.epoch, oc,
+                                      static_cast<K*>(out_keys), o, capacity, rb)));
   JFX_HIP(hipGetLastError());
   return read_back_wait(s, rb, 2 * sizeof(uint32_t), res);
 }

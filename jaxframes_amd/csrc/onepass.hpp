@@ -283,6 +283,8 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
   const uint32_t rt0 = op_residues<TE>(sspan);
   const uint32_t rt = rt0 > a.rt_min ? rt0 : a.rt_min;
   const uint32_t M = rt - 1u;
+  // every wave chose the same rt; the final merge reads it from ctl[1]
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctl[1] = rt;
   // lane group g holds residue r at entry (r ^ g) mod rt of its table: with
   // tables rt entries apart, an unswizzled residue would put the lanes of a
   // wave on a few LDS banks
@@ -532,9 +534,8 @@ k_onepass_merge(const OpPart<V, MM>* part, uint32_t nblocks, const uint32_t* ctl
 template <typename K, typename V, typename MT, bool MM, uint32_t TE>
 __global__ void __launch_bounds__(1024)
 k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint32_t nblocks,
-                const uint32_t* ctl, uint32_t epoch, const K* keys, uint64_t n, int oc,
-                uint32_t rt_min, K* out_keys, GroupOut<V, MT> out, uint64_t capacity,
-                ReadBackSlot rb) {
+                const uint32_t* ctl, uint32_t epoch, int oc, K* out_keys, GroupOut<V, MT> out,
+                uint64_t capacity, ReadBackSlot rb) {
   using T = typename OpT<V>::T;
   using U = typename ValTraits<V>::U;
   __shared__ K s_lo[16], s_hi[16];
@@ -545,17 +546,13 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
   __shared__ U s_min[MM ? TE : 1], s_max[MM ? TE : 1];
   __shared__ uint32_t s_gid[TE];
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
-  __shared__ uint32_t s_rt;
   const int tid = threadIdx.x;
   const int wid = tid >> 6;
   const uint32_t lane = lane_id();
   const bool stopped = ctl[0] == epoch;
-  // the table kernel's residues per table, from the same 64 keys
-  if (wid == 0) {
-    const K sspan = op_sample_span(keys, n, oc, lane);
-    const uint32_t rt0 = op_residues<TE>(sspan);
-    if (lane == 0) s_rt = rt0 > rt_min ? rt0 : rt_min;
-  }
+  // the table kernel's residues per table (loaded with the merge's loads, used
+  // after them)
+  const uint32_t rt = ctl[1];
   // block min / max loads first, so that they are in flight with the merge's
   K lo = ~K(0), hi = K(0);
   for (uint32_t b = tid; b < nblocks; b += 1024) {
@@ -591,7 +588,6 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
     lo = s_lo[w] < lo ? s_lo[w] : lo;
     hi = s_hi[w] > hi ? s_hi[w] : hi;
   }
-  const uint32_t rt = s_rt;
   const bool hit = !stopped && hi >= lo && hi - lo < (K)rt;
   if (!hit) {
     if (tid == 0) {
