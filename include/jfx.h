@@ -141,6 +141,28 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
                    void* out_max, void* out_mean, uint64_t* out_ngroups, uint64_t out_capacity,
                    int digit_bits, void* workspace, uint64_t ws_bytes, void* stream);
 
+/* jfx_groupby_ex without a workspace, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. It runs only the
+ * one-pass path of jfx_groupby_ex -- the same kernels with the same launch
+ * shapes, so its outputs are bit-identical to jfx_groupby_ex's whenever that
+ * call takes the one-pass path (n >= 2^16) -- and needs
+ *  - integer keys (JFX_U32/I32/U64/I64), any value type, 1 <= n < 2^30;
+ *  - a key span max - min below 2048 (below 1024 for float64 values with a
+ *    min or max output); a wider span returns JFX_ERR_UNSUPPORTED with
+ *    *out_ngroups = 0 and nothing written (use jfx_groupby_ex).
+ * Scratch: a library-owned 27 MiB device array per GPU (held until the
+ * process ends), shared by every call on that GPU, which keeps the last
+ * call's contents: calls that overlap on one GPU (other streams or threads)
+ * race and must be serialised by the caller. The call makes no hipMalloc /
+ * hipFree / hipHostMalloc; it synchronises the stream once per one-pass
+ * attempt (one or two) to read the result words with a stream-ordered
+ * copy. Output and capacity semantics are jfx_groupby_ex's. */
+int jfx_groupby_noalloc(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
+                        void* out_keys, void* out_sum, int64_t* out_count, void* out_min,
+                        void* out_max, void* out_mean, uint64_t* out_ngroups,
+                        uint64_t out_capacity, void* stream);
+
 /* ---- hashing and the local hash partition ----------------------------- */
 /* Row hash of 1..8 key columns (device pointers, one per column, n rows
  * each). Replaces hash_multi_columns (parallel_algorithms.py:69-136): one key

@@ -17,7 +17,9 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
+#include <vector>
 
 #include "blockscan.hpp"
 #include "runtime.hpp"
@@ -551,6 +553,10 @@ struct SegWs {
   void* part;       // [chunks][2048] chunk accumulators (small-range path)
   uint64_t* op_mm;  // [SORT_MAXCHUNKS * 2][2] one-pass: block min / max of the ordered keys
   uint32_t* op_ctl; // [4] one-pass: [0] stop word, [1] residues per table
+  uint64_t part_bytes = 0;  // bytes at part
+  // one-pass result words in device memory (allocation-free calls: read
+  // with a stream-ordered copy); null: the calling thread's pinned slot
+  uint32_t* op_res = nullptr;
 };
 
 uint64_t seg_tiles(uint64_t n) { return (n + SEG_TILE - 1) / SEG_TILE; }
@@ -567,7 +573,8 @@ SegWs carve_seg(Carver& c, int vb, uint64_t n) {
   w.ngroups = c.take<uint32_t>(4);
   w.seg_start = c.take<uint32_t>(2049);
   w.dgid = c.take<uint32_t>(2048);
-  w.part = c.take<char>((uint64_t)SORT_MAXCHUNKS * 2048 * st);
+  w.part_bytes = (uint64_t)SORT_MAXCHUNKS * 2048 * st;
+  w.part = c.take<char>(w.part_bytes);
   w.op_mm = c.take<uint64_t>((uint64_t)SORT_MAXCHUNKS * 2 * 2);
   w.op_ctl = c.take<uint32_t>(4);
   return w;
@@ -862,6 +869,7 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   a.n = n;
   a.oc = oc;
   a.epoch = ++g_op_epoch;
+  if (a.epoch == 0u) a.epoch = ++g_op_epoch;  // 0 is the value of fresh scratch memory
   a.rt_min = rt_min;
   a.part = static_cast<OpPart<V, MM>*>(w.part);
   a.mm = w.op_mm;
@@ -888,9 +896,7 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   // SORT_MAXCHUNKS x 2048 min / max states, more than both need)
   const uint32_t ng1 = (grid + OP_GROUP - 1) / OP_GROUP;
   JFX_CHECK(ng1 <= OP_L1, JFX_ERR_UNSUPPORTED, "one-pass grid exceeds the final merge");
-  const uint64_t part_bytes = (uint64_t)SORT_MAXCHUNKS * 2048 *
-                              (sizeof(V) == 4 ? sizeof(AggState<float>) : sizeof(AggState<double>));
-  JFX_CHECK(((uint64_t)grid + ng1) * TE * sizeof(OpPart<V, MM>) <= part_bytes,
+  JFX_CHECK(((uint64_t)grid + ng1) * TE * sizeof(OpPart<V, MM>) <= w.part_bytes,
             JFX_ERR_WORKSPACE, "one-pass partials exceed the partial buffer");
   OpPart<V, MM>* l1 = a.part + (size_t)grid * TE;
   static_assert(TE % OP_MT == 0, "level-1 merge blocks cover whole tables");
@@ -898,14 +904,25 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
                                       a.part, grid, a.ctl, a.epoch, l1)));
   JFX_HIP(hipGetLastError());
   // the final kernel publishes its result words to the host itself (no
-  // separate read-back kernel)
+  // separate read-back kernel), or to device words that are copied back
   ReadBackSlot rb{};
-  const int rc = read_back_slot(rb);
-  if (rc != JFX_OK) return rc;
+  if (w.op_res) {
+    rb.dst = w.op_res;
+    rb.seq_slot = reinterpret_cast<uint64_t*>(w.op_res + 2);
+    rb.seq = a.epoch;
+  } else {
+    const int rc = read_back_slot(rb);
+    if (rc != JFX_OK) return rc;
+  }
   JFX_LAUNCH("onepass_final", s, (k_onepass_final<K, V, MT, MM, TE><<<1, 1024, 0, s>>>(
                                       l1, ng1, a.mm, grid, a.ctl, a.epoch, oc,
                                       static_cast<K*>(out_keys), o, capacity, rb)));
   JFX_HIP(hipGetLastError());
+  if (w.op_res) {
+    JFX_HIP(hipMemcpyAsync(res, w.op_res, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    JFX_HIP(hipStreamSynchronize(s));
+    return JFX_OK;
+  }
   return read_back_wait(s, rb, 2 * sizeof(uint32_t), res);
 }
 
@@ -944,8 +961,71 @@ int onepass_key(int oc, int vt, const void* keys, const void* vals, uint64_t n, 
   return set_error(JFX_ERR_INVALID, "bad value type for the one-pass groupby");
 }
 
+// The one-pass try of jfx_groupby_ex: tables of 1024 residues, and on a miss
+// whose span seen so far still fits, one retry (1024 residues without lane
+// groups, or 2048). res[0] = OP_HIT / OP_MISS / OP_CAPACITY, res[1] = the
+// group count, the capacity needed or a lower bound of the key span.
+int onepass_try(int key_type, int val_type, const void* keys, const void* vals, uint64_t n,
+                const SegWs& w, void* out_keys, void* out_sum, int64_t* out_count,
+                void* out_min, void* out_max, void* out_mean, uint64_t capacity, uint32_t* res,
+                hipStream_t s) {
+  const SegCall oc{nullptr, order_code(key_type), vals, nullptr, n, out_keys, out_sum,
+                   out_count, out_min, out_max, out_mean, true, 0u};
+  const bool k4 = key_bytes(key_type) == 4;
+  auto attempt = [&](uint32_t te, uint32_t rt_min) -> int {
+    return k4 ? onepass_key<uint32_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc, capacity,
+                                      te, rt_min, res, s)
+              : onepass_key<uint64_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc, capacity,
+                                      te, rt_min, res, s);
+  };
+  res[0] = OP_MISS;
+  res[1] = 0u;
+  int rc = attempt(OP_TE, 16u);
+  if (rc != JFX_OK) return rc;
+  if (res[0] == OP_MISS && res[1] < 2 * OP_TE) {
+    const uint32_t te = res[1] < OP_TE ? OP_TE : 2 * OP_TE;
+    rc = attempt(te, te);
+  }
+  return rc;
+}
+
 }  // namespace
 
+}  // namespace jfx
+
+// ---- allocation-free one-pass groupby ------------------------------------------
+// Scratch of jfx_groupby_noalloc, one copy per device, held until the process
+// ends: the one-pass block and level-1 tables (at most 780 tables of 1024
+// residues x 32 bytes, 25.6 MB, for 64-bit integer values with min / max),
+// the block min / max, the control words and the result words. Host code uses
+// the address from hipGetSymbolAddress, never the array's name.
+constexpr uint64_t JFX_NA_PART_BYTES = 26ull << 20;
+constexpr uint64_t JFX_NA_SCRATCH_BYTES = 27ull << 20;
+static __device__ __align__(256) unsigned char scratch[JFX_NA_SCRATCH_BYTES];
+
+namespace jfx {
+namespace {
+
+// Device address of this device's scratch, resolved once per device with
+// that device current.
+int noalloc_scratch(void** out) {
+  static std::mutex mu;
+  static std::vector<void*> addr;
+  int dev = 0;
+  JFX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    JFX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    JFX_CHECK(p != nullptr, JFX_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return JFX_OK;
+}
+
+}  // namespace
 }  // namespace jfx
 
 extern "C" {
@@ -1034,27 +1114,12 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
   // a key span known to be at least 2048: the sort-based path will run
   bool wide = false;
   if (jfx::onepass_wanted(key_type, val_type, n)) {
-    // one read of keys and values when the key range turns out below 1024;
-    // on a miss the statistics-first path below runs
-    jfx::SegCall oc{nullptr, jfx::order_code(key_type), vals, nullptr, n, out_keys, out_sum,
-                    out_count, out_min, out_max, out_mean, true, 0u};
+    // one read of keys and values when the key range turns out below 1024
+    // (2048 on the retry); on a miss the statistics-first path below runs
     uint32_t res[2] = {0u, 0u};
-    auto attempt = [&](uint32_t te, uint32_t rt_min) -> int {
-      return kb == 4 ? jfx::onepass_key<uint32_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc,
-                                                  out_capacity, te, rt_min, res, s)
-                     : jfx::onepass_key<uint64_t>(oc.oc, val_type, keys, vals, n, w, out_keys, oc,
-                                                  out_capacity, te, rt_min, res, s);
-    };
-    int rc = attempt(jfx::OP_TE, 16u);
+    const int rc = jfx::onepass_try(key_type, val_type, keys, vals, n, w, out_keys, out_sum,
+                                    out_count, out_min, out_max, out_mean, out_capacity, res, s);
     if (rc != JFX_OK) return rc;
-    // one retry on a miss whose span seen so far still fits: 1024-residue
-    // tables without lane groups (the sample chose too narrow ones), or
-    // 2048-residue tables
-    if (res[0] == jfx::OP_MISS && res[1] < 2 * jfx::OP_TE) {
-      const uint32_t te = res[1] < jfx::OP_TE ? jfx::OP_TE : 2 * jfx::OP_TE;
-      rc = attempt(te, te);
-      if (rc != JFX_OK) return rc;
-    }
     if (res[0] == jfx::OP_HIT) {
       *out_ngroups = res[1];
       return JFX_OK;
@@ -1112,6 +1177,49 @@ int jfx_groupby_ex(int key_type, const void* keys, int val_type, const void* val
   rc = jfx::segreduce_dispatch(w, key_type, val_type, c, s);
   if (rc != JFX_OK) return rc;
   return jfx::read_ngroups(w, out_ngroups, s);
+}
+
+int jfx_groupby_noalloc(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
+                        void* out_keys, void* out_sum, int64_t* out_count, void* out_min,
+                        void* out_max, void* out_mean, uint64_t* out_ngroups,
+                        uint64_t out_capacity, void* stream) {
+  JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64, JFX_ERR_INVALID, "bad key type");
+  JFX_CHECK(jfx::int_key(key_type), JFX_ERR_UNSUPPORTED,
+            "the allocation-free groupby takes integer keys only");
+  JFX_CHECK(jfx::valid_vt(val_type), JFX_ERR_INVALID, "bad value type");
+  JFX_CHECK(n < (1ull << 30), JFX_ERR_UNSUPPORTED, "n must be below 2^30 rows");
+  JFX_CHECK(out_ngroups != nullptr, JFX_ERR_INVALID, "out_ngroups is null");
+  if (n == 0) {
+    *out_ngroups = 0;
+    return JFX_OK;
+  }
+  JFX_CHECK(keys != nullptr && vals != nullptr, JFX_ERR_INVALID, "null input");
+  void* base = nullptr;
+  int rc = jfx::noalloc_scratch(&base);
+  if (rc != JFX_OK) return rc;
+  jfx::Carver cv(base, JFX_NA_SCRATCH_BYTES);
+  jfx::SegWs w{};
+  w.part_bytes = JFX_NA_PART_BYTES;
+  w.part = cv.take<char>(JFX_NA_PART_BYTES);
+  w.op_mm = cv.take<uint64_t>((uint64_t)jfx::SORT_MAXCHUNKS * 2 * 2);
+  w.op_ctl = cv.take<uint32_t>(4);
+  w.op_res = cv.take<uint32_t>(4);
+  JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "allocation-free groupby scratch layout too large");
+  hipStream_t s = jfx::as_stream(stream);
+  uint32_t res[2] = {0u, 0u};
+  rc = jfx::onepass_try(key_type, val_type, keys, vals, n, w, out_keys, out_sum, out_count,
+                        out_min, out_max, out_mean, out_capacity, res, s);
+  if (rc != JFX_OK) return rc;
+  *out_ngroups = res[1];
+  if (res[0] == jfx::OP_HIT) return JFX_OK;
+  if (res[0] == jfx::OP_CAPACITY)
+    return jfx::set_error(JFX_ERR_CAPACITY, "outputs hold fewer entries than the groups this "
+                                            "call produces; *out_ngroups holds the capacity needed");
+  *out_ngroups = 0;
+  return jfx::set_error(JFX_ERR_UNSUPPORTED,
+                        "key span (max - min) is 2048 or more (1024 or more for float64 values "
+                        "with min / max): beyond the allocation-free groupby; use jfx_groupby_ex "
+                        "with a workspace");
 }
 
 // ---- batched groupby ---------------------------------------------------------
