@@ -16,11 +16,19 @@ Beside the step: the batched groupby (4096 frames in one HIP-graph replay)
 and the served stream (request loop over resident frames, Poisson arrivals)
 against their plain per-query paths, and the 2e8-key hash partition.
 
+Beside them, the other workloads at their full sizes, each with the
+algorithmic bytes and HIP-event time of every kernel it launches (achieved
+GB/s against the 8 TB/s peak per pass) and an on-device check of its result:
+the 100M-key uint64 radix sorts (uniform, and Zipf(1.1) over 10^6 distinct
+keys), the 1e8 x 1e7 inner join and the 2e8-key hash partition.
+
 Prints one JSON line with rows/s, per-step p50/p99, the roofline of the
 dominant kernel (HIP events recorded around that kernel's launches only, on
 its launch stream, inside the timed region; the per-kernel breakdown comes
-from an untimed pass with events around every launch), and a CPU baseline
-(the numpy restatement timed on a bounded sample on this host).
+from an untimed pass with events around every launch), the error of the
+timed step's result against the oracle on the same inputs, and a CPU
+baseline (the numpy restatement and pandas, timed on those inputs on this
+host).
 """
 
 from __future__ import annotations
@@ -62,11 +70,11 @@ def _kernel_bytes(name: str, n: int, kb: int, vb: int) -> float | None:
     return None
 
 
-def _traffic(kernel: str):
-    """Measured HBM bytes per launch from the committed rocprof PMC summary,
-    if one exists for this kernel and workload."""
+def _traffic(kernel: str, workload: str = "groupby_sum_100M"):
+    """Measured HBM bytes per launch from the committed rocprof PMC summary
+    profiles/<workload>_traffic.json, if one exists for this kernel."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "groupby_sum_100M_traffic.json")
+                        f"{workload}_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
@@ -75,18 +83,49 @@ def _traffic(kernel: str):
         return None
 
 
-def cpu_baseline(n_sample: int, seed: int):
-    """The numpy restatement of the reference groupby (oracle) on a bounded
-    sample of the same distribution, single-threaded numpy."""
+def cpu_baseline(k: np.ndarray, v: np.ndarray, gpu_keys: np.ndarray, gpu_sums: np.ndarray):
+    """The numpy restatement of the reference groupby (the oracle) and pandas,
+    each timed on the host over the same rows the GPU step aggregated
+    (single-threaded numpy semantics). The oracle's output is also the
+    checker of the GPU result: group keys must be equal and the float32 sums
+    are compared with its float64 sums (max relative error per group, and
+    max error over the group's sum of |x|)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import oracle
-    rng = np.random.default_rng(seed)
-    k = rng.integers(0, 1000, n_sample, dtype=np.int32)
-    v = rng.standard_normal(n_sample, dtype=np.float32)
+    import pandas as pd
+    n = k.shape[0]
     t0 = time.perf_counter()
-    oracle.groupby_aggregate(k, {"v": v}, {"v": "sum"})
+    ek, eres = oracle.groupby_aggregate(k, {"v": v}, {"v": "sum"})
     dt = time.perf_counter() - t0
-    return n_sample / dt, dt
+    df = pd.DataFrame({"k": k, "v": v})
+    t1 = time.perf_counter()
+    ps = df.groupby("k", sort=True)["v"].sum()
+    pdt = time.perf_counter() - t1
+    keys_equal = bool(ek.shape == gpu_keys.shape and np.array_equal(ek, gpu_keys))
+    ref = np.asarray(eres["v"], np.float64)
+    kmin = int(ek[0]) if ek.shape[0] else 0
+    absx = np.bincount(k.astype(np.int64) - kmin, weights=np.abs(v.astype(np.float64)))
+    absx = absx[ek.astype(np.int64) - kmin]
+    err = np.abs(gpu_sums.astype(np.float64) - ref) if keys_equal else np.array([np.inf])
+    rel = err / np.maximum(np.abs(ref), np.finfo(np.float64).tiny) if keys_equal else err
+    nrm = err / np.maximum(absx, np.finfo(np.float64).tiny) if keys_equal else err
+    pandas_keys_equal = bool(np.array_equal(ps.index.to_numpy(), ek))
+    check = {"vs": "oracle (numpy restatement, float64 sums) on the timed inputs",
+             "keys_equal": keys_equal, "groups": int(ek.shape[0]),
+             "max_rel_err": float(rel.max()), "max_norm_err": float(nrm.max()),
+             "tolerance": {"max_norm_err": 2e-7, "max_rel_err_where_not_cancelling": 1e-6}}
+    base = {"value": round(n / dt, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"the {n} timed rows (int32 keys in [0,1000), float32 values), "
+                      f"numpy restatement of SortBasedGroupBy.groupby_aggregate ({dt:.1f} s)",
+            "pandas": {"value": round(n / pdt, 1), "unit": "rows/s",
+                       "what": "DataFrame.groupby('k')['v'].sum() on the same rows",
+                       "seconds": round(pdt, 3), "keys_equal_oracle": pandas_keys_equal},
+            "versions": {"numpy": np.__version__, "pandas": pd.__version__,
+                         "python": platform.python_version()},
+            "threads": {v_: os.environ.get(v_) for v_ in
+                        ("OMP_NUM_THREADS", "MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS")},
+            "host_cpu": _cpu_model(), "os_cpu_count": os.cpu_count()}
+    return base, check
 
 
 def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 256):
@@ -114,6 +153,20 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
     dt = (time.perf_counter() - t0) / reps
     groups = int(plan.result().keys.shape[0])
     digit_bits, pipelines = plan.digit_bits, (1 if plan.parts is None else len(plan.parts))
+    cb = 4 if (plan.parts or [plan])[0].total_bits <= 32 else 8
+
+    def bytes_of(name, launches):
+        # int32 keys and float32 values; composite keys of cb bytes
+        if name == "batch_compose":
+            return n * (4 + cb)
+        if name.startswith("onesweep"):
+            return launches * 2 * n * (cb + 4) // pipelines
+        if name == "segreduce_tiles":
+            return n * (cb + 4) + groups * (cb + 4 + 8)
+        return None
+
+    # per-kernel HIP-event times of eager launches of the same pipeline
+    eager = _measure(lambda: plan.launch(graph=False), 3, bytes_of)
     # the plain path is timed without the plan's buffers held by the allocator
     del plan
     torch.cuda.empty_cache()
@@ -138,7 +191,11 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
            "plain_rows_per_s": round(plain_rps, 1),
            "plain_sample": f"first {plain_frames} frames, {rows_plain} rows",
            "speedup_vs_plain": round(n / dt / plain_rps, 2),
-           "hip_graph": True, "digit_bits": digit_bits, "pipelines": pipelines}
+           "hip_graph": True, "digit_bits": digit_bits, "pipelines": pipelines,
+           "eager_ms_per_batch": eager["ms_per_call"],
+           "alg_bytes_per_batch": eager["alg_bytes_per_call"],
+           "batch_hbm_frac": round(eager["alg_bytes_per_call"] / dt / 1e9 / PEAK_HBM_GBPS, 4),
+           "roofline": eager["roofline"], "kernels": eager["kernels"]}
     del keys, vals
     torch.cuda.empty_cache()
     return out
@@ -218,44 +275,215 @@ def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
                                "keys_per_s": round(2e8 / pdt, 1)}}
 
 
-def only_workload(name: str, reps: int, batched_rows: int = 400_000_000) -> dict:
-    """One workload by itself, for profiling: the kernels it launches, with
-    their HIP-event time per call, and the wall time per call."""
+def _ordered_host(t) -> np.ndarray:
+    """Host copy of an integer key column as unsigned integers in key order."""
     import torch
-    from jaxframes_amd import _lib, datagen, joining, partition, sorting
+    if t.dtype in (torch.int64, torch.uint64):
+        return t.view(torch.int64).cpu().numpy().view(np.uint64) ^ np.uint64(
+            0 if t.dtype == torch.uint64 else 1 << 63)
+    return t.view(torch.int32).cpu().numpy().view(np.uint32) ^ np.uint32(
+        0 if t.dtype == torch.uint32 else 1 << 31)
+
+
+def _varying_digits(t, bits: int = 8) -> int:
+    """Digits of `bits` bits whose value differs between keys: the radix
+    sort runs one pass per such digit (a digit equal in every key is
+    skipped), so this is the pass count of the sort of t."""
+    o = _ordered_host(t)
+    x = int(np.bitwise_or.reduce(o) ^ np.bitwise_and.reduce(o))
+    nb = o.dtype.itemsize * 8
+    return sum(1 for d in range(0, nb, bits) if (x >> d) & ((1 << bits) - 1))
+
+
+def _sort_pass_bytes(n: int, kb: int, pb: int, passes: int) -> int:
+    """Algorithmic bytes of `passes` scatter passes sorting n keys of kb bytes
+    that carry a pb-byte payload the first pass creates (the row index):
+    the first pass reads the keys, every pass writes keys and payload, and
+    each later pass reads both back."""
+    if passes <= 0:
+        return 0
+    return n * kb + passes * n * (kb + pb) + (passes - 1) * n * (kb + pb)
+
+
+def _measure(step, reps: int, bytes_of, traffic_key: str | None = None) -> dict:
+    """Wall time per call of step (no events), then the HIP-event time of
+    every kernel launch over reps more calls; bytes_of(name, launches) gives
+    a kernel's algorithmic bytes per call (None: not priced)."""
+    import torch
+    from jaxframes_amd import _lib
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    with _lib.KernelTimer() as kt:
+        for _ in range(reps):
+            step()
+        torch.cuda.synchronize()
+    kernels = {}
+    for name, (ms, cnt) in sorted(kt.stats().items(), key=lambda kv: -kv[1][0]):
+        us = ms / reps * 1e3
+        launches = max(cnt // reps, 1)
+        e = {"us_per_call": round(us, 1), "launches_per_call": launches}
+        b = bytes_of(name, launches)
+        if b:
+            gbps = b / (us * 1e-6) / 1e9
+            e.update(alg_bytes_per_call=int(b), achieved_GBps=round(gbps, 1),
+                     frac=round(gbps / PEAK_HBM_GBPS, 4))
+        kernels[name] = e
+    dom = max(kernels, key=lambda k: kernels[k]["us_per_call"])
+    de = kernels[dom]
+    roof = None
+    if "alg_bytes_per_call" in de:
+        roof = {"bound": "hbm", "kernel": dom, "achieved": de["achieved_GBps"],
+                "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": de["frac"],
+                "traffic": _traffic(dom, traffic_key) if traffic_key else None,
+                "avg_launch_us": round(de["us_per_call"] / de["launches_per_call"], 1),
+                "alg_bytes_per_launch": de["alg_bytes_per_call"] // de["launches_per_call"]}
+    total = sum(e.get("alg_bytes_per_call", 0) for e in kernels.values())
+    return {"ms_per_call": round(wall * 1e3, 3), "reps": reps, "alg_bytes_per_call": total,
+            "call_hbm_frac": round(total / wall / 1e9 / PEAK_HBM_GBPS, 4),
+            "roofline": roof, "kernels": kernels}
+
+
+def workload_sort(kind: str, reps: int, n: int = 100_000_000) -> dict:
+    """Radix sort of n uint64 keys (uniform, or Zipf(1.1) over 10^6 distinct
+    values) carrying the uint32 row index; the result is checked on the
+    device: the index is a permutation, keys_out = keys_in[index], keys_out
+    is non-decreasing and equal keys keep increasing row indices."""
+    import torch
+    from jaxframes_amd import datagen, sorting
+    keys = datagen.sort_keys(n, kind=kind)
+    passes = _varying_digits(keys)
+
+    def step():
+        return sorting.radix_sort(keys, None, True, return_index=True, index_dtype=torch.int32)
+
+    def bytes_of(name, launches):
+        if name.startswith("sort_stats") or name == "sort_hist":
+            return n * 8 * launches
+        if name.startswith("onesweep"):
+            return _sort_pass_bytes(n, 8, 4, launches)
+        return None
+
+    out = {"workload": f"radix sort of {n} {kind} uint64 keys with the uint32 row index"
+                       + (" (Zipf s=1.1 over 10^6 distinct values)" if kind == "zipf" else ""),
+           "rows": n, "passes": passes, **_measure(step, reps, bytes_of, f"sort_u64_{kind}_100M")}
+    out["rows_per_s"] = round(n / (out["ms_per_call"] / 1e3), 1)
+    sk, _, idx = step()
+    sign = -(1 << 63)
+    ok = sk.view(torch.int64) ^ sign
+    il = idx.long()
+    ties = ok[1:] == ok[:-1]
+    out["check"] = {
+        "sorted": bool((ok[1:] >= ok[:-1]).all()),
+        "permutation": bool(torch.bincount(il, minlength=n).max().item() == 1),
+        "gather_equal": bool(torch.equal(keys.view(torch.int64)[il], sk.view(torch.int64))),
+        "ties": int(ties.sum()),
+        "ties_in_row_order": bool((il[1:][ties] > il[:-1][ties]).all())}
+    return out
+
+
+def workload_join(reps: int) -> dict:
+    """Inner join of 1e8 x 1e7 int64 keys (right keys twice each, 80% of the
+    left rows matching by Zipf(1.2) rank), two float32 payloads per side;
+    checked: the pair count and non-decreasing output keys."""
+    import torch
+    from jaxframes_amd import datagen, joining
+    lk, lv, rk, rv, m = datagen.join_tables()
+    L, R = lk.shape[0], rk.shape[0]
+    pl, pr = _varying_digits(lk), _varying_digits(rk)
+
+    def step():
+        return joining.sort_merge_join(lk, lv, rk, rv, "inner")
+
+    def bytes_of(name, launches):
+        if name.startswith("sort_stats") or name == "sort_hist":
+            return (L + R) * 8
+        if name.startswith("onesweep"):
+            if launches != pl + pr:
+                return None
+            return _sort_pass_bytes(L, 8, 4, pl) + _sort_pass_bytes(R, 8, 4, pr)
+        if name == "join_count":
+            return (L + R) * 8
+        if name == "join_fill":
+            # sorted keys and row indices of both sides, each side's first
+            # payload read once; keys, two payloads and both row columns
+            # (for the second payloads) written per pair
+            return (L + R) * (8 + 4 + 4) + m * (8 + 4 + 4 + 4 + 4)
+        if name == "gather":
+            return launches * m * (4 + 4 + 4)
+        return None
+
+    out = {"workload": f"inner join of {L} x {R} int64 keys (right keys twice each, 80% of left "
+                       f"rows match by Zipf(1.2)), 2 float32 payloads per side, {m} pairs",
+           "rows": L + R, "pairs": m, "passes": [pl, pr],
+           **_measure(step, reps, bytes_of, "join_1e8x1e7")}
+    out["rows_per_s"] = round((L + R) / (out["ms_per_call"] / 1e3), 1)
+    keys, cols = step()
+    out["check"] = {"pairs": int(keys.shape[0]), "pairs_expected": m,
+                    "keys_sorted": bool((keys[1:] >= keys[:-1]).all()),
+                    "columns": sorted(cols)}
+    return out
+
+
+def _fmix32_bucket(k, nb: int):
+    """fmix32 of the low 32 bits of int64 keys, mod nb (int64 arithmetic)."""
+    m = 0xFFFFFFFF
+    h = k & m
+    h = h ^ (h >> 16)
+    h = (h * 0x85EBCA6B) & m
+    h = h ^ (h >> 13)
+    h = (h * 0xC2B2AE35) & m
+    h = h ^ (h >> 16)
+    return h % nb
+
+
+def workload_partition(reps: int, n: int = 200_000_000, nb: int = 256) -> dict:
+    """Hash partition of n uniform int64 keys into nb buckets (keys only);
+    checked on the device: the offsets are the bucket counts of fmix32(low 32
+    bits) mod nb, and the partitioned keys' buckets are non-decreasing."""
+    import torch
+    from jaxframes_amd import datagen, partition
+    keys = datagen.partition_keys(n)
+
+    def step():
+        return partition.hash_partition(keys, nb, return_keys=True, return_rows=False)
+
+    def bytes_of(name, launches):
+        if name == "part_hist":
+            return n * 8
+        if name.startswith("part_scatter"):
+            return n * 16
+        return None
+
+    out = {"workload": f"hash partition of {n} uniform int64 keys into {nb} buckets "
+                       "(fmix32), keys only",
+           "rows": n, **_measure(step, reps, bytes_of, "hash_partition_2e8")}
+    out["rows_per_s"] = round(n / (out["ms_per_call"] / 1e3), 1)
+    r = step()
+    b = _fmix32_bucket(r["keys"], nb)
+    cnt = torch.bincount(_fmix32_bucket(keys, nb), minlength=nb)
+    out["check"] = {"offsets_equal_counts": bool(torch.equal(r["offsets"][1:] - r["offsets"][:-1], cnt)),
+                    "buckets_sorted": bool((b[1:] >= b[:-1]).all())}
+    return out
+
+
+def only_workload(name: str, reps: int, batched_rows: int = 400_000_000) -> dict:
+    """One workload by itself (for profiling)."""
     if name == "batched":
         return {"only": name, **batched_vs_plain(batched_rows or None)}
     if name == "served":
         return {"only": name, **served_vs_plain()}
-    if name == "sort":
-        keys = datagen.sort_keys(100_000_000)
-        desc = "radix sort of 100M uniform uint64 keys with the uint32 row index"
-        step = lambda: sorting.radix_sort(keys, None, True, return_index=True,
-                                          index_dtype=torch.int32)
-    elif name == "join":
-        lk, lv, rk, rv, m = datagen.join_tables()
-        desc = (f"inner join of 1e8 x 1e7 int64 keys (right keys twice each, 80% of left rows "
-                f"match by Zipf(1.2)), 2 float32 payloads per side, {m} pairs")
-        step = lambda: joining.sort_merge_join(lk, lv, rk, rv, "inner")
-    elif name == "partition":
-        keys = datagen.partition_keys(200_000_000)
-        desc = "hash partition of 2e8 uniform int64 keys into 256 buckets (fmix32), keys only"
-        step = lambda: partition.hash_partition(keys, 256, return_keys=True, return_rows=False)
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    step()
-    torch.cuda.synchronize()
-    with _lib.KernelTimer() as kt:
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            step()
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / reps
-    ks = kt.stats()
-    return {"only": name, "workload": desc, "ms_per_call": round(dt * 1e3, 3), "reps": reps,
-            "kernels": {k: {"us_per_call": round(v[0] / reps * 1e3, 1),
-                            "launches_per_call": v[1] // reps}
-                        for k, v in sorted(ks.items(), key=lambda kv: -kv[1][0])}}
+    if name in ("sort", "sort_zipf"):
+        return {"only": name, **workload_sort("zipf" if name == "sort_zipf" else "uniform", reps)}
+    if name == "join":
+        return {"only": name, **workload_join(reps)}
+    if name == "partition":
+        return {"only": name, **workload_partition(reps)}
+    raise SystemExit(f"unknown workload {name}")
 
 
 def main() -> int:
@@ -271,7 +499,10 @@ def main() -> int:
                     help="rows of the batched groupby (the 4096 frames' row counts are scaled "
                          "to it); 0 = the unscaled distribution (about 2e9 rows)")
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--only", choices=("sort", "join", "partition", "batched", "served"),
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the sort, join and partition workloads")
+    ap.add_argument("--only", choices=("sort", "sort_zipf", "join", "partition", "batched",
+                                       "served"),
                     help="run one other workload by itself (for profiling) and print its line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 400 MB (float32[1e8]) so that PMC byte "
@@ -370,18 +601,32 @@ def main() -> int:
             ran.pop(k)
     step_bytes = sum((_kernel_bytes(k, n, 4, 4) or 0) * (v[1] // nb) for k, v in ran.items())
 
-    sort_based = n * 4 + 2 * n * 8 + n * 8 + ngroups * (4 + 16)
-    batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows or None)
-    served_line = None if args.no_served else served_vs_plain()
-
-    cpu = None
+    # the timed step's inputs and result, copied to the host after the timed
+    # region for the CPU baseline and the check against the oracle
+    cpu, check = None, None
     if not args.no_cpu_baseline:
         sample = min(args.cpu_sample, n)
-        rps, dt = cpu_baseline(sample, seed=1234)
-        cpu = {"value": round(rps, 1), "unit": "rows/s", "cores": 1, "kind": "port",
-               "sample": f"{sample} rows, same distribution, numpy restatement of "
-                         f"SortBasedGroupBy.groupby_aggregate ({dt:.1f} s)",
-               "host_cpu": _cpu_model(), "os_cpu_count": os.cpu_count()}
+        hk = keys[:sample].cpu().numpy()
+        hv = vals[:sample].cpu().numpy()
+        if sample == n:
+            gk, gs = uk.cpu().numpy(), res["v"].cpu().numpy()
+        else:
+            uk2, res2 = grouping.groupby_aggregate(keys[:sample], {"v": vals[:sample]},
+                                                   {"v": "sum"})
+            gk, gs = uk2.cpu().numpy(), res2["v"].cpu().numpy()
+        cpu, check = cpu_baseline(hk, hv, gk, gs)
+        del hk, hv
+
+    workloads = None
+    if not args.no_workloads:
+        reps = max(3, min(args.steps // 4, 10))
+        workloads = {"sort_u64_uniform": workload_sort("uniform", reps),
+                     "sort_u64_zipf": workload_sort("zipf", reps),
+                     "join": workload_join(reps),
+                     "hash_partition": workload_partition(reps)}
+        torch.cuda.empty_cache()
+    batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows or None)
+    served_line = None if args.no_served else served_vs_plain()
 
     line = {
         "metric": "groupby-sum rows/s + % HBM peak at 100M rows; batched vs plain; p50/p99 ms",
@@ -405,12 +650,9 @@ def main() -> int:
                            "key range checked after"},
         "step_alg_bytes": step_bytes,
         "step_hbm_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / PEAK_HBM_GBPS, 4),
-        # the same step priced at the sort-based groupby's algorithmic bytes:
-        # key histogram N*kb + one LSD pass 2N(kb+vb) + segmented-reduce read
-        # N(kb+vb) + group outputs G*(kb+16); this path reads N(kb+vb) once
-        "step_bytes_sort_based": sort_based,
-        "step_frac_sort_based": round(sort_based / (ms_per_step / 1e3) / 1e9 / PEAK_HBM_GBPS, 4),
         "roofline": roof,
+        "check": check,
+        "workloads": workloads,
         "batched": batched_line,
         "served": served_line,
         "kernels": kernels,

@@ -11,7 +11,8 @@
  *    returns a thread-local message for the last failure on this thread.
  *  - All data pointers are device pointers owned by the caller. The library
  *    never allocates device memory: scratch lives in a caller-provided
- *    workspace whose size comes from the matching *_workspace_size call.
+ *    workspace whose size comes from the matching *_workspace_size call
+ *    (jfx_groupby_noalloc alone uses a library-owned static device array).
  *  - `stream` is a hipStream_t (NULL = default stream). Calls enqueue work on
  *    it. Calls whose output size is data dependent (they write the size to a
  *    host pointer) synchronise that stream once, as the reference does with
@@ -269,7 +270,9 @@ int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void
  * group by their order-preserving image, NaN last (nan_start[g] = first NaN
  * row of group g; ninf_end[g] = end of its leading -inf rows; pinf_start[g] =
  * its first +inf row) -- and prefix[] = float64 prefix sums of those values
- * (NaN and +-inf as 0, n + 1 entries): count, sum and mean of the group's rows with
+ * restarted at every group (n + 1 entries, NaN and +-inf as 0: prefix[i + 1]
+ * = sum of the values of row i's group up to row i, so a group's sums never
+ * carry the rounding of other groups' values): count, sum and mean of the group's rows with
  * value <cmp> *value (JFX_CMP_*; IEEE semantics, a NaN value passes only !=
  * and makes its group's sum NaN), found by binary searches and prefix-sum
  * differences; out_mask[g] = 1 where the count is nonzero. The result of
@@ -375,6 +378,17 @@ int jfx_compact_rows(const uint8_t* mask, uint64_t n, uint32_t* out_rows, uint64
 int jfx_filter_rows(int col_type, const void* col, int cmp, const void* value, uint64_t n,
                     uint32_t* out_rows, uint64_t* out_count, void* workspace, uint64_t ws_bytes,
                     void* stream);
+
+/* ---- HBM bandwidth probes ---------------------------------------------- */
+/* No counterpart in the reference: the read and copy ceilings the kernels'
+ * achieved bandwidth is compared with. mode 0 streams `bytes` of src with
+ * 16-byte loads (`unroll` = 1, 2, 4 or 8 loads in flight per thread, a grid
+ * of `blocks` workgroups of 256 threads, non-temporal loads when
+ * nontemporal != 0) and stores one word per workgroup to dst (dst holds
+ * `blocks` uint32); mode 1 copies src to dst (both `bytes`). Buffers and
+ * bytes 16-byte aligned. Asynchronous on `stream`. */
+int jfx_bandwidth_probe(int mode, const void* src, void* dst, uint64_t bytes, uint32_t blocks,
+                        uint32_t unroll, int nontemporal, void* stream);
 
 /* ---- per-kernel timing ------------------------------------------------ */
 /* No counterpart in the reference. When enabled, every kernel launch is

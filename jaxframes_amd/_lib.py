@@ -93,6 +93,8 @@ _SIGNATURES = {
                     c_int, c_vp, c_u64, c_vp],
     "jfx_groupby_ex": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64p,
                        c_u64, c_int, c_vp, c_u64, c_vp],
+    "jfx_bandwidth_probe": [c_int, c_vp, c_vp, c_u64, ctypes.c_uint32, ctypes.c_uint32, c_int,
+                            c_vp],
     "jfx_groupby_noalloc": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_u64p, c_u64, c_vp],
 }

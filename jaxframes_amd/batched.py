@@ -220,7 +220,8 @@ class BatchedGroupBy:
             self._launch()
 
     def result(self) -> BatchResult:
-        """Read the group count (one sync) and split the composite keys."""
+        """Read the group count (one sync), check the device error word (raises
+        JfxError when a kernel set it) and split the composite keys."""
         if self.parts is not None:
             rs = [p.result() for p in self.parts]
             offs, base = [torch.zeros(1, dtype=torch.int64, device=self.dev)], 0
@@ -230,6 +231,14 @@ class BatchedGroupBy:
             return BatchResult(torch.cat(offs), torch.cat([r.keys for r in rs]),
                                {a: torch.cat([r.values[a] for r in rs]) for a in self.aggs})
         G = int(self.ngroups.item())
+        # a key outside the declared range, or a bounded wait that ran out,
+        # raised the device error word: the groups are not valid
+        code = _lib.device_error()
+        if code:
+            raise _lib.JfxError(
+                f"batched groupby: device error {code:#x}"
+                + (" (a key lies outside the declared key_range)" if code & _lib.DEVERR_BOUNDS else "")
+                + (" (a cross-workgroup wait timed out)" if code & _lib.DEVERR_LOOKBACK else ""))
         comp = self.out_keys[:G]
         c64 = (comp.long() & 0xFFFFFFFF) if self.comp_dtype == torch.int32 else comp
         kmask = (1 << self.key_bits) - 1

@@ -3,13 +3,15 @@
 // A resident frame is indexed once: rows ordered by (key, value) -- groups
 // contiguous, values ascending inside each group in the order-preserving
 // unsigned image (NaN last) -- and the float64 prefix sums of the ordered
-// values (NaN and infinities counted as 0, infinities counted per group). A query `groupby(key).agg(sum, count, mean) of
+// values restarted at every group (NaN and infinities counted as 0,
+// infinities counted per group). A query `groupby(key).agg(sum, count, mean) of
 // the rows with value <cmp> t` then needs, per group, one or two binary
 // searches for t inside the group's values and a difference of two prefix
 // sums: the rows that pass form at most two runs of the group. The result of
 // a group is the exact sum of its passing float32 values up to the float64
-// prefix rounding (<= ~2^-52 * sum |x| over the group), rounded to the value
-// type; groups where nothing passes are dropped by the caller (mask output).
+// rounding of the group's own prefix sums (about log2(rows) * 2^-53 *
+// sum |x| over the group, independent of the other groups), rounded to the
+// value type; groups where nothing passes are dropped by the caller (mask output).
 // Semantics are those of filtering the rows and running
 // SortBasedGroupBy.groupby_aggregate (parallel_algorithms.py:733-803) on them:
 // IEEE comparisons (a NaN value passes only !=, and then makes its group's sum
@@ -27,7 +29,7 @@ namespace jfx {
 template <typename V>
 struct ServedArgs {
   const V* vals;              // values in (key, value) order
-  const double* prefix;       // n + 1 prefix sums of vals (NaN and +-inf as 0)
+  const double* prefix;       // n + 1 per-group prefix sums of vals (NaN and +-inf as 0)
   const uint32_t* starts;     // G + 1 group starts
   const uint32_t* nan_start;  // G: first NaN row of each group (its end if none)
   const uint32_t* ninf_end;   // G: end of the group's leading -inf rows
@@ -110,9 +112,12 @@ __device__ __forceinline__ SvEval served_eval(const V* vals, const double* prefi
   const uint32_t nn = nan_pass ? hi - hn : 0u;
   SvEval e;
   e.cnt = (uint64_t)(r1 - r0) + (uint64_t)(r3 - r2) + nn;
-  // finite rows through the prefix sums; -inf rows lead and +inf rows end the
-  // group's non-NaN run, so the passing runs' overlaps with them count them
-  double s = (prefix[r1] - prefix[r0]) + (prefix[r3] - prefix[r2]);
+  // finite rows through the group's prefix sums (prefix[i] = sum of the
+  // group's rows before row i for i > lo; 0 at the group's first row);
+  // -inf rows lead and +inf rows end the group's non-NaN run, so the passing
+  // runs' overlaps with them count them
+  auto pre = [&](uint32_t i) -> double { return i > lo ? prefix[i] : 0.0; };
+  double s = (pre(r1) - pre(r0)) + (pre(r3) - pre(r2));
   const uint32_t ne = ninf_end[g], ps = pinf_start[g];
   auto overlap = [](uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
     const uint32_t b = x0 > y0 ? x0 : y0, en = x1 < y1 ? x1 : y1;

@@ -53,8 +53,8 @@ def sort_keys(n: int, kind: str = "uniform", seed: int = 1234, device="cuda"):
     if kind == "uniform":
         return _u64(n, g, device)
     if kind == "zipf":
-        distinct = _u64(10**6, g, device)
-        return distinct[zipf_ranks(n, 10**6, 1.1, g, device)]
+        distinct = _u64(10**6, g, device).view(torch.int64)
+        return distinct[zipf_ranks(n, 10**6, 1.1, g, device)].view(torch.uint64)
     raise ValueError(kind)
 
 

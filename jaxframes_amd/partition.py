@@ -107,13 +107,40 @@ def partition_columns(keys, values: dict, nbuckets: int):
     return pk, vals, r["offsets"]
 
 
-def bucketize_padded(column, offsets: torch.Tensor, rows: torch.Tensor, pad_value):
+_INT_PAD = -999999
+
+
+def logical_pad_value(dtype_name, column: torch.Tensor):
+    """The pad value of a column in the padded bucket layout, by the rule of
+    _logical_pad_value (parallel_algorithms.py:150-160): dictionary-encoded
+    string codes (dtype_name "string") -1, floats NaN, integers -999999, bools
+    False, anything else 0 -- as a Python scalar of the column's dtype
+    (-999999 wraps modulo 2^bits in unsigned and narrow integer columns, as a
+    C cast does)."""
+    dt = column.dtype
+    if dtype_name == "string":
+        return torch.tensor(-1, dtype=torch.int64).to(dt).item()
+    if dt.is_floating_point:
+        return float("nan")
+    if dt == torch.bool:
+        return False
+    if not dt.is_complex:
+        return torch.tensor(_INT_PAD, dtype=torch.int64).to(dt).item()
+    return 0
+
+
+def bucketize_padded(column, offsets: torch.Tensor, rows: torch.Tensor, pad_value=None,
+                     dtype_name=None):
     """The reference's padded per-destination layout of one column
     (_bucketize_local_column, parallel_algorithms.py:163-182): [P, n] with row
     r of bucket d at [d, r] and pad_value elsewhere, built from the compact
-    layout of hash_partition (offsets, rows). For inputs small enough that a
-    P x n buffer is reasonable; the compact layout is what the exchange sends."""
+    layout of hash_partition (offsets, rows). pad_value None takes the
+    per-dtype rule of logical_pad_value (dtype_name "string" for string code
+    columns). For inputs small enough that a P x n buffer is reasonable; the
+    compact layout is what the exchange sends."""
     col = to_device(column)
+    if pad_value is None:
+        pad_value = logical_pad_value(dtype_name, col)
     n = col.shape[0]
     P = offsets.shape[0] - 1
     counts = offsets[1:] - offsets[:-1]

@@ -40,7 +40,8 @@ class Frame:
     - key order (stable): sorted keys and the row permutation, reused by joins
       (pairs then come in the reference's (key, left row, right row) order);
     - (key, value) order for float values: the values ascending inside each
-      key group (NaN last), float64 prefix sums of them, group starts, the
+      key group (NaN last), float64 prefix sums of them restarted at every
+      group (segmented_prefix), group starts, the
       first NaN row of each group and the group keys, which answer
       filter -> groupby by binary search (jfx_served_groups).
     """
@@ -88,7 +89,7 @@ class Frame:
             fin = torch.where(torch.isfinite(self.kv_values), self.kv_values,
                               torch.zeros_like(self.kv_values))
             self.prefix = torch.zeros(self.n + 1, dtype=torch.float64, device=dev)
-            self.prefix[1:] = torch.cumsum(fin.double(), 0)
+            self.prefix[1:] = segmented_prefix(fin.double(), self.group_starts)
             # the index columns in jfx_served_query field order
             self.index_ptrs = (self.kv_values.data_ptr(), self.prefix.data_ptr(),
                                self.group_starts.data_ptr(), self.group_nan_start.data_ptr(),
@@ -97,6 +98,33 @@ class Frame:
             parts += [self.kv_values, self.group_nan_start, self.group_pinf_start,
                       self.group_ninf_end, self.prefix]
         self.nbytes = sum(t.numel() * t.element_size() for t in parts)
+
+
+def segmented_prefix(x: torch.Tensor, starts: torch.Tensor) -> torch.Tensor:
+    """Inclusive prefix sums of x restarted at every group (groups are the
+    row ranges [starts[g], starts[g + 1])): out[i] = sum of x[starts[g] .. i]
+    for the group g of row i. A log-step (Hillis-Steele) scan whose adds
+    only ever combine partial sums of one group, so each result's rounding
+    error is bounded by the group's own values (about log2(group rows) *
+    2^-53 * sum |x| over the group), whatever the other groups hold; a
+    global cumsum differenced at the group start would carry the rounding of
+    the running total of every earlier group."""
+    n = x.shape[0]
+    if n == 0:
+        return x.clone()
+    st = starts.long()
+    sizes = st[1:] - st[:-1]
+    row_start = torch.repeat_interleave(st[:-1], sizes)
+    pos = torch.arange(n, device=x.device) - row_start     # row's offset in its group
+    out = x.clone()
+    d = 1
+    longest = int(sizes.max())
+    while d < longest:
+        add = torch.zeros_like(out)
+        add[d:] = out[:-d]
+        out = out + torch.where(pos >= d, add, torch.zeros_like(add))
+        d *= 2
+    return out
 
 
 class FrameStore:

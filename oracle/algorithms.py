@@ -25,7 +25,7 @@ __all__ = [
     "head_flags", "groupby_aggregate", "groupby_aggregate_multi_column",
     "segment_sums_f64", "inner_join_pairs", "inner_join_pairs_bruteforce",
     "sort_merge_join", "fmix32", "hash_multi_columns", "hash_partition",
-    "bucketize_padded", "compact_rows", "filter_mask_compact", "join_multi_column",
+    "bucketize_padded", "logical_pad_value", "compact_rows", "filter_mask_compact", "join_multi_column",
     "gather_join_column", "encode_string_array", "string_join_plan",
     "groupby_encoded_strings",
     "murmur3_x86_32", "sum_error_metrics", "PINNED_BY",
@@ -435,6 +435,24 @@ def join_multi_column(left_keys, left_key_names, left_values, right_keys, right_
     for c, v in right_values.items():
         vals[f"right_{c}"] = gather_join_column(v, ar, (right_value_dtypes or {}).get(c))
     return keys, vals
+
+
+def logical_pad_value(dtype_name, arr):
+    """Pad value of the padded bucket layout (``pa:150-160``): string codes
+    -1, floats NaN, integers -999999, bools False, else 0, in the column's
+    dtype. -999999 is cast to unsigned and narrow integer dtypes with numpy's
+    wrapping cast (``jnp.asarray(-999999, dtype=uint32)`` is not pinnable
+    without JAX: parity unpinned for those dtypes)."""
+    dt = np.asarray(arr).dtype
+    if dtype_name == "string":
+        return np.array(-1).astype(dt)[()]
+    if dt.kind == "f":
+        return dt.type(np.nan)
+    if dt.kind in "iu":
+        return np.array(-999999).astype(dt)[()]
+    if dt == np.bool_:
+        return np.False_
+    return dt.type(0)
 
 
 def bucketize_padded(arr, targets, num_devices, pad_value):

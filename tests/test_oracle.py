@@ -427,3 +427,41 @@ def test_groupby_encoded_strings_vs_pandas():
     np.testing.assert_allclose(r["s"], p.sum().values, rtol=1e-12)
     assert r["c"].tolist() == p.count().values.tolist() and r["c"].dtype == np.int32
     assert r["m"].tolist() == p.max().values.tolist()
+
+
+def test_logical_pad_value_known_answers():
+    """pa:150-160: string codes -1, floats NaN, integers -999999, bools False
+    (a bool column is not an integer column: its pad is False, not a wrapped
+    -999999), other dtypes 0."""
+    lp = oracle.logical_pad_value
+    assert lp("string", np.zeros(3, np.int32)) == -1
+    assert lp("string", np.zeros(3, np.int32)).dtype == np.int32
+    assert np.isnan(lp("float32", np.zeros(3, np.float32)))
+    assert lp("float32", np.zeros(3, np.float32)).dtype == np.float32
+    assert np.isnan(lp("float64", np.zeros(3, np.float64)))
+    assert lp("int32", np.zeros(3, np.int32)) == -999999
+    assert lp("int64", np.zeros(3, np.int64)) == -999999
+    b = lp("bool", np.zeros(3, np.bool_))
+    assert b is np.False_ or b == False  # noqa: E712
+    assert lp("complex", np.zeros(3, np.complex64)) == 0
+    # narrow and unsigned integers take -999999 through a wrapping cast
+    assert lp("uint32", np.zeros(3, np.uint32)) == 2**32 - 999999
+    assert lp("int16", np.zeros(3, np.int16)) == (-999999) % 2**16 - 2**16
+
+
+def test_partition_pad_rule_matches_oracle_on_host():
+    """The product's pad rule (evaluated on CPU tensors, no GPU) against the
+    oracle's, for every dtype a frame column can have."""
+    import torch
+    from jaxframes_amd import partition
+    cases = [(np.int8, "int8"), (np.int16, "int16"), (np.int32, "int32"), (np.int64, "int64"),
+             (np.uint8, "uint8"), (np.float16, "float16"), (np.float32, "float32"),
+             (np.float64, "float64"), (np.bool_, "bool"), (np.int32, "string")]
+    for dt, name in cases:
+        a = np.zeros(4, dt)
+        got = partition.logical_pad_value(name, torch.from_numpy(a))
+        exp = oracle.logical_pad_value(name, a)
+        if np.dtype(dt).kind == "f":
+            assert np.isnan(got) and np.isnan(exp), name
+        else:
+            assert got == exp, (name, got, exp)

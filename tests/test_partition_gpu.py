@@ -82,3 +82,44 @@ def test_multi_key_partition():
 def test_partition_empty():
     r = partition.hash_partition(torch.empty(0, dtype=torch.int64, device="cuda"), 8)
     assert r["offsets"].cpu().numpy().tolist() == [0] * 9
+
+
+PAD_CASES = [
+    # column dtype, dtype name as the frame records it, expected pad (pa:150-160)
+    (np.int32, "int32", -999999),
+    (np.int64, "int64", -999999),
+    (np.float32, "float32", np.nan),
+    (np.float64, "float64", np.nan),
+    (np.bool_, "bool", False),
+    (np.int32, "string", -1),     # dictionary codes of a string column
+]
+
+
+@pytest.mark.parametrize("dt,name,pad", PAD_CASES, ids=[c[1] + "-" + np.dtype(c[0]).name for c in PAD_CASES])
+def test_bucketize_padded_default_pad_matches_oracle(dt, name, pad):
+    """The padded [P, n] layout with the per-dtype pad rule (pad_value=None)
+    against oracle.bucketize_padded with oracle.logical_pad_value."""
+    rng = np.random.default_rng(len(name))
+    n, nb = 5000, 8
+    keys = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    if dt == np.bool_:
+        col = rng.random(n) < 0.5
+    elif np.dtype(dt).kind == "f":
+        col = rng.standard_normal(n).astype(dt)
+    else:
+        col = rng.integers(0, 500, n).astype(dt)
+    r = partition.hash_partition(cuda(keys), nb)
+    got = partition.bucketize_padded(cuda(col), r["offsets"], r["rows"], dtype_name=name)
+    targets = oracle.hash_multi_columns([keys]) % np.uint32(nb)
+    epad = oracle.logical_pad_value(name, col)
+    exp = oracle.bucketize_padded(col, targets, nb, epad)
+    g = got.cpu().numpy()
+    assert g.dtype == exp.dtype and g.shape == (nb, n)
+    assert np.array_equal(g, exp, equal_nan=np.dtype(dt).kind == "f")
+    # the pad fills exactly the slots past each bucket's rows
+    counts = np.diff(r["offsets"].cpu().numpy())
+    filler = g[nb - 1, counts[nb - 1]:]
+    if np.dtype(dt).kind == "f":
+        assert np.isnan(pad) and np.isnan(filler).all()
+    else:
+        assert (filler == pad).all()

@@ -279,3 +279,29 @@ def test_join_by_groups_duplicates_and_types(kdt, vdt, ddt):
     far = (np.arange(50) + 10_000).astype(kdt)
     keys, out = serving.execute(st, serving.Join("j", cuda(far), cuda(far.astype(np.float32))))
     assert keys.shape[0] == 0
+
+
+def test_served_sums_independent_of_earlier_groups():
+    """A group of huge values ahead of small groups in key order: each group's
+    sum comes from its own prefix sums, so the small groups are exact (a
+    prefix over the whole frame would carry the rounding of 1e23 into them)."""
+    keys = np.concatenate([np.zeros(1000, np.int64), np.array([1, 1, 2, 3, 3, 3], np.int64)])
+    vals = np.concatenate([np.full(1000, 1e20, np.float32),
+                           np.array([1.5, 2.25, -0.75, 1e-3, 2e-3, -5.0], np.float32)])
+    perm = np.random.default_rng(1).permutation(keys.size)
+    keys, vals = keys[perm], vals[perm]
+    st = serving.FrameStore(1 << 30)
+    st.load("big", cuda(keys), cuda(vals))
+    for op, t in ((">", 0.0), (">=", -1.0), ("!=", 7.0), ("<", 1e19)):
+        q = serving.FilterGroupBy("big", op, t)
+        uk, res = serving.execute(st, q)
+        pk, pres = serving.run_plain(st, q)
+        mask = {">": vals > t, ">=": vals >= t, "!=": vals != t, "<": vals < t}[op]
+        ek, eres = oracle.groupby_aggregate(keys[mask], {"s": vals[mask]}, {"s": "sum"})
+        assert np.array_equal(uk.cpu().numpy(), ek) and torch.equal(uk, pk)
+        got = res["sum"].cpu().numpy().astype(np.float64)
+        small = ek != 0
+        # exact for the small groups (their float32 sums are exact in float64)
+        assert np.array_equal(got[small], eres["s"][small].astype(np.float32).astype(np.float64)), (op, got, eres["s"])
+        assert torch.equal(res["sum"], pres["sum"]) or np.allclose(
+            got, pres["sum"].cpu().numpy(), rtol=1e-6, atol=0)
