@@ -128,7 +128,8 @@ def cpu_baseline(k: np.ndarray, v: np.ndarray, gpu_keys: np.ndarray, gpu_sums: n
     return base, check
 
 
-def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 256):
+def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 256,
+                     plain: bool = True):
     """Batched groupby of 4096 frames (row counts log-uniform in [1K, 4M]
     scaled to total_rows, or unscaled when it is None; int32 keys with
     per-frame cardinality log-uniform in [10, 1M], float32 N(0,1) values) as
@@ -168,8 +169,18 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
     # per-kernel HIP-event times of eager launches of the same pipeline
     eager = _measure(lambda: plan.launch(graph=False), 3, bytes_of)
     # the plain path is timed without the plan's buffers held by the allocator
+    tuned = {"digit_bits": plan.digit_bits}
     del plan
     torch.cuda.empty_cache()
+    if not plain:
+        return {"workload": f"4096 frames, {n} rows, int32 keys (per-frame cardinality "
+                            f"log-uniform [10, 1M]), float32 values; sum + count per frame",
+                "rows": n, "groups": groups, "ms_per_batch": round(dt * 1e3, 3),
+                "rows_per_s": round(n / dt, 1), "tuned": tuned, "pipelines": pipelines,
+                "eager_ms_per_batch": eager["ms_per_call"],
+                "alg_bytes_per_batch": eager["alg_bytes_per_call"],
+                "batch_hbm_frac": round(eager["alg_bytes_per_call"] / dt / 1e9 / PEAK_HBM_GBPS, 4),
+                "roofline": eager["roofline"], "kernels": eager["kernels"]}
     # warmed over the whole sample (the allocator grows its cached blocks to
     # every frame's workspace size once), then the median of 3 passes
     rows_plain = int(offs[plain_frames])
@@ -474,7 +485,9 @@ def workload_partition(reps: int, n: int = 200_000_000, nb: int = 256) -> dict:
 def only_workload(name: str, reps: int, batched_rows: int = 400_000_000) -> dict:
     """One workload by itself (for profiling)."""
     if name == "batched":
-        return {"only": name, **batched_vs_plain(batched_rows or None)}
+        # the batched pipeline alone (its plain per-frame comparison launches
+        # the same kernels thousands of times)
+        return {"only": name, **batched_vs_plain(batched_rows or None, plain=False)}
     if name == "served":
         return {"only": name, **served_vs_plain()}
     if name in ("sort", "sort_zipf"):
@@ -484,6 +497,42 @@ def only_workload(name: str, reps: int, batched_rows: int = 400_000_000) -> dict
     if name == "partition":
         return {"only": name, **workload_partition(reps)}
     raise SystemExit(f"unknown workload {name}")
+
+
+def bandwidth_probe(nbytes: int = 800_000_000, reps: int = 5) -> dict:
+    """HIP-event time of jfx_bandwidth_probe's read-only and copy kernels
+    over nbytes, for every grid / loads-in-flight / non-temporal setting;
+    returns the best of each mode (GB/s counts each byte read or written
+    once) and the whole sweep."""
+    import ctypes
+    import torch
+    from jaxframes_amd import _lib
+    dev = torch.device("cuda", 0)
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    stream = ctypes.c_void_p(_lib.stream_handle(dev))
+    sweep = []
+    for mode in (0, 1):
+        for blocks in (1024, 2048, 4096, 8192, 16384):
+            for unroll in (1, 2, 4, 8):
+                for nt in (0, 1):
+                    args = (mode, src.data_ptr(), dst.data_ptr(), nbytes, blocks, unroll, nt, stream)
+                    _lib.call("jfx_bandwidth_probe", *args)
+                    with _lib.KernelTimer() as kt:
+                        for _ in range(reps):
+                            _lib.call("jfx_bandwidth_probe", *args)
+                        torch.cuda.synchronize()
+                    ms, cnt = next(iter(kt.stats().values()))
+                    us = ms / cnt * 1e3
+                    moved = nbytes * (1 if mode == 0 else 2)
+                    sweep.append({"mode": "read" if mode == 0 else "copy", "blocks": blocks,
+                                  "unroll": unroll, "nontemporal": nt, "us": round(us, 1),
+                                  "GBps": round(moved / (us * 1e-6) / 1e9, 1)})
+    best = {}
+    for m in ("read", "copy"):
+        b = max((e for e in sweep if e["mode"] == m), key=lambda e: e["GBps"])
+        best[m] = {**b, "frac_of_peak": round(b["GBps"] / PEAK_HBM_GBPS, 4)}
+    return {"bytes": nbytes, "best": best, "sweep": sweep}
 
 
 def main() -> int:
@@ -504,6 +553,10 @@ def main() -> int:
     ap.add_argument("--only", choices=("sort", "sort_zipf", "join", "partition", "batched",
                                        "served"),
                     help="run one other workload by itself (for profiling) and print its line")
+    ap.add_argument("--bandwidth-probe", action="store_true",
+                    help="only sweep the hand-written read-only and copy kernels over 800 MB "
+                         "(grid, loads in flight, non-temporal loads) and print the best "
+                         "achieved GB/s of each: the ceilings the kernels are read against")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 400 MB (float32[1e8]) so that PMC byte "
                          "counters can be scaled against a transfer of known size")
@@ -521,6 +574,9 @@ def main() -> int:
     if args.only:
         print(json.dumps(only_workload(args.only, max(args.steps // 4, 3), args.batched_rows)),
               flush=True)
+        return 0
+    if args.bandwidth_probe:
+        print(json.dumps({"bandwidth_probe": bandwidth_probe()}), flush=True)
         return 0
     if args.copy_calibration:
         a = torch.ones(100_000_000, dtype=torch.float32, device=dev)
