@@ -241,6 +241,42 @@ __device__ __forceinline__ K op_sample_span(const K* keys, uint64_t n, int oc, u
   return wave_max_k(o) - wave_min_k(o);
 }
 
+// Rows per lane per load of the table kernel's steps: E rows of the wider of
+// the key and value types fill 16 bytes.
+template <typename K, typename V>
+constexpr int OP_E = 16 / (sizeof(K) > sizeof(V) ? sizeof(K) : sizeof(V));
+typedef uint32_t op_v4u __attribute__((ext_vector_type(4)));
+typedef uint32_t op_v2u __attribute__((ext_vector_type(2)));
+// One step's 16 rows of a lane: rows [c*64*E + lane*E, +E) of the step for c
+// = 0 .. 16/E - 1 into o[c*E .. c*E+E). Each load instruction covers 64
+// consecutive pieces, so it consumes whole cache lines, and the loads are
+// non-temporal (every row is read once). Measured at 100M int32 keys and
+// float32 values: 138-143 us, against 151-154 us for 16 consecutive rows per
+// lane with ordinary loads and 234 us with non-temporal ones (each such
+// instruction touches 64 lines in part). Runs of equal keys then fold in
+// registers over E consecutive rows instead of 16: keys in runs of 64 rows
+// measured 186 us against 152 us (16 lanes flush one residue together).
+template <int E, typename T>
+__device__ __forceinline__ void op_load_coal(const T* p, uint32_t lane, T (&o)[WT_RPL]) {
+  constexpr int B = E * sizeof(T);
+#pragma unroll
+  for (int c = 0; c < WT_RPL / E; ++c) {
+    const T* src = p + (uint64_t)c * 64 * E + (uint64_t)lane * E;
+    T t[E];
+    if constexpr (B == 16) {
+      const op_v4u* q = reinterpret_cast<const op_v4u*>(src);
+      const op_v4u v = __builtin_nontemporal_load(q);
+      __builtin_memcpy(t, &v, 16);
+    } else {
+      const op_v2u* q = reinterpret_cast<const op_v2u*>(src);
+      const op_v2u v = __builtin_nontemporal_load(q);
+      __builtin_memcpy(t, &v, 8);
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) o[c * E + i] = t[i];
+  }
+}
+
 template <typename K, typename V, bool MM, bool VEC, uint32_t TE>
 __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, MM> a) {
   using T = typename OpT<V>::T;
@@ -376,9 +412,9 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
     K ka[WT_RPL], kb[WT_RPL];
     V va[WT_RPL], vb[WT_RPL];
     auto load = [&](uint64_t b, K (&kk)[WT_RPL], V (&vv)[WT_RPL]) {
-      const uint64_t bc = (b < last ? b : last) + (uint64_t)lane * WT_RPL;
-      wt_load_rows(a.keys + bc, kk);
-      wt_load_rows(a.vals + bc, vv);
+      const uint64_t bc = (b < last ? b : last);
+      op_load_coal<OP_E<K, V>>(a.keys + bc, lane, kk);
+      op_load_coal<OP_E<K, V>>(a.vals + bc, lane, vv);
     };
     auto fold = [&](K (&kk)[WT_RPL], V (&vv)[WT_RPL]) {
 #pragma unroll
