@@ -200,7 +200,10 @@ int jfx_hash_partition(int key_type, const void* keys, uint64_t n, uint32_t nbuc
 int jfx_join_workspace_size(uint64_t n_left, uint64_t* out_bytes);
 
 /* Counts the pairs (per 2048-row left tile, scanned) and writes the total to
- * *out_pairs; synchronises the stream once. */
+ * *out_pairs; synchronises the stream once. JFX_ERR_UNSUPPORTED when the
+ * join has 2^32 pairs or more. The fills that follow write each tile's pairs
+ * in chunks of at most 8192, one workgroup each, so a key hot on both sides
+ * is spread over many workgroups. */
 int jfx_join_count(int key_type, const void* sorted_left, uint64_t n_left, const void* sorted_right,
                    uint64_t n_right, uint64_t* out_pairs, void* workspace, uint64_t ws_bytes,
                    void* stream);
@@ -229,6 +232,51 @@ int jfx_join_fill_values(int key_type, const void* sorted_left, const uint32_t* 
                          int right_by_row, uint32_t* out_left_rows, uint32_t* out_right_rows,
                          void* out_keys, void* out_left_values, void* out_right_values,
                          void* workspace, uint64_t ws_bytes, void* stream);
+
+/* One side of jfx_join_fill_columns: what the fill writes for that side of
+ * every pair. `rows` maps a sorted position to its source row (NULL: the
+ * position itself); `out_rows` (optional) receives each pair's source row.
+ * ncols payload columns are written to out_cols[j] (M entries, col_bytes[j]
+ * = 4 or 8 bytes each), read either
+ *  - from cols[j] at the source row (by_row = 1) or at the sorted position
+ *    (by_row = 0), or
+ *  - when record_bytes is 4 or 8, as the field at byte col_offset[j] of the
+ *    record at the sorted position in `records`: a payload the sort carried
+ *    in key order (jfx_pack_columns packs up to 8 bytes of columns into one
+ *    record per row), read sequentially instead of gathered. */
+#define JFX_JOIN_MAXCOLS 8
+typedef struct jfx_join_side {
+  const uint32_t* rows;
+  uint32_t* out_rows;
+  int32_t ncols;
+  int32_t by_row;
+  int32_t record_bytes;
+  const void* records;
+  const void* cols[JFX_JOIN_MAXCOLS];
+  int32_t col_bytes[JFX_JOIN_MAXCOLS];
+  int32_t col_offset[JFX_JOIN_MAXCOLS];
+  void* out_cols[JFX_JOIN_MAXCOLS];
+} jfx_join_side;
+
+/* jfx_join_fill with any number of payload columns per side (up to 8), all
+ * written by the fill: the pair enumeration of ParallelSortMergeJoin.join
+ * (parallel_algorithms.py:987-994) with the payload gathers of :1052-1073
+ * fused in, so no row columns need to be written for later gathers. left /
+ * right may be NULL (nothing written for that side). Same workspace and
+ * preceding jfx_join_count as jfx_join_fill. */
+int jfx_join_fill_columns(int key_type, const void* sorted_left, uint64_t n_left,
+                          const jfx_join_side* left, const void* sorted_right, uint64_t n_right,
+                          const jfx_join_side* right, void* out_keys, void* workspace,
+                          uint64_t ws_bytes, void* stream);
+
+/* Packs ncols columns of n rows (4 or 8 bytes each, at most 8 bytes in all)
+ * into one record per row, column j at the byte offset of the columns before
+ * it; *out_record_bytes (optional) receives the record width. A join side's
+ * payload packed this way rides through its radix sort as one 4- or 8-byte
+ * payload (jfx_sort), so the fill reads it in key order: the column gather
+ * of parallel_algorithms.py:1052-1073 moved into the sort. */
+int jfx_pack_columns(int ncols, const void* const* cols, const int* col_bytes, uint64_t n,
+                     void* out, int* out_record_bytes, void* stream);
 
 /* For each sorted probe key, build_rows[first index of an equal key in
  * sorted_build], or 0xFFFFFFFF when none: the searchsorted + equality test

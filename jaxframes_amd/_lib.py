@@ -30,6 +30,20 @@ class JfxError(RuntimeError):
     """A libjfx call failed."""
 
 
+JOIN_MAXCOLS = 8
+
+
+class JoinSide(ctypes.Structure):
+    """jfx_join_side (include/jfx.h)."""
+    _fields_ = [("rows", ctypes.c_void_p), ("out_rows", ctypes.c_void_p),
+                ("ncols", ctypes.c_int32), ("by_row", ctypes.c_int32),
+                ("record_bytes", ctypes.c_int32), ("records", ctypes.c_void_p),
+                ("cols", ctypes.c_void_p * JOIN_MAXCOLS),
+                ("col_bytes", ctypes.c_int32 * JOIN_MAXCOLS),
+                ("col_offset", ctypes.c_int32 * JOIN_MAXCOLS),
+                ("out_cols", ctypes.c_void_p * JOIN_MAXCOLS)]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -65,6 +79,9 @@ _SIGNATURES = {
                       c_vp],
     "jfx_join_fill_values": [c_int, c_vp, c_vp, c_u64, c_vp, c_vp, c_u64, c_int, c_vp, c_int, c_int,
                              c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_vp],
+    "jfx_join_fill_columns": [c_int, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u64, c_vp],
+    "jfx_pack_columns": [c_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_int), c_u64, c_vp,
+                         ctypes.POINTER(c_int), c_vp],
     "jfx_join_first_match": [c_int, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp],
     "jfx_gather": [c_int, c_vp, c_vp, c_u64, c_vp, c_u64, c_vp],
     "jfx_batched_groupby_workspace_size": [ctypes.c_uint32, c_int, c_u64, c_u64p],

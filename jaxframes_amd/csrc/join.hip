@@ -142,94 +142,367 @@ k_join_count(const K* sl, uint64_t L, const K* sr, int oc, const uint64_t* wlo, 
   }
 }
 
-// Exclusive scan of n uint64 counts in place by one workgroup; writes the
-// total to *total.
-__global__ void __launch_bounds__(1024) k_join_scan(uint64_t* c, uint32_t n, uint64_t* total) {
-  __shared__ uint64_t s_w[17];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t per = (n + 1023u) / 1024u;
-  const uint32_t a = tid * per, b = (a + per) < n ? (a + per) : n;
-  uint64_t sum = 0;
-  for (uint32_t i = a; i < b; ++i) sum += c[i];
-  const uint64_t incl = wave_incl_scan64(sum);
-  if (lane_id() == 63) s_w[tid >> 6] = incl;
-  __syncthreads();
-  if (tid == 0) {
-    uint64_t acc = 0;
-    for (int w = 0; w < 16; ++w) {
-      const uint64_t v = s_w[w];
-      s_w[w] = acc;
-      acc += v;
+// Fill chunks: a tile's pairs are written in chunks of at most JFCH outputs,
+// each chunk by one workgroup, so a tile whose rows match long right runs (a
+// key hot on both sides) is spread over many workgroups.
+constexpr uint32_t JFCH = 4096;
+// chunk map entries beyond one per tile: a tile of c pairs has
+// ceil(c / JFCH) chunks, so fewer than 2^32 pairs need at most
+// tiles + 2^32 / JFCH entries
+constexpr uint64_t JCMAP_EXTRA = (1ull << 32) / JFCH;
+constexpr uint32_t JSCAN = 8192;  // tile counts staged per round of k_join_scan
+
+// One workgroup: exclusive scan of the n tile pair counts in place (c), the
+// first chunk of every tile (cfirst), the tile of every chunk (cmap, at most
+// cmap_cap entries), and the totals: total[0] = pairs, total[1] = chunks.
+// The counts are staged through LDS in rounds of JSCAN, each thread scanning
+// 8 consecutive ones.
+__global__ void __launch_bounds__(1024)
+k_join_scan(uint64_t* c, uint32_t n, uint64_t* total, uint32_t* cfirst, uint32_t* cmap,
+            uint64_t cmap_cap) {
+  __shared__ uint64_t s_c[JSCAN];
+  __shared__ uint64_t s_w[2][17];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
+  uint64_t carry_p = 0, carry_c = 0;
+  for (uint32_t base = 0; base < n; base += JSCAN) {
+    const uint32_t m = (n - base) < JSCAN ? (n - base) : JSCAN;
+    for (uint32_t i = tid; i < m; i += 1024) s_c[i] = c[base + i];
+    __syncthreads();
+    uint64_t v[8], sp = 0, sc = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t i = tid * 8 + j;
+      v[j] = i < m ? s_c[i] : 0ull;
+      sp += v[j];
+      sc += (v[j] + JFCH - 1) / JFCH;
     }
-    s_w[16] = acc;
+    const uint64_t ip = wave_incl_scan64(sp), ic = wave_incl_scan64(sc);
+    if (lane == 63) {
+      s_w[0][wid] = ip;
+      s_w[1][wid] = ic;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint64_t ap = 0, ac = 0;
+      for (int w = 0; w < 16; ++w) {
+        const uint64_t tp = s_w[0][w], tc = s_w[1][w];
+        s_w[0][w] = ap;
+        s_w[1][w] = ac;
+        ap += tp;
+        ac += tc;
+      }
+      s_w[0][16] = ap;
+      s_w[1][16] = ac;
+    }
+    __syncthreads();
+    uint64_t ep = carry_p + s_w[0][wid] + ip - sp, ec = carry_c + s_w[1][wid] + ic - sc;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t i = tid * 8 + j;
+      if (i < m) {
+        c[base + i] = ep;
+        cfirst[base + i] = (uint32_t)ec;
+        const uint64_t nch = (v[j] + JFCH - 1) / JFCH;
+        for (uint64_t q = 0; q < nch; ++q)
+          if (ec + q < cmap_cap) cmap[ec + q] = base + i;
+        ep += v[j];
+        ec += nch;
+      }
+    }
+    carry_p += s_w[0][16];
+    carry_c += s_w[1][16];
+    __syncthreads();
   }
-  __syncthreads();
-  uint64_t run = s_w[tid >> 6] + incl - sum;
-  for (uint32_t i = a; i < b; ++i) {
-    const uint64_t v = c[i];
-    c[i] = run;
-    run += v;
+  if (tid == 0) {
+    total[0] = carry_p;
+    total[1] = carry_c;
   }
-  if (tid == 0) *total = s_w[16];
 }
 
-// One payload column per side written by the fill itself (the value gather
-// of parallel_algorithms.py:1052-1073 fused into pair enumeration): src is
-// indexed by the side's source row (by_row) or by its sorted position.
-template <typename LT, typename RT>
-struct FillPay {
-  const LT* lv;
-  const RT* rv;
-  LT* out_lv;
-  RT* out_rv;
-  int l_by_row;
-  int r_by_row;
+// What the fill writes for one side of every pair: the source row, and
+// payload columns read at the source row or at the sorted position, either
+// from separate columns or as fields of one packed record per sorted
+// position (a payload the sort carried).
+struct JoinSideOut {
+  const uint32_t* perm;   // source row of each sorted position (null: the position)
+  uint32_t* out_rows;
+  int ncols;
+  int by_row;
+  int rec_bytes;          // 0, 4 or 8
+  const void* rec;
+  const void* src[JFX_JOIN_MAXCOLS];
+  int bytes[JFX_JOIN_MAXCOLS];
+  int off[JFX_JOIN_MAXCOLS];
+  void* out[JFX_JOIN_MAXCOLS];
 };
 
-template <typename K, typename LT, typename RT>
+// Layout of a side's payload, chosen once per launch: the cases the inner
+// join produces get their own straight-line stores.
+enum JoinPayMode : int { JP_NONE = 0, JP_REC8_2X4 = 1, JP_REC8_1X8 = 2, JP_REC4 = 3, JP_COLS = 4 };
+
+__host__ inline int join_pay_mode(const JoinSideOut& s) {
+  if (s.ncols == 0) return JP_NONE;
+  if (s.rec_bytes == 8 && s.ncols == 2 && s.bytes[0] == 4 && s.bytes[1] == 4 && s.off[0] == 0 &&
+      s.off[1] == 4)
+    return JP_REC8_2X4;
+  if (s.rec_bytes == 8 && s.ncols == 1 && s.bytes[0] == 8) return JP_REC8_1X8;
+  if (s.rec_bytes == 4 && s.ncols == 1) return JP_REC4;
+  return JP_COLS;
+}
+
+// Global-memory pointers: the fill's loads and stores through them are
+// global (not flat) instructions, with a scalar base and a 32-bit offset
+// when the base is wave-uniform.
+#define JFX_GLOBAL __attribute__((address_space(1)))
+
+// A wave-uniform pointer held in scalar registers, as a global pointer.
+template <typename T>
+__device__ __forceinline__ JFX_GLOBAL T* uniform_ptr(const void* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (JFX_GLOBAL T*)(((uint64_t)hi << 32) | lo);
+}
+
+// Element idx of a global array, the byte offset computed in 32 bits (the
+// caller's arrays are below 4 GB from the base): a scalar base plus a 32-bit
+// vector offset, one address form of the global instructions.
+template <typename T>
+__device__ __forceinline__ JFX_GLOBAL T& gat(JFX_GLOBAL T* base, uint32_t idx) {
+  return *(JFX_GLOBAL T*)((JFX_GLOBAL char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ const JFX_GLOBAL T& gat(JFX_GLOBAL const T* base, uint32_t idx) {
+  return *(JFX_GLOBAL const T*)((JFX_GLOBAL const char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+
+// One side's pointers for one chunk: sources offset so that a side's
+// sorted-position index and the chunk's output index are 32-bit offsets from
+// wave-uniform bases.
+struct JoinChunkSide {
+  JFX_GLOBAL const uint32_t* perm;
+  JFX_GLOBAL uint32_t* out_rows;
+  JFX_GLOBAL const char* rec;
+  JFX_GLOBAL char* out0;
+  JFX_GLOBAL char* out1;
+  uint64_t out_base;
+};
+
+__device__ __forceinline__ JoinChunkSide join_chunk_side(const JoinSideOut& s, uint64_t pos0,
+                                                         uint64_t out0) {
+  JoinChunkSide c;
+  c.perm = uniform_ptr<const uint32_t>(s.perm ? s.perm + pos0 : nullptr);
+  c.out_rows = uniform_ptr<uint32_t>(s.out_rows ? s.out_rows + out0 : nullptr);
+  c.rec = uniform_ptr<const char>(
+      s.rec_bytes ? static_cast<const char*>(s.rec) + pos0 * s.rec_bytes : nullptr);
+  c.out0 = uniform_ptr<char>(s.ncols > 0 ? static_cast<char*>(s.out[0]) + out0 * s.bytes[0] : nullptr);
+  c.out1 = uniform_ptr<char>(s.ncols > 1 ? static_cast<char*>(s.out[1]) + out0 * s.bytes[1] : nullptr);
+  c.out_base = out0;
+  return c;
+}
+
+// Writes output q of the chunk for sorted position pos0 + p of this side.
+__device__ __forceinline__ void join_emit_side(const JoinSideOut& s, const JoinChunkSide& c,
+                                               int mode, uint32_t q, uint32_t p, uint64_t pos0) {
+  const bool need_row = c.out_rows || (mode == JP_COLS && s.by_row);
+  const uint32_t row = (need_row && c.perm) ? gat(c.perm, p) : (uint32_t)(pos0 + p);
+  if (c.out_rows) gat(c.out_rows, q) = row;
+  switch (mode) {
+    case JP_NONE:
+      return;
+    case JP_REC8_2X4: {
+      const uint64_t r = gat(reinterpret_cast<JFX_GLOBAL const uint64_t*>(c.rec), p);
+      gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out0), q) = (uint32_t)r;
+      gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out1), q) = (uint32_t)(r >> 32);
+      return;
+    }
+    case JP_REC8_1X8:
+      gat(reinterpret_cast<JFX_GLOBAL uint64_t*>(c.out0), q) =
+          gat(reinterpret_cast<JFX_GLOBAL const uint64_t*>(c.rec), p);
+      return;
+    case JP_REC4:
+      gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out0), q) =
+          gat(reinterpret_cast<JFX_GLOBAL const uint32_t*>(c.rec), p);
+      return;
+    default:
+      break;
+  }
+  // any other layout: per column, from the kernel arguments
+  const uint64_t k = c.out_base + q;
+  if (s.rec_bytes == 8) {
+    const uint64_t r = reinterpret_cast<JFX_GLOBAL const uint64_t*>(c.rec)[p];
+    for (int j = 0; j < s.ncols; ++j) {
+      if (s.bytes[j] == 8) static_cast<uint64_t*>(s.out[j])[k] = r;
+      else static_cast<uint32_t*>(s.out[j])[k] = (uint32_t)(r >> (8 * s.off[j]));
+    }
+    return;
+  }
+  const uint64_t idx = s.by_row ? (uint64_t)row : pos0 + p;
+  for (int j = 0; j < s.ncols; ++j) {
+    if (s.bytes[j] == 8)
+      static_cast<uint64_t*>(s.out[j])[k] = static_cast<const uint64_t*>(s.src[j])[idx];
+    else
+      static_cast<uint32_t*>(s.out[j])[k] = static_cast<const uint32_t*>(s.src[j])[idx];
+  }
+}
+
+// The record of sorted position p of a side in a record mode (JP_NONE: 0).
+__device__ __forceinline__ uint64_t join_load_rec(const JoinChunkSide& c, int mode, uint32_t p) {
+  if (mode == JP_REC8_2X4 || mode == JP_REC8_1X8)
+    return gat(reinterpret_cast<JFX_GLOBAL const uint64_t*>(c.rec), p);
+  if (mode == JP_REC4) return gat(reinterpret_cast<JFX_GLOBAL const uint32_t*>(c.rec), p);
+  return 0;
+}
+
+__device__ __forceinline__ void join_store_rec(const JoinChunkSide& c, int mode, uint32_t q,
+                                               uint64_t r) {
+  if (mode == JP_REC8_2X4) {
+    gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out0), q) = (uint32_t)r;
+    gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out1), q) = (uint32_t)(r >> 32);
+  } else if (mode == JP_REC8_1X8) {
+    gat(reinterpret_cast<JFX_GLOBAL uint64_t*>(c.out0), q) = r;
+  } else if (mode == JP_REC4) {
+    gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out0), q) = (uint32_t)r;
+  }
+}
+
+constexpr int JFU = 8;  // outputs per thread per round of the batched fill loop (4: 1.77 ms, 8: 1.73 ms at 1.6e8 pairs)
+
+// Persistent over the chunks of k_join_scan: chunk c of tile t covers the
+// tile's outputs [(c - cfirst[t]) * JFCH, +JFCH). Per chunk the tile's
+// bounds and in-tile scan are rebuilt (window staged in LDS), then output
+// position k of the chunk finds its left row by binary search over the scan,
+// so a row with many matches is spread over the workgroup.
+template <typename K>
 __global__ void __launch_bounds__(JB)
-k_join_fill(const K* sl, const uint32_t* lperm, uint64_t L, const K* sr, const uint32_t* rperm,
-            int oc, const uint64_t* wlo, const uint64_t* whi, const uint64_t* tile_off,
-            uint32_t* out_l, uint32_t* out_r, K* out_key, FillPay<LT, RT> pay) {
+k_join_fill(const K* sl, uint64_t L, const K* sr, int oc, const uint64_t* wlo, const uint64_t* whi,
+            const uint64_t* tile_off, const uint32_t* cfirst, const uint32_t* cmap,
+            uint64_t cmap_cap, const uint64_t* total, K* out_key, JoinSideOut left,
+            JoinSideOut right, int lmode, int rmode) {
   __shared__ K s_w[JWMAX];
-  __shared__ uint32_t s_incl[JT];   // inclusive scan of the tile's counts
+  __shared__ uint32_t s_incl[JT];   // exclusive scan of the tile's counts
   __shared__ uint32_t s_lo[JT];
   __shared__ uint32_t s_tmp[JB / 64 + 1];
-  const uint32_t t = blockIdx.x;
-  const uint64_t tb = (uint64_t)t * JT;
-  const uint64_t w0 = wlo[t], w1 = whi[t];
-  const bool staged = stage_window(sr, oc, w0, w1, s_w);
-  __syncthreads();
-  uint64_t lo[JIPT];
-  uint32_t cnt[JIPT];
-  tile_bounds(sl, L, sr, oc, tb, w0, w1, s_w, staged, lo, cnt);
+  // tile row of each output of the chunk, in the window's LDS (the window is
+  // read only by tile_bounds, before the scan's barriers)
+  static_assert(JWMAX * sizeof(K) >= JFCH * sizeof(uint16_t), "row map fits the window");
+  uint16_t* const s_row = reinterpret_cast<uint16_t*>(s_w);
+  const uint64_t nchunks = total[1] < cmap_cap ? total[1] : cmap_cap, npairs = total[0];
+  // record payloads (or none) on both sides and no row outputs: the batched
+  // output loop
+  const bool fast = lmode != JP_COLS && rmode != JP_COLS && !left.out_rows && !right.out_rows;
+  const uint32_t ntiles = (uint32_t)((L + JT - 1) / JT);
+  for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const uint32_t t = cmap[c];
+    const uint64_t tb = (uint64_t)t * JT;
+    const uint64_t w0 = wlo[t], w1 = whi[t];
+    const bool staged = stage_window(sr, oc, w0, w1, s_w);
+    __syncthreads();
+    uint64_t lo[JIPT];
+    uint32_t cnt[JIPT];
+    tile_bounds(sl, L, sr, oc, tb, w0, w1, s_w, staged, lo, cnt);
 #pragma unroll
-  for (int j = 0; j < JIPT; ++j) {
-    s_incl[threadIdx.x * JIPT + j] = cnt[j];
-    s_lo[threadIdx.x * JIPT + j] = (uint32_t)lo[j];
-  }
-  __syncthreads();
-  const uint32_t tot = block_excl_scan_lds<JT, JB>(s_incl, s_tmp);  // now exclusive
-  const uint64_t base = tile_off[t];
-  for (uint32_t k = threadIdx.x; k < tot; k += JB) {
-    // row r: last row whose exclusive start <= k
-    uint32_t a = 0, b = JT;
-    while (a + 1 < b) {
-      const uint32_t m = (a + b) >> 1;
-      if (s_incl[m] <= k) a = m;
-      else b = m;
+    for (int j = 0; j < JIPT; ++j) {
+      s_incl[threadIdx.x * JIPT + j] = cnt[j];
+      s_lo[threadIdx.x * JIPT + j] = (uint32_t)lo[j];
     }
-    const uint32_t r = a;
-    const uint64_t i = tb + r;
-    const uint64_t rr = (uint64_t)s_lo[r] + (k - s_incl[r]);
-    // left rows: through lperm, or the sorted position when lperm is null
-    const uint32_t lrow = lperm ? lperm[i] : (uint32_t)i;
-    const uint32_t rrow = rperm ? rperm[rr] : (uint32_t)rr;
-    if (out_l) out_l[base + k] = lrow;
-    if (out_r) out_r[base + k] = rrow;
-    if (out_key) out_key[base + k] = sl[i];
-    if (pay.out_lv) pay.out_lv[base + k] = pay.lv[pay.l_by_row ? lrow : i];
-    if (pay.out_rv) pay.out_rv[base + k] = pay.rv[pay.r_by_row ? rrow : rr];
+    __syncthreads();
+    const uint32_t tot = block_excl_scan_lds<JT, JB>(s_incl, s_tmp);  // now exclusive
+    const uint64_t base = tile_off[t];
+    const uint64_t tend = (t + 1 < ntiles ? tile_off[t + 1] : npairs) - base;
+    const uint32_t k0 = (uint32_t)((c - cfirst[t]) * JFCH);
+    const uint32_t k1 = (uint64_t)k0 + JFCH < tend ? k0 + JFCH : (uint32_t)tend;
+    (void)tot;
+    // each row marks its outputs inside [k0, k1) with its index, so an
+    // output finds its row with one LDS read
+#pragma unroll
+    for (int j = 0; j < JIPT; ++j) {
+      const uint32_t r = threadIdx.x * JIPT + j;
+      const uint32_t a0 = s_incl[r], a1 = a0 + cnt[j];
+      const uint32_t b0 = a0 > k0 ? a0 : k0, b1 = a1 < k1 ? a1 : k1;
+      for (uint32_t q = b0; q < b1; ++q) s_row[q - k0] = (uint16_t)r;
+    }
+    __syncthreads();
+    // wave-uniform bases: the chunk's outputs, the tile's left rows and the
+    // tile's right window; the indices below are 32-bit offsets from them
+    const uint64_t out0 = base + k0;
+    const JoinChunkSide lc = join_chunk_side(left, tb, out0);
+    const JoinChunkSide rc = join_chunk_side(right, w0, out0);
+    JFX_GLOBAL const K* slt = uniform_ptr<const K>(sl + tb);
+    JFX_GLOBAL K* okey = uniform_ptr<K>(out_key ? out_key + out0 : nullptr);
+    const uint32_t w0l = (uint32_t)w0;
+    const uint32_t nout = k1 - k0;
+    if (fast) {
+      // JFU outputs per thread per round: every load of the round is issued
+      // before its stores (one memory round trip per JFU outputs)
+      for (uint32_t q0 = threadIdx.x; q0 < nout; q0 += JB * JFU) {
+        uint32_t a[JFU], rp[JFU];
+        K key[JFU];
+        uint64_t lr[JFU], rr[JFU];
+#pragma unroll
+        for (int u = 0; u < JFU; ++u) {
+          const uint32_t q = q0 + u * JB;
+          const uint32_t qc = q < nout ? q : nout - 1;
+          a[u] = s_row[qc];
+          rp[u] = s_lo[a[u]] - w0l + (k0 + qc - s_incl[a[u]]);
+        }
+#pragma unroll
+        for (int u = 0; u < JFU; ++u) {
+          key[u] = okey ? gat(slt, a[u]) : K(0);
+          lr[u] = join_load_rec(lc, lmode, a[u]);
+          rr[u] = join_load_rec(rc, rmode, rp[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < JFU; ++u) {
+          const uint32_t q = q0 + u * JB;
+          if (q < nout) {
+            if (okey) gat(okey, q) = key[u];
+            join_store_rec(lc, lmode, q, lr[u]);
+            join_store_rec(rc, rmode, q, rr[u]);
+          }
+        }
+      }
+    } else {
+      for (uint32_t q = threadIdx.x; q < nout; q += JB) {
+        const uint32_t a = s_row[q];
+        const uint32_t rp = s_lo[a] - w0l + (k0 + q - s_incl[a]);
+        if (okey) gat(okey, q) = gat(slt, a);
+        join_emit_side(left, lc, lmode, q, a, tb);
+        join_emit_side(right, rc, rmode, q, rp, w0);
+      }
+    }
+    __syncthreads();   // LDS is rebuilt for the next chunk
+  }
+}
+
+// Two 4-byte columns into 8-byte records, four rows per thread with 16-byte
+// loads and stores (n4 = rows / 4; the rows beyond 4 * n4 go to k_pack_cols).
+__global__ void __launch_bounds__(256)
+k_pack_2x4(const uint4* lo, const uint4* hi, uint64_t n4, uint4* out) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += stride) {
+    const uint4 a = lo[i], b = hi[i];
+    out[2 * i] = make_uint4(a.x, b.x, a.y, b.y);
+    out[2 * i + 1] = make_uint4(a.z, b.z, a.w, b.w);
+  }
+}
+
+// Packs ncols columns (4 or 8 bytes each, at most 8 bytes together) into one
+// record per row: column j at byte offset off[j]; rows [first, n).
+__global__ void __launch_bounds__(256)
+k_pack_cols(JoinSideOut p, uint64_t first, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t i = first + (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+    uint64_t r = 0;
+#pragma unroll
+    for (int j = 0; j < JFX_JOIN_MAXCOLS; ++j) {
+      if (j >= p.ncols) break;
+      const uint64_t v = p.bytes[j] == 8 ? static_cast<const uint64_t*>(p.src[j])[i]
+                                         : (uint64_t)static_cast<const uint32_t*>(p.src[j])[i];
+      r |= v << (8 * p.off[j]);
+    }
+    if (p.rec_bytes == 8) static_cast<uint64_t*>(p.out[0])[i] = r;
+    else static_cast<uint32_t*>(p.out[0])[i] = (uint32_t)r;
   }
 }
 
@@ -270,8 +543,11 @@ uint64_t jtiles(uint64_t L) { return (L + JT - 1) / JT; }
 struct JoinWs {
   uint64_t* wlo;
   uint64_t* whi;
-  uint64_t* tcnt;
-  uint64_t* total;
+  uint64_t* tcnt;    // tile pair counts, then their exclusive offsets
+  uint64_t* total;   // [0] pairs, [1] fill chunks
+  uint32_t* cfirst;  // first fill chunk of each tile
+  uint32_t* cmap;    // tile of each fill chunk
+  uint64_t cmap_cap;
 };
 
 JoinWs carve_join(Carver& c, uint64_t L) {
@@ -281,6 +557,9 @@ JoinWs carve_join(Carver& c, uint64_t L) {
   w.whi = c.take<uint64_t>(t);
   w.tcnt = c.take<uint64_t>(t);
   w.total = c.take<uint64_t>(2);
+  w.cfirst = c.take<uint32_t>(t);
+  w.cmap_cap = t + JCMAP_EXTRA;
+  w.cmap = c.take<uint32_t>(w.cmap_cap);
   return w;
 }
 
@@ -294,54 +573,75 @@ int join_count_typed(int oc, const void* sl, uint64_t L, const void* sr, uint64_
   JFX_LAUNCH("join_count", s, (k_join_count<K><<<nt, JB, 0, s>>>((const K*)sl, L, (const K*)sr, oc,
                                                                   w.wlo, w.whi, w.tcnt)));
   JFX_HIP(hipGetLastError());
-  JFX_LAUNCH("join_scan", s, (k_join_scan<<<1, 1024, 0, s>>>(w.tcnt, nt, w.total)));
+  JFX_LAUNCH("join_scan", s, (k_join_scan<<<1, 1024, 0, s>>>(w.tcnt, nt, w.total, w.cfirst, w.cmap,
+                                                             w.cmap_cap)));
   JFX_HIP(hipGetLastError());
-  return read_back(s, w.total, sizeof(uint64_t), out_m);
+  const int rc = read_back(s, w.total, sizeof(uint64_t), out_m);
+  if (rc != JFX_OK) return rc;
+  JFX_CHECK(*out_m < (1ull << 32), JFX_ERR_UNSUPPORTED,
+            "the join produces 2^32 pairs or more; at most 2^32 - 1 are supported");
+  return JFX_OK;
 }
 
-template <typename K, typename LT = uint32_t, typename RT = uint32_t>
-int join_fill_typed(int oc, const void* sl, const uint32_t* lperm, uint64_t L, const void* sr,
-                    const uint32_t* rperm, const JoinWs& w, uint32_t* out_l, uint32_t* out_r,
-                    void* out_key, hipStream_t s, FillPay<LT, RT> pay = {}) {
-  const uint32_t nt = (uint32_t)jtiles(L);
-  JFX_LAUNCH("join_fill", s, (k_join_fill<K, LT, RT><<<nt, JB, 0, s>>>(
-                                 (const K*)sl, lperm, L, (const K*)sr, rperm, oc, w.wlo, w.whi,
-                                 w.tcnt, out_l, out_r, (K*)out_key, pay)));
+// Workgroups of the persistent fill: enough to fill every CU several times
+// over, never more than the tiles' worst-case chunk count.
+uint32_t fill_grid(uint64_t L) {
+  const uint64_t bound = jtiles(L) + JCMAP_EXTRA;
+  return (uint32_t)std::min<uint64_t>(bound, 256u * 8u);
+}
+
+template <typename K>
+int join_fill_typed(int oc, const void* sl, uint64_t L, const void* sr, const JoinWs& w,
+                    void* out_key, const JoinSideOut& left, const JoinSideOut& right,
+                    hipStream_t s) {
+  JFX_LAUNCH("join_fill", s, (k_join_fill<K><<<fill_grid(L), JB, 0, s>>>(
+                                 (const K*)sl, L, (const K*)sr, oc, w.wlo, w.whi, w.tcnt, w.cfirst,
+                                 w.cmap, w.cmap_cap, w.total, (K*)out_key, left, right,
+                                 join_pay_mode(left), join_pay_mode(right))));
   JFX_HIP(hipGetLastError());
   return JFX_OK;
 }
 
-template <typename K, typename LT, typename RT>
-int join_fill_pay3(int oc, const void* sl, const uint32_t* lperm, uint64_t L, const void* sr,
-                   const uint32_t* rperm, const JoinWs& w, uint32_t* out_l, uint32_t* out_r,
-                   void* out_key, const void* lv, int l_by_row, void* out_lv, const void* rv,
-                   int r_by_row, void* out_rv, hipStream_t s) {
-  FillPay<LT, RT> pay{(const LT*)lv, (const RT*)rv, (LT*)out_lv, (RT*)out_rv, l_by_row, r_by_row};
-  return join_fill_typed<K, LT, RT>(oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key, s, pay);
+int join_fill_dispatch(int key_type, const void* sl, uint64_t L, const void* sr,
+                       const JoinWs& w, void* out_key, const JoinSideOut& left,
+                       const JoinSideOut& right, hipStream_t s) {
+  const int oc = order_code(key_type);
+  if (key_bytes(key_type) == 4)
+    return join_fill_typed<uint32_t>(oc, sl, L, sr, w, out_key, left, right, s);
+  return join_fill_typed<uint64_t>(oc, sl, L, sr, w, out_key, left, right, s);
 }
 
-template <typename K, typename LT>
-int join_fill_pay2(int rb, int oc, const void* sl, const uint32_t* lperm, uint64_t L,
-                   const void* sr, const uint32_t* rperm, const JoinWs& w, uint32_t* out_l,
-                   uint32_t* out_r, void* out_key, const void* lv, int l_by_row, void* out_lv,
-                   const void* rv, int r_by_row, void* out_rv, hipStream_t s) {
-  if (rb == 8)
-    return join_fill_pay3<K, LT, uint64_t>(oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key,
-                                           lv, l_by_row, out_lv, rv, r_by_row, out_rv, s);
-  return join_fill_pay3<K, LT, uint32_t>(oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key, lv,
-                                         l_by_row, out_lv, rv, r_by_row, out_rv, s);
-}
-
-template <typename K>
-int join_fill_pay(int lb, int rb, int oc, const void* sl, const uint32_t* lperm, uint64_t L,
-                  const void* sr, const uint32_t* rperm, const JoinWs& w, uint32_t* out_l,
-                  uint32_t* out_r, void* out_key, const void* lv, int l_by_row, void* out_lv,
-                  const void* rv, int r_by_row, void* out_rv, hipStream_t s) {
-  if (lb == 8)
-    return join_fill_pay2<K, uint64_t>(rb, oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key,
-                                       lv, l_by_row, out_lv, rv, r_by_row, out_rv, s);
-  return join_fill_pay2<K, uint32_t>(rb, oc, sl, lperm, L, sr, rperm, w, out_l, out_r, out_key, lv,
-                                     l_by_row, out_lv, rv, r_by_row, out_rv, s);
+// One side of jfx_join_fill_columns in the kernel's form, checked.
+int side_from_abi(const jfx_join_side* a, JoinSideOut& o) {
+  o = JoinSideOut{};
+  if (!a) return JFX_OK;
+  JFX_CHECK(a->ncols >= 0 && a->ncols <= JFX_JOIN_MAXCOLS, JFX_ERR_INVALID,
+            "a join side has 0 to 8 payload columns");
+  JFX_CHECK(a->record_bytes == 0 || a->record_bytes == 4 || a->record_bytes == 8,
+            JFX_ERR_UNSUPPORTED, "record_bytes must be 0, 4 or 8");
+  o.perm = a->rows;
+  o.out_rows = a->out_rows;
+  o.ncols = a->ncols;
+  o.by_row = a->by_row;
+  o.rec_bytes = a->record_bytes;
+  o.rec = a->records;
+  JFX_CHECK(!o.rec_bytes || o.rec, JFX_ERR_INVALID, "record_bytes without records");
+  JFX_CHECK(!o.rec_bytes || !o.by_row, JFX_ERR_INVALID, "records are read by sorted position");
+  JFX_CHECK(o.rec_bytes != 4 || o.ncols <= 1, JFX_ERR_INVALID, "a 4-byte record holds one column");
+  for (int j = 0; j < o.ncols; ++j) {
+    JFX_CHECK(a->col_bytes[j] == 4 || a->col_bytes[j] == 8, JFX_ERR_UNSUPPORTED,
+              "payload columns must be 4 or 8 bytes");
+    JFX_CHECK(a->out_cols[j] != nullptr, JFX_ERR_INVALID, "null payload output column");
+    JFX_CHECK(o.rec_bytes || a->cols[j] != nullptr, JFX_ERR_INVALID, "null payload column");
+    JFX_CHECK(!o.rec_bytes || (a->col_offset[j] >= 0 &&
+                               a->col_offset[j] + a->col_bytes[j] <= o.rec_bytes),
+              JFX_ERR_INVALID, "a column lies outside its record");
+    o.src[j] = a->cols[j];
+    o.bytes[j] = a->col_bytes[j];
+    o.off[j] = a->col_offset[j];
+    o.out[j] = a->out_cols[j];
+  }
+  return JFX_OK;
 }
 
 }  // namespace
@@ -392,14 +692,13 @@ int jfx_join_fill(int key_type, const void* sorted_left, const uint32_t* left_ro
   jfx::Carver c(workspace, ws_bytes);
   jfx::JoinWs w = jfx::carve_join(c, n_left);
   JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "join workspace too small");
-  const int oc = jfx::order_code(key_type);
-  hipStream_t s = jfx::as_stream(stream);
-  (void)n_right;
-  if (jfx::key_bytes(key_type) == 4)
-    return jfx::join_fill_typed<uint32_t>(oc, sorted_left, left_rows, n_left, sorted_right,
-                                          right_rows, w, out_left_rows, out_right_rows, out_keys, s);
-  return jfx::join_fill_typed<uint64_t>(oc, sorted_left, left_rows, n_left, sorted_right,
-                                        right_rows, w, out_left_rows, out_right_rows, out_keys, s);
+  jfx::JoinSideOut l{}, r{};
+  l.perm = left_rows;
+  l.out_rows = out_left_rows;
+  r.perm = right_rows;
+  r.out_rows = out_right_rows;
+  return jfx::join_fill_dispatch(key_type, sorted_left, n_left, sorted_right, w, out_keys, l, r,
+                                 jfx::as_stream(stream));
 }
 
 int jfx_join_fill_values(int key_type, const void* sorted_left, const uint32_t* left_rows,
@@ -425,19 +724,89 @@ int jfx_join_fill_values(int key_type, const void* sorted_left, const uint32_t* 
   jfx::Carver c(workspace, ws_bytes);
   jfx::JoinWs w = jfx::carve_join(c, n_left);
   JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "join workspace too small");
-  const int oc = jfx::order_code(key_type);
+  jfx::JoinSideOut l{}, r{};
+  l.perm = left_rows;
+  l.out_rows = out_left_rows;
+  l.by_row = left_by_row;
+  if (out_left_values) {
+    l.ncols = 1;
+    l.src[0] = left_values;
+    l.bytes[0] = left_elem_bytes;
+    l.out[0] = out_left_values;
+  }
+  r.perm = right_rows;
+  r.out_rows = out_right_rows;
+  r.by_row = right_by_row;
+  if (out_right_values) {
+    r.ncols = 1;
+    r.src[0] = right_values;
+    r.bytes[0] = right_elem_bytes;
+    r.out[0] = out_right_values;
+  }
+  return jfx::join_fill_dispatch(key_type, sorted_left, n_left, sorted_right, w, out_keys, l, r,
+                                 jfx::as_stream(stream));
+}
+
+int jfx_join_fill_columns(int key_type, const void* sorted_left, uint64_t n_left,
+                          const jfx_join_side* left, const void* sorted_right, uint64_t n_right,
+                          const jfx_join_side* right, void* out_keys, void* workspace,
+                          uint64_t ws_bytes, void* stream) {
+  JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64, JFX_ERR_INVALID, "bad key type");
+  if (n_left == 0 || n_right == 0) return JFX_OK;
+  JFX_CHECK(sorted_left && sorted_right && workspace, JFX_ERR_INVALID, "null argument");
+  jfx::JoinSideOut l{}, r{};
+  int rc = jfx::side_from_abi(left, l);
+  if (rc != JFX_OK) return rc;
+  rc = jfx::side_from_abi(right, r);
+  if (rc != JFX_OK) return rc;
+  jfx::Carver c(workspace, ws_bytes);
+  jfx::JoinWs w = jfx::carve_join(c, n_left);
+  JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "join workspace too small");
+  return jfx::join_fill_dispatch(key_type, sorted_left, n_left, sorted_right, w, out_keys, l, r,
+                                 jfx::as_stream(stream));
+}
+
+int jfx_pack_columns(int ncols, const void* const* cols, const int* col_bytes, uint64_t n,
+                     void* out, int* out_record_bytes, void* stream) {
+  JFX_CHECK(ncols >= 1 && ncols <= JFX_JOIN_MAXCOLS && cols && col_bytes, JFX_ERR_INVALID,
+            "1 to 8 columns");
+  jfx::JoinSideOut p{};
+  p.ncols = ncols;
+  int off = 0;
+  for (int j = 0; j < ncols; ++j) {
+    JFX_CHECK(col_bytes[j] == 4 || col_bytes[j] == 8, JFX_ERR_UNSUPPORTED,
+              "columns must be 4 or 8 bytes");
+    JFX_CHECK(n == 0 || cols[j] != nullptr, JFX_ERR_INVALID, "null column");
+    p.src[j] = cols[j];
+    p.bytes[j] = col_bytes[j];
+    p.off[j] = off;
+    off += col_bytes[j];
+  }
+  JFX_CHECK(off <= 8, JFX_ERR_UNSUPPORTED, "packed records hold at most 8 bytes");
+  p.rec_bytes = off;
+  p.out[0] = out;
+  if (out_record_bytes) *out_record_bytes = off;
+  if (n == 0) return JFX_OK;
+  JFX_CHECK(out != nullptr, JFX_ERR_INVALID, "null output");
   hipStream_t s = jfx::as_stream(stream);
-  if (jfx::key_bytes(key_type) == 4)
-    return jfx::join_fill_pay<uint32_t>(left_elem_bytes, right_elem_bytes, oc, sorted_left,
-                                        left_rows, n_left, sorted_right, right_rows, w,
-                                        out_left_rows, out_right_rows, out_keys, left_values,
-                                        left_by_row, out_left_values, right_values, right_by_row,
-                                        out_right_values, s);
-  return jfx::join_fill_pay<uint64_t>(left_elem_bytes, right_elem_bytes, oc, sorted_left, left_rows,
-                                      n_left, sorted_right, right_rows, w, out_left_rows,
-                                      out_right_rows, out_keys, left_values, left_by_row,
-                                      out_left_values, right_values, right_by_row,
-                                      out_right_values, s);
+  uint64_t first = 0;
+  if (ncols == 2 && col_bytes[0] == 4 && col_bytes[1] == 4 && (uintptr_t)cols[0] % 16 == 0 &&
+      (uintptr_t)cols[1] % 16 == 0 && (uintptr_t)out % 16 == 0) {
+    const uint64_t n4 = n / 4;
+    if (n4) {
+      const uint32_t b4 = (uint32_t)std::min<uint64_t>(4096, (n4 + 255) / 256);
+      JFX_LAUNCH("pack_cols", s, (jfx::k_pack_2x4<<<b4, 256, 0, s>>>(
+                                     (const uint4*)cols[0], (const uint4*)cols[1], n4, (uint4*)out)));
+      JFX_HIP(hipGetLastError());
+    }
+    first = n4 * 4;
+  }
+  if (first < n) {
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(8192, (n - first + 255) / 256);
+    JFX_LAUNCH("pack_cols", s, (jfx::k_pack_cols<<<blocks, 256, 0, s>>>(p, first, n)));
+    JFX_HIP(hipGetLastError());
+  }
+  return JFX_OK;
 }
 
 int jfx_join_first_match(int key_type, const void* sorted_probe, uint64_t n_probe,

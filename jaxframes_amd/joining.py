@@ -22,7 +22,7 @@ import ctypes
 import torch
 
 from . import _lib
-from ._tensors import key_code, to_device, widen_key
+from ._tensors import bits_view, key_code, to_device, widen_key
 from .sorting import radix_sort
 
 NO_ROW = 0xFFFFFFFF
@@ -109,39 +109,107 @@ def fill_values(kc, sl, lperm, L, sr, rperm, R, M, lsrc, l_by_row, rsrc, r_by_ro
     return keys, lv, rv, li, ri
 
 
+def pack_columns(cols: list) -> torch.Tensor:
+    """One record per row holding the columns' bytes in order (4 + 4 or one
+    column of up to 8 bytes): int32 or int64 storage."""
+    n = cols[0].shape[0]
+    width = sum(c.element_size() for c in cols)
+    out = torch.empty(n, dtype=torch.int32 if width == 4 else torch.int64, device=cols[0].device)
+    ptrs = (ctypes.c_void_p * len(cols))(*[c.data_ptr() for c in cols])
+    nbytes = (ctypes.c_int * len(cols))(*[c.element_size() for c in cols])
+    _lib.call("jfx_pack_columns", len(cols), ptrs, nbytes, n, _lib.ptr(out), None,
+              _stream(cols[0].device))
+    return out
+
+
+class _JoinSide:
+    """One side of the inner join, sorted: its payload either rides through
+    the radix sort packed into one record per row (at most 8 bytes of
+    columns: read in key order by the fill) or is gathered at the source row
+    the sort carried (wider payloads)."""
+
+    def __init__(self, keys, cols: list):
+        self.k, self.orig = widen_key(to_device(keys))
+        dev = self.k.device
+        self.cols = [payload_source(c, dev) for c in cols]
+        width = sum(c.element_size() for c in self.cols)
+        self.record_bytes = width if self.cols and width <= 8 else 0
+        self.perm = self.records = None
+        if not self.cols:
+            self.sorted, _ = radix_sort(self.k)
+        elif self.record_bytes:
+            rec = bits_view(self.cols[0]) if len(self.cols) == 1 else pack_columns(self.cols)
+            self.sorted, self.records = radix_sort(self.k, rec)
+        else:
+            self.sorted, _, self.perm = radix_sort(self.k, None, True, return_index=True,
+                                                   index_dtype=torch.int32)
+
+    def abi(self, outs: list) -> _lib.JoinSide:
+        s = _lib.JoinSide()
+        s.rows = _lib.ptr(self.perm).value or None
+        s.ncols = len(self.cols)
+        s.by_row = 0 if self.record_bytes else 1
+        s.record_bytes = self.record_bytes
+        s.records = _lib.ptr(self.records).value or None
+        off = 0
+        for j, (c, o) in enumerate(zip(self.cols, outs)):
+            s.cols[j] = c.data_ptr()
+            s.col_bytes[j] = c.element_size()
+            s.col_offset[j] = off
+            s.out_cols[j] = o.data_ptr()
+            off += c.element_size()
+        return s
+
+
 def _inner_join_fused(left_keys, left_values: dict, right_keys, right_values: dict):
-    """Inner join whose first payload column on each side is written by the
-    fill; further columns are gathered by row."""
-    lk, sl, lperm, lorig = _sorted_side(left_keys)
-    rk, sr, rperm, _ = _sorted_side(right_keys)
-    if lk.dtype != rk.dtype:
-        raise TypeError(f"join keys have different dtypes: {lk.dtype} vs {rk.dtype}")
-    dev = lk.device
-    L, R = lk.shape[0], rk.shape[0]
-    kc = key_code(lk)
+    """Inner join whose payload columns are all written by the fill
+    (jfx_join_fill_columns); payloads of at most 8 bytes per side are carried
+    through that side's sort, so the fill reads them in key order."""
+    lcols, rcols = list(left_values.items()), list(right_values.items())
+    if len(lcols) > _lib.JOIN_MAXCOLS or len(rcols) > _lib.JOIN_MAXCOLS:
+        return _inner_join_by_rows(left_keys, left_values, right_keys, right_values)
+    left = _JoinSide(left_keys, [v for _, v in lcols])
+    right = _JoinSide(right_keys, [v for _, v in rcols])
+    if left.k.dtype != right.k.dtype:
+        raise TypeError(f"join keys have different dtypes: {left.k.dtype} vs {right.k.dtype}")
+    dev = left.k.device
+    L, R = left.k.shape[0], right.k.shape[0]
+    kc = key_code(left.k)
     nb = ctypes.c_uint64(0)
     _lib.call("jfx_join_workspace_size", L, ctypes.byref(nb))
     ws = _lib.workspace(nb.value, dev)
     m = ctypes.c_uint64(0)
-    _lib.call("jfx_join_count", kc, _lib.ptr(sl), L, _lib.ptr(sr), R, ctypes.byref(m),
-              _lib.ptr(ws), ws.numel(), _stream(dev))
+    _lib.call("jfx_join_count", kc, _lib.ptr(left.sorted), L, _lib.ptr(right.sorted), R,
+              ctypes.byref(m), _lib.ptr(ws), ws.numel(), _stream(dev))
     M = m.value
-    if M >= (1 << 32):
-        raise _lib.JfxError(f"inner join produces {M} pairs; at most 2^32-1 are supported")
-    lcols, rcols = list(left_values.items()), list(right_values.items())
-    lsrc = payload_source(lcols[0][1], dev) if lcols else None
-    rsrc = payload_source(rcols[0][1], dev) if rcols else None
-    keys, lv, rv, li, ri = fill_values(kc, sl, lperm, L, sr, rperm, R, M, lsrc, True, rsrc, True,
-                                       ws, keys_dtype=lk.dtype,
-                                       want_rows=(len(lcols) > 1, len(rcols) > 1))
-    if keys.dtype != lorig:
-        keys = keys.to(lorig)
+    keys = torch.empty(M, dtype=left.k.dtype, device=dev)
+    louts = [torch.empty(M, dtype=c.dtype, device=dev) for c in left.cols]
+    routs = [torch.empty(M, dtype=c.dtype, device=dev) for c in right.cols]
+    if M:
+        ls, rs = left.abi(louts), right.abi(routs)
+        _lib.call("jfx_join_fill_columns", kc, _lib.ptr(left.sorted), L, ctypes.byref(ls),
+                  _lib.ptr(right.sorted), R, ctypes.byref(rs), _lib.ptr(keys), _lib.ptr(ws),
+                  ws.numel(), _stream(dev))
+    if keys.dtype != left.orig:
+        keys = keys.to(left.orig)
     out = {}
-    for j, (c, v) in enumerate(lcols):
-        out[f"left_{c}"] = lv if j == 0 else gather(v, li)
-    for j, (c, v) in enumerate(rcols):
-        out[f"right_{c}"] = rv if j == 0 else gather(v, ri)
+    for (c, _), o in zip(lcols, louts):
+        out[f"left_{c}"] = o
+    for (c, _), o in zip(rcols, routs):
+        out[f"right_{c}"] = o
     return keys, out
+
+
+def _inner_join_by_rows(left_keys, left_values: dict, right_keys, right_values: dict):
+    """Inner join through the row pairs and one gather per payload column
+    (for more than 8 payload columns on a side)."""
+    k, li, ri = inner_join_rows(left_keys, right_keys)
+    out = {}
+    for c, v in left_values.items():
+        out[f"left_{c}"] = gather(v, li)
+    for c, v in right_values.items():
+        out[f"right_{c}"] = gather(v, ri)
+    return k, out
 
 
 def gather(col: torch.Tensor, rows: torch.Tensor, fill_nan: bool = True) -> torch.Tensor:

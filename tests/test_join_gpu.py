@@ -108,3 +108,60 @@ def test_join_reference_vectors():
 def test_no_device_error_after_join():
     from jaxframes_amd import _lib
     assert _lib.device_error() == 0
+
+
+PAYLOAD_LAYOUTS = [
+    # (left column dtypes, right column dtypes): carried through the sort when
+    # a side's columns take at most 8 bytes, else gathered at the source row
+    ([], []),
+    ([np.float32], [np.float32]),
+    ([np.float32, np.float32], [np.float32, np.int32]),       # packed 4 + 4
+    ([np.float64], [np.int64]),                               # one 8-byte record
+    ([np.float32, np.float32, np.float32], [np.float64, np.float32]),  # by row
+    ([np.float32] * 9, [np.int16]),                           # beyond 8 columns
+]
+
+
+@pytest.mark.parametrize("lcols,rcols", PAYLOAD_LAYOUTS,
+                         ids=["none", "1+1", "2+2", "8B", "rows", "9cols"])
+def test_inner_join_payload_layouts(lcols, rcols):
+    rng = np.random.default_rng(len(lcols) * 10 + len(rcols))
+    L, R = 30000, 7000
+    lk = rng.integers(0, 5000, L).astype(np.int64)
+    rk = rng.integers(2000, 9000, R).astype(np.int64)
+
+    def col(dt, n):
+        return (rng.standard_normal(n) * 100).astype(dt) if np.dtype(dt).kind == "f" else \
+            rng.integers(-1000, 1000, n).astype(dt)
+    lv = {f"l{j}": col(dt, L) for j, dt in enumerate(lcols)}
+    rv = {f"r{j}": col(dt, R) for j, dt in enumerate(rcols)}
+    ek, eout, _, _ = oracle.sort_merge_join(lk, lv, rk, rv, "inner")
+    k, out = joining.sort_merge_join(cuda(lk), {c: cuda(v) for c, v in lv.items()},
+                                     cuda(rk), {c: cuda(v) for c, v in rv.items()}, "inner")
+    assert np.array_equal(k.cpu().numpy(), ek)
+    assert sorted(out) == sorted(eout)
+    for c, e in eout.items():
+        g = out[c].cpu().numpy()
+        assert g.dtype == e.dtype, (c, g.dtype, e.dtype)
+        assert np.array_equal(g, e), c
+
+
+def test_inner_join_heavy_tiles_split_into_chunks():
+    """Left rows whose key matches 3000 right rows: a 2048-row tile holds
+    millions of pairs, written in many fill chunks; pairs and payloads equal
+    the oracle's."""
+    rng = np.random.default_rng(21)
+    lk = np.concatenate([np.full(3000, 7), rng.integers(0, 50, 3000)]).astype(np.int64)
+    rng.shuffle(lk)
+    rk = np.concatenate([np.full(3000, 7), np.arange(0, 50)]).astype(np.int64)
+    rng.shuffle(rk)
+    lv = {"a": rng.standard_normal(lk.size).astype(np.float32)}
+    rv = {"c": rng.standard_normal(rk.size).astype(np.float32)}
+    _check_inner(lk, rk)
+    ek, eout, _, _ = oracle.sort_merge_join(lk, lv, rk, rv, "inner")
+    k, out = joining.sort_merge_join(cuda(lk), {c: cuda(v) for c, v in lv.items()},
+                                     cuda(rk), {c: cuda(v) for c, v in rv.items()}, "inner")
+    assert ek.size > 9_000_000
+    assert np.array_equal(k.cpu().numpy(), ek)
+    for c, e in eout.items():
+        assert np.array_equal(out[c].cpu().numpy(), e), c
