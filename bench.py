@@ -466,7 +466,7 @@ def workload_partition(reps: int, n: int = 200_000_000, nb: int = 256) -> dict:
     def bytes_of(name, launches):
         if name == "part_hist":
             return n * 8
-        if name.startswith("part_scatter"):
+        if name.startswith("part_scatter") or name.startswith("part_onesweep"):
             return n * 16
         return None
 

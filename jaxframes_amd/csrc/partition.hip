@@ -7,10 +7,16 @@
 // rows grouped by bucket (original order inside a bucket) plus P+1 offsets,
 // which is what an all-to-all with exact counts consumes.
 //   k_part_hist     one read of the keys; each chunk's bucket histogram
-//   k_chunk_colscan each chunk's starting offset per bucket
-//   k_part_offsets  bucket offsets (int64, P+1)
-//   k_chunk_scatter stable chunked scatter of keys (+ row index) by bucket
+//   k_chunk_colscan the bucket totals (and each chunk's starting offsets)
+//   k_part_offsets  bucket offsets (int64, P+1), the scatter's bucket starts,
+//                   its lookback descriptors cleared
+//   k_onesweep      the sort's stable scatter pass with the bucket as digit:
+//                   keys (+ row index) by bucket, input order kept inside a
+//                   bucket (k_chunk_scatter, the chunked scatter, with
+//                   JFX_PART_SCATTER=chunked)
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "chunked.hpp"
 #include "runtime.hpp"
@@ -92,15 +98,32 @@ __global__ void k_fill_i64(int64_t* p, uint32_t n, int64_t v) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Bucket offsets from the bucket totals: int64 offsets[nb + 1] for the
+// caller and, when base is non-null, the uint32 starts the onesweep scatter
+// adds its lookback offsets to; also clears the scatter's lookback
+// descriptors (ndesc words) and tile counter.
 __global__ void __launch_bounds__(1024)
-k_part_offsets(const uint32_t* total, uint32_t nb, int64_t* offsets) {
+k_part_offsets(const uint32_t* total, uint32_t nb, int64_t* offsets, uint32_t* base,
+               uint32_t* desc, uint64_t ndesc, uint32_t* tile_counter) {
+  if (blockIdx.x > 0) {
+    const uint64_t stride = (uint64_t)(gridDim.x - 1) * 1024;
+    for (uint64_t i = (uint64_t)(blockIdx.x - 1) * 1024 + threadIdx.x; i < ndesc; i += stride)
+      desc[i] = 0u;
+    return;
+  }
   __shared__ uint32_t s[2048];
   __shared__ uint32_t s_tmp[17];
   for (uint32_t i = threadIdx.x; i < 2048; i += 1024) s[i] = i < nb ? total[i] : 0u;
   __syncthreads();
   const uint32_t tot = block_excl_scan_lds<2048, 1024>(s, s_tmp);
-  for (uint32_t i = threadIdx.x; i < nb; i += 1024) offsets[i] = (int64_t)s[i];
-  if (threadIdx.x == 0) offsets[nb] = (int64_t)tot;
+  for (uint32_t i = threadIdx.x; i < nb; i += 1024) {
+    offsets[i] = (int64_t)s[i];
+    if (base) base[i] = s[i];
+  }
+  if (threadIdx.x == 0) {
+    offsets[nb] = (int64_t)tot;
+    if (tile_counter) *tile_counter = 0u;
+  }
 }
 
 // Multi-key row hash (the K > 1 branch of hash_multi_columns): seed 0x1337;
@@ -139,13 +162,20 @@ __global__ void __launch_bounds__(256) k_hash_rows(HashCols c, uint64_t n, uint3
 namespace {
 
 struct PartWs {
-  void* alt;  // unused, kept for layout clarity
   uint32_t* counts;
   uint32_t* colex;
   uint32_t* total;
+  uint32_t* base;     // [2048] bucket starts (onesweep scatter)
+  uint32_t* counter;  // onesweep tile counter
+  uint32_t* desc;     // [tiles][nb] onesweep lookback descriptors
   uint64_t chunk;
   uint32_t nchunks;
 };
+
+// onesweep scatter of the partition: the sort's tile of 8192 keys, 32-bit
+// keys as 16 waves x 8 keys and 64-bit keys as 8 x 16
+constexpr uint32_t PART_TILE = 8192;
+uint64_t part_tiles(uint64_t n) { return (n + PART_TILE - 1) / PART_TILE; }
 
 constexpr uint32_t PART_MAXCHUNKS = 512;
 
@@ -158,10 +188,23 @@ void part_geometry(PartWs& w, int kb, uint64_t n) {
   w.nchunks = (uint32_t)((n + w.chunk - 1) / w.chunk);
 }
 
-void part_carve(PartWs& w, Carver& c, uint32_t nb) {
+void part_carve(PartWs& w, Carver& c, uint32_t nb, uint64_t n) {
   w.counts = c.take<uint32_t>((uint64_t)PART_MAXCHUNKS * nb);
   w.colex = c.take<uint32_t>((uint64_t)PART_MAXCHUNKS * nb);
   w.total = c.take<uint32_t>(2048);
+  w.base = c.take<uint32_t>(2048);
+  w.counter = c.take<uint32_t>(64);
+  w.desc = c.take<uint32_t>(part_tiles(n) * nb);
+}
+
+template <typename K, typename PT, int D>
+void part_onesweep_launch(const ScatterArgs<K, PT>& a, HashDigit<K> hd, hipStream_t s) {
+  constexpr int W = sizeof(K) == 4 ? 16 : 8;
+  constexpr int KPT = PART_TILE / (W * 64);
+  static_assert(W * 64 * KPT == (int)PART_TILE, "partition tile shape");
+  JFX_LAUNCH(D == 8 ? "part_onesweep_d8" : "part_onesweep_d11", s,
+             (k_onesweep<K, PT, D, W, KPT, HashDigit<K>><<<(uint32_t)part_tiles(a.n), W * 64, 0, s>>>(
+                 a, hd)));
 }
 
 template <typename K, typename PT, int D>
@@ -171,6 +214,15 @@ void part_scatter_launch(const ScatterArgs<K, PT>& a, const ChunkGeom& g, const 
   JFX_LAUNCH(D == 8 ? "part_scatter_d8" : "part_scatter_d11", s,
              (k_chunk_scatter<K, PT, D, 8, KPT, false, 1, HashDigit<K>><<<g.nchunks, 512, 0, s>>>(
                  a, g, colex, total, hd)));
+}
+
+// Scatter of the partition: the sort's onesweep pass (tiles ranked in LDS,
+// digit offsets by decoupled lookback), or with JFX_PART_SCATTER=chunked the
+// chunked scatter fed by the histogram pass's chunk counts. Measured at 2e8
+// int64 keys, 256 buckets: see the partition profile.
+bool part_chunked() {
+  const char* e = getenv("JFX_PART_SCATTER");
+  return e && strcmp(e, "chunked") == 0;
 }
 
 template <typename K, typename PT>
@@ -186,7 +238,12 @@ int partition_typed(int kt, const void* keys, uint64_t n, uint32_t nb, int raw, 
   JFX_LAUNCH("chunk_colscan", s, (k_chunk_colscan<<<(nb + 3) / 4, 256, 0, s>>>(
                                      w.counts, nb, CM_DIRECT, 0u, nb, w.nchunks, w.colex, w.total)));
   JFX_HIP(hipGetLastError());
-  JFX_LAUNCH("part_offsets", s, (k_part_offsets<<<1, 1024, 0, s>>>(w.total, nb, out_offsets)));
+  const bool chunked = part_chunked();
+  const uint64_t ndesc = chunked ? 0 : part_tiles(n) * nb;
+  const uint32_t zero_blocks = ndesc ? (uint32_t)std::min<uint64_t>(256, (ndesc + 1023) / 1024) : 0u;
+  JFX_LAUNCH("part_offsets", s, (k_part_offsets<<<1 + zero_blocks, 1024, 0, s>>>(
+                                     w.total, nb, out_offsets, chunked ? nullptr : w.base, w.desc,
+                                     ndesc, chunked ? nullptr : w.counter)));
   JFX_HIP(hipGetLastError());
   ScatterArgs<K, PT> a{};
   a.keys_in = static_cast<const K*>(keys);
@@ -199,9 +256,18 @@ int partition_typed(int kt, const void* keys, uint64_t n, uint32_t nb, int raw, 
   a.write_keys = out_keys != nullptr;
   a.reverse = 0;
   a.iota = 1;
-  ChunkGeom g{n, w.chunk, w.nchunks};
-  if (nb <= 256) part_scatter_launch<K, PT, 8>(a, g, w.colex, w.total, hd, s);
-  else part_scatter_launch<K, PT, 11>(a, g, w.colex, w.total, hd, s);
+  if (chunked) {
+    ChunkGeom g{n, w.chunk, w.nchunks};
+    if (nb <= 256) part_scatter_launch<K, PT, 8>(a, g, w.colex, w.total, hd, s);
+    else part_scatter_launch<K, PT, 11>(a, g, w.colex, w.total, hd, s);
+  } else {
+    a.base = w.base;
+    a.desc = w.desc;
+    a.desc_next = nullptr;
+    a.tile_counter = w.counter;
+    if (nb <= 256) part_onesweep_launch<K, PT, 8>(a, hd, s);
+    else part_onesweep_launch<K, PT, 11>(a, hd, s);
+  }
   JFX_HIP(hipGetLastError());
   return JFX_OK;
 }
@@ -237,10 +303,9 @@ int jfx_hash_partition_workspace_size(int key_type, uint64_t n, uint32_t nbucket
   JFX_CHECK(out_bytes != nullptr, JFX_ERR_INVALID, "out_bytes is null");
   JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64, JFX_ERR_INVALID, "bad key type");
   JFX_CHECK(nbuckets >= 1 && nbuckets <= 2048, JFX_ERR_UNSUPPORTED, "1 to 2048 buckets");
-  (void)n;
   jfx::Carver c(nullptr, 0);
   jfx::PartWs w{};
-  jfx::part_carve(w, c, nbuckets);
+  jfx::part_carve(w, c, nbuckets, n);
   *out_bytes = c.used();
   return JFX_OK;
 }
@@ -263,7 +328,7 @@ int jfx_hash_partition(int key_type, const void* keys, uint64_t n, uint32_t nbuc
   JFX_CHECK(out_keys == nullptr || out_keys != keys, JFX_ERR_INVALID, "out_keys must not alias keys");
   jfx::Carver c(workspace, ws_bytes);
   jfx::PartWs w{};
-  jfx::part_carve(w, c, nbuckets);
+  jfx::part_carve(w, c, nbuckets, n);
   JFX_CHECK(c.fits(), JFX_ERR_WORKSPACE, "partition workspace too small");
   jfx::part_geometry(w, jfx::key_bytes(key_type), n);
   if (jfx::key_bytes(key_type) == 4) {
