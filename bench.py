@@ -128,6 +128,18 @@ def cpu_baseline(k: np.ndarray, v: np.ndarray, gpu_keys: np.ndarray, gpu_sums: n
     return base, check
 
 
+def _tuned(plan) -> dict:
+    """The sort configuration the autotuner chose for a batched plan, with
+    the timings of every candidate of its size class."""
+    from jaxframes_amd import batched
+    p0 = (plan.parts or [plan])[0]
+    waves, kpt = batched.SORT_SHAPES[plan.sort_shape][0 if p0.total_bits <= 32 else 1]
+    key = batched.Autotuner.key(p0.n, p0.total_bits)
+    return {"digit_bits": plan.digit_bits, "sort_shape": plan.sort_shape,
+            "waves_per_tile": waves, "keys_per_lane": kpt, "tile_keys": waves * 64 * kpt,
+            "size_class": key, "candidates_ms": batched.TUNER.timings.get(key)}
+
+
 def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 256,
                      plain: bool = True):
     """Batched groupby of 4096 frames (row counts log-uniform in [1K, 4M]
@@ -169,7 +181,7 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
     # per-kernel HIP-event times of eager launches of the same pipeline
     eager = _measure(lambda: plan.launch(graph=False), 3, bytes_of)
     # the plain path is timed without the plan's buffers held by the allocator
-    tuned = {"digit_bits": plan.digit_bits}
+    tuned = _tuned(plan)
     del plan
     torch.cuda.empty_cache()
     if not plain:
@@ -202,7 +214,7 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
            "plain_rows_per_s": round(plain_rps, 1),
            "plain_sample": f"first {plain_frames} frames, {rows_plain} rows",
            "speedup_vs_plain": round(n / dt / plain_rps, 2),
-           "hip_graph": True, "digit_bits": digit_bits, "pipelines": pipelines,
+           "hip_graph": True, "tuned": tuned, "pipelines": pipelines,
            "eager_ms_per_batch": eager["ms_per_call"],
            "alg_bytes_per_batch": eager["alg_bytes_per_call"],
            "batch_hbm_frac": round(eager["alg_bytes_per_call"] / dt / 1e9 / PEAK_HBM_GBPS, 4),

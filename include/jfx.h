@@ -307,11 +307,17 @@ int jfx_gather(int elem_bytes, const void* src, const uint32_t* rows, uint64_t n
  * groups. No counterpart in the reference, which runs one groupby per frame. */
 int jfx_batched_groupby_workspace_size(uint32_t total_bits, int val_type, uint64_t n,
                                        uint64_t* out_bytes);
+/* sort_shape picks the onesweep tile shape of the composite sort (waves x
+ * keys per lane, for 32-bit / 64-bit composites): 0 = 16 x 8 / 8 x 16 and 1 =
+ * 8 x 16 / 16 x 8 (8192-key tiles), 2 = 8 x 8 / 4 x 16 and 3 = 4 x 16 / 8 x 8
+ * (4096-key tiles); with digit_bits (8 or 11) it is what the autotuner of
+ * jaxframes_amd.batched chooses per size class. */
 int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
                         const int64_t* frame_offsets, uint32_t nframes, uint64_t key_min,
-                        uint32_t key_bits, int digit_bits, void* out_keys, void* out_sum,
-                        int64_t* out_count, void* out_min, void* out_max, void* out_mean,
-                        uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes, void* stream);
+                        uint32_t key_bits, int digit_bits, int sort_shape, void* out_keys,
+                        void* out_sum, int64_t* out_count, void* out_min, void* out_max,
+                        void* out_mean, uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes,
+                        void* stream);
 
 /* ---- indexed filter -> groupby (serving) ------------------------------- */
 /* Per group of a frame indexed by (key, value) -- values ascending inside each

@@ -86,8 +86,8 @@ _SIGNATURES = {
     "jfx_gather": [c_int, c_vp, c_vp, c_u64, c_vp, c_u64, c_vp],
     "jfx_batched_groupby_workspace_size": [ctypes.c_uint32, c_int, c_u64, c_u64p],
     "jfx_batched_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, ctypes.c_uint32, c_u64,
-                            ctypes.c_uint32, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
-                            c_u64, c_vp],
+                            ctypes.c_uint32, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                            c_vp, c_u64, c_vp],
     "jfx_served_join_workspace_size": [c_u64, ctypes.c_uint32, c_u64p],
     "jfx_served_join_count": [c_int, c_vp, c_vp, ctypes.c_uint32, c_vp, c_u64, c_u64p, c_vp, c_u64,
                               c_vp],

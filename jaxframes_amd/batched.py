@@ -7,9 +7,9 @@ frame by frame, with a CSR offset array per frame.
 
 The device work is jfx_batched_groupby (composite (frame, key) keys, one radix
 sort, one segmented reduction) and has no host synchronisation, so a plan can
-be captured once in a HIP graph and replayed; the digit width of the sort is
-picked per size class by a small autotuner that times the candidates on the
-first call.
+be captured once in a HIP graph and replayed; the digit width and the tile
+shape of the sort are picked per size class by a small autotuner that times
+the candidates on the first call.
 """
 
 from __future__ import annotations
@@ -29,20 +29,30 @@ from .grouping import _VAL_CODE, _alloc_out, _value_tensor
 AGGS = ("sum", "count", "min", "max", "mean")
 
 
-class Autotuner:
-    """Best digit width (8 or 11 bits per radix pass) per (row-count class,
-    composite bit width), measured on first use. Thread-safe; optionally
-    persisted as JSON (path from JFX_TUNE_FILE)."""
+# onesweep tile shapes of jfx_batched_groupby (sort_shape): (waves, keys per
+# lane) for 32-bit and for 64-bit composite keys
+SORT_SHAPES = {0: ((16, 8), (8, 16)), 1: ((8, 16), (16, 8)), 2: ((8, 8), (4, 16)),
+               3: ((4, 16), (8, 8))}
 
-    CANDIDATES = (8, 11)
+
+class Autotuner:
+    """Best sort configuration -- digit width (8 or 11 bits per radix pass)
+    and onesweep tile shape (waves per workgroup x keys per lane, which sets
+    the tile size) -- per (row-count class, composite bit width), measured
+    on first use by timing every candidate. Thread-safe; optionally persisted
+    as JSON (path from JFX_TUNE_FILE)."""
+
+    CANDIDATES = tuple((d, sh) for d in (8, 11) for sh in sorted(SORT_SHAPES))
 
     def __init__(self, path: str | None = None):
         self._lock = threading.Lock()
         self.path = path if path is not None else os.environ.get("JFX_TUNE_FILE")
-        self.table: dict[str, int] = {}
+        self.table: dict[str, tuple] = {}
+        self.timings: dict[str, dict] = {}
         if self.path and os.path.exists(self.path):
             with open(self.path) as f:
-                self.table = {k: int(v) for k, v in json.load(f).items()}
+                for k, v in json.load(f).items():
+                    self.table[k] = (int(v), 0) if isinstance(v, int) else (int(v[0]), int(v[1]))
 
     @staticmethod
     def key(n: int, total_bits: int) -> str:
@@ -52,28 +62,31 @@ class Autotuner:
         with self._lock:
             return self.table.get(self.key(n, total_bits))
 
-    def tune(self, n: int, total_bits: int, run, reps: int = 2) -> int:
-        """run(digit_bits) launches one batch on the current stream."""
+    def tune(self, n: int, total_bits: int, run, reps: int = 2) -> tuple:
+        """run(digit_bits, sort_shape) launches one batch on the current
+        stream; returns the fastest (digit_bits, sort_shape)."""
         k = self.key(n, total_bits)
         with self._lock:
             if k in self.table:
                 return self.table[k]
-        best, best_t = None, None
-        for d in self.CANDIDATES:
-            run(d)
+        best, best_t, times = None, None, {}
+        for cand in self.CANDIDATES:
+            run(*cand)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
-                run(d)
+                run(*cand)
             torch.cuda.synchronize()
             t = (time.perf_counter() - t0) / reps
+            times[f"d{cand[0]}/shape{cand[1]}"] = round(t * 1e3, 3)
             if best_t is None or t < best_t:
-                best, best_t = d, t
+                best, best_t = cand, t
         with self._lock:
             self.table[k] = best
+            self.timings[k] = times
             if self.path:
                 with open(self.path, "w") as f:
-                    json.dump(self.table, f)
+                    json.dump({kk: list(v) for kk, v in self.table.items()}, f)
         return best
 
 
@@ -110,7 +123,8 @@ class BatchedGroupBy:
     """
 
     def __init__(self, keys, values, frame_offsets, *, key_range=None, aggs=("sum",),
-                 digit_bits: int = 0, tuner: Autotuner | None = TUNER, max_rows: int = MAX_ROWS):
+                 digit_bits: int = 0, sort_shape: int = 0, tuner: Autotuner | None = TUNER,
+                 max_rows: int = MAX_ROWS):
         _lib.require_gpu()
         for a in aggs:
             if a not in AGGS:
@@ -133,7 +147,7 @@ class BatchedGroupBy:
         self.parts = None
         self.aggs = tuple(aggs)
         if self.n > max_rows:
-            self._split(key_range, digit_bits, tuner, max_rows)
+            self._split(key_range, digit_bits, sort_shape, tuner, max_rows)
             return
         self.kmin, kmax = int(key_range[0]), int(key_range[1])
         span = kmax - self.kmin
@@ -155,13 +169,13 @@ class BatchedGroupBy:
         self.ws = torch.empty(nb.value, dtype=torch.uint8, device=self.dev)
         self.graph = None
         if digit_bits:
-            self.digit_bits = digit_bits
+            self.digit_bits, self.sort_shape = digit_bits, sort_shape
         elif tuner is not None:
-            self.digit_bits = tuner.tune(self.n, self.total_bits, self._launch)
+            self.digit_bits, self.sort_shape = tuner.tune(self.n, self.total_bits, self._launch)
         else:
-            self.digit_bits = 8
+            self.digit_bits, self.sort_shape = 8, sort_shape
 
-    def _split(self, key_range, digit_bits, tuner, max_rows):
+    def _split(self, key_range, digit_bits, sort_shape, tuner, max_rows):
         offs = self.offsets.cpu().tolist()
         cuts, start = [0], 0
         for f in range(self.nframes):
@@ -177,16 +191,18 @@ class BatchedGroupBy:
             sub = torch.tensor([o - r0 for o in offs[a:b + 1]], dtype=torch.int64)
             self.parts.append(BatchedGroupBy(self.keys[r0:r1], self.vals[r0:r1], sub,
                                              key_range=key_range, aggs=self.aggs,
-                                             digit_bits=digit_bits, tuner=tuner,
-                                             max_rows=max_rows))
+                                             digit_bits=digit_bits, sort_shape=sort_shape,
+                                             tuner=tuner, max_rows=max_rows))
         self.digit_bits = self.parts[0].digit_bits
+        self.sort_shape = self.parts[0].sort_shape
 
-    def _launch(self, digit_bits: int | None = None, stream=None):
+    def _launch(self, digit_bits: int | None = None, sort_shape: int | None = None, stream=None):
         s = stream if stream is not None else torch.cuda.current_stream(self.dev)
         ptr = lambda a: _lib.ptr(self.outs.get(a))
         _lib.call("jfx_batched_groupby", self.kc, _lib.ptr(self.keys), self.vcode, _lib.ptr(self.vals),
                   self.n, _lib.ptr(self.offsets), self.nframes, self.kmin & ((1 << 64) - 1),
-                  self.key_bits, digit_bits or self.digit_bits, _lib.ptr(self.out_keys),
+                  self.key_bits, digit_bits or self.digit_bits,
+                  self.sort_shape if sort_shape is None else sort_shape, _lib.ptr(self.out_keys),
                   ptr("sum"), ptr("count"), ptr("min"), ptr("max"), ptr("mean"),
                   _lib.ptr(self.ngroups), _lib.ptr(self.ws), self.ws.numel(),
                   ctypes.c_void_p(s.cuda_stream))

@@ -1235,16 +1235,17 @@ int jfx_batched_groupby_workspace_size(uint32_t total_bits, int val_type, uint64
   c.take<char>(n * cb);
   c.take<char>(n * vb);
   jfx::carve_seg(c, vb, n);
-  c.take<char>(jfx::sort_ws_bytes(cb == 4 ? JFX_U32 : JFX_U64, vb, n));
+  c.take<char>(jfx::sort_ws_bytes(cb == 4 ? JFX_U32 : JFX_U64, vb, n, true));
   *out_bytes = c.used();
   return JFX_OK;
 }
 
 int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void* vals, uint64_t n,
                         const int64_t* frame_offsets, uint32_t nframes, uint64_t key_min,
-                        uint32_t key_bits, int digit_bits, void* out_keys, void* out_sum,
-                        int64_t* out_count, void* out_min, void* out_max, void* out_mean,
-                        uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes, void* stream) {
+                        uint32_t key_bits, int digit_bits, int sort_shape, void* out_keys,
+                        void* out_sum, int64_t* out_count, void* out_min, void* out_max,
+                        void* out_mean, uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes,
+                        void* stream) {
   JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64 && jfx::int_key(key_type), JFX_ERR_INVALID,
             "batched keys must be integers");
   JFX_CHECK(jfx::valid_vt(val_type), JFX_ERR_INVALID, "bad value type");
@@ -1258,6 +1259,8 @@ int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void
   JFX_CHECK(out_ngroups != nullptr, JFX_ERR_INVALID, "out_ngroups is null");
   JFX_CHECK(digit_bits == 0 || digit_bits == 8 || digit_bits == 11, JFX_ERR_INVALID,
             "digit_bits must be 0, 8 or 11");
+  JFX_CHECK(sort_shape >= 0 && sort_shape < jfx::SORT_SHAPES, JFX_ERR_INVALID,
+            "sort_shape must be 0 to 3");
   hipStream_t s = jfx::as_stream(stream);
   if (n == 0) {
     JFX_LAUNCH("copy_ngroups", s, (jfx::k_zero_u32<<<1, 64, 0, s>>>(out_ngroups)));
@@ -1272,7 +1275,7 @@ int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void
   void* scomp = cv.take<char>(n * cb);
   void* svals = cv.take<char>(n * vb);
   jfx::SegWs w = jfx::carve_seg(cv, vb, n);
-  const uint64_t sws = jfx::sort_ws_bytes(ckt, vb, n);
+  const uint64_t sws = jfx::sort_ws_bytes(ckt, vb, n, true);
   void* sort_ws = cv.take<char>(sws);
   JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "batched groupby workspace too small");
   uint32_t* err = jfx::device_error_word();
@@ -1305,7 +1308,8 @@ int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void
   }
   JFX_HIP(hipGetLastError());
   jfx::SortCtx ctx;
-  int rc = jfx::sort_prepare_known(ctx, ckt, comp, n, vb, digit_bits, total, sort_ws, sws, s);
+  int rc = jfx::sort_prepare_known(ctx, ckt, comp, n, vb, digit_bits, total, sort_ws, sws, s,
+                                   sort_shape, true);
   if (rc != JFX_OK) return rc;
   rc = jfx::sort_execute(ctx, comp, scomp, 1, vals, svals, 0, 0);
   if (rc != JFX_OK) return rc;

@@ -375,7 +375,10 @@ void geometry(SortCtx& c) {
   c.nchunks = (uint32_t)((c.n + c.chunk - 1) / c.chunk);
 }
 
-uint64_t os_tiles(uint64_t n) { return (n + OS_TILE - 1) / OS_TILE; }
+uint64_t os_tiles(uint64_t n, uint32_t tile = OS_TILE) { return (n + tile - 1) / tile; }
+
+// keys per tile of a shape
+uint32_t shape_tile(int shape) { return shape >= 2 ? 4096u : 8192u; }
 
 void carve(SortCtx& c, Carver& cv) {
   c.alt_keys = cv.take<char>(c.n * c.kb);
@@ -390,8 +393,8 @@ void carve(SortCtx& c, Carver& cv) {
   c.maxbin = cv.take<uint32_t>(SORT_MAXCHUNKS);
   c.counters = cv.take<uint32_t>(64);
   c.hdr = cv.take<uint32_t>(16);
-  c.desc[0] = cv.take<uint32_t>(os_tiles(c.n) * SORT_MAXR);
-  c.desc[1] = cv.take<uint32_t>(os_tiles(c.n) * SORT_MAXR);
+  c.desc[0] = cv.take<uint32_t>(os_tiles(c.n, c.desc_tile) * SORT_MAXR);
+  c.desc[1] = cv.take<uint32_t>(os_tiles(c.n, c.desc_tile) * SORT_MAXR);
 }
 
 template <typename K, typename PT>
@@ -403,13 +406,27 @@ struct Passes {
   static constexpr int W = sizeof(K) == 4 ? 16 : 8;
   static constexpr int KPT = OS_TILE / (W * 64);
   static_assert(W * 64 * KPT == OS_TILE, "onesweep tile shape");
-  static hipError_t onesweep(int D, const ScatterArgs<K, PT>& a, RadixDigit<K> dig, hipStream_t s) {
-    const uint32_t tiles = (uint32_t)os_tiles(a.n);
+  template <int W_, int KPT_>
+  static void onesweep_wk(int D, const ScatterArgs<K, PT>& a, RadixDigit<K> dig, hipStream_t s) {
+    const uint32_t tiles = (uint32_t)os_tiles(a.n, W_ * 64 * KPT_);
     if (D == 8) {
-      JFX_LAUNCH("onesweep_d8", s, (k_onesweep<K, PT, 8, W, KPT, RadixDigit<K>><<<tiles, W * 64, 0, s>>>(a, dig)));
+      JFX_LAUNCH("onesweep_d8", s, (k_onesweep<K, PT, 8, W_, KPT_, RadixDigit<K>><<<tiles, W_ * 64, 0, s>>>(a, dig)));
     } else {
-      JFX_LAUNCH("onesweep_d11", s, (k_onesweep<K, PT, 11, W, KPT, RadixDigit<K>><<<tiles, W * 64, 0, s>>>(a, dig)));
+      JFX_LAUNCH("onesweep_d11", s, (k_onesweep<K, PT, 11, W_, KPT_, RadixDigit<K>><<<tiles, W_ * 64, 0, s>>>(a, dig)));
     }
+  }
+  static hipError_t onesweep(int D, int shape, const ScatterArgs<K, PT>& a, RadixDigit<K> dig,
+                             hipStream_t s) {
+    constexpr bool K4 = sizeof(K) == 4;
+    if constexpr (!std::is_same<PT, NoPay>::value) {
+      switch (shape) {
+        case 1: onesweep_wk<K4 ? 8 : 16, K4 ? 16 : 8>(D, a, dig, s); return hipGetLastError();
+        case 2: onesweep_wk<K4 ? 8 : 4, K4 ? 8 : 16>(D, a, dig, s); return hipGetLastError();
+        case 3: onesweep_wk<K4 ? 4 : 8, K4 ? 16 : 8>(D, a, dig, s); return hipGetLastError();
+        default: break;
+      }
+    }
+    onesweep_wk<W, KPT>(D, a, dig, s);
     return hipGetLastError();
   }
   template <int D, int W, int KPT, bool PF, int WPE>
@@ -516,7 +533,7 @@ int multi_pass_typed(SortCtx& c, const void* keys_in, void* keys_out, int want_k
   for (int p = 0; p < npass; ++p)
     if ((vary >> p) & 1u) passes.push_back(p);
   const int A = (int)passes.size();
-  const uint64_t tiles = os_tiles(c.n);
+  const uint64_t tiles = os_tiles(c.n, shape_tile(c.shape));
   // every layout pass is a onesweep pass: measured faster than a chunked
   // first pass fed by the statistics pass's chunk histograms (692 vs 814 us
   // per pass at 100M u64 keys + u32 rows, which also wrote 27% more)
@@ -545,7 +562,7 @@ int multi_pass_typed(SortCtx& c, const void* keys_in, void* keys_out, int want_k
     args.reverse = (last && descending) ? 1 : 0;
     args.iota = (a == 0 && iota) ? 1 : 0;
     RadixDigit<K> dig{c.oc, (uint32_t)(passes[a] * D), (K)0};
-    JFX_HIP(Passes<K, PT>::onesweep(D, args, dig, c.s));
+    JFX_HIP(Passes<K, PT>::onesweep(D, c.shape, args, dig, c.s));
     src_k = dst_k;
     src_p = dst_p;
   }
@@ -568,11 +585,12 @@ int execute_k(SortCtx& c, const void* ki, void* ko, int want_keys, const void* p
 
 }  // namespace
 
-uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n) {
+uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n, bool small_tiles) {
   SortCtx c;
   c.kb = key_bytes(kt);
   c.pb = pay_bytes;
   c.n = n;
+  c.desc_tile = small_tiles ? 4096u : 8192u;
   Carver cv(nullptr, 0);
   carve(c, cv);
   return cv.used();
@@ -620,8 +638,13 @@ int sort_prepare(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_by
 
 int sort_prepare_known(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_bytes,
                        int digit_bits, uint32_t key_bits, void* ws, uint64_t ws_bytes,
-                       hipStream_t s) {
+                       hipStream_t s, int shape, bool small_tiles) {
   JFX_CHECK(kt == JFX_U32 || kt == JFX_U64, JFX_ERR_INVALID, "known-range sorts take unsigned keys");
+  JFX_CHECK(shape >= 0 && shape < SORT_SHAPES, JFX_ERR_INVALID, "bad sort shape");
+  JFX_CHECK(small_tiles || shape_tile(shape) == 8192u, JFX_ERR_INVALID,
+            "4096-key tiles need a workspace carved for them");
+  c.shape = shape;
+  c.desc_tile = small_tiles ? 4096u : 8192u;
   JFX_CHECK(key_bits >= 1 && key_bits <= (uint32_t)key_bytes(kt) * 8, JFX_ERR_INVALID, "bad key_bits");
   c.kt = kt;
   c.kb = key_bytes(kt);

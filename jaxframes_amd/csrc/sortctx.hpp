@@ -18,6 +18,8 @@ struct SortCtx {
   int oc = 0;            // order code
   int pb = 0;            // payload bytes (0, 4, 8)
   int digit_bits = 8;    // layout digit width for multi-pass sorts
+  int shape = 0;         // onesweep tile shape (SORT_SHAPES)
+  uint32_t desc_tile = 8192;  // keys per tile the lookback descriptors are carved for
   uint64_t n = 0;
   hipStream_t s = nullptr;
   // chunk geometry of the chunked passes
@@ -50,7 +52,14 @@ struct SortCtx {
   uint32_t range_radix() const { return (uint32_t)(mx - mn) + 1u; }
 };
 
-uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n);
+// Onesweep tile shapes (waves x keys per lane): 0 = 32-bit keys 16 x 8,
+// 64-bit keys 8 x 16 (8192-key tiles, the default); 1 = 8 x 16 / 16 x 8
+// (8192); 2 = 8 x 8 / 4 x 16 (4096); 3 = 4 x 16 / 8 x 8 (4096). Shapes 1-3
+// are taken by sorts that carry a payload.
+constexpr int SORT_SHAPES = 4;
+
+// small_tiles: carve lookback descriptors for 4096-key tiles (any shape).
+uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n, bool small_tiles = false);
 int default_digit_bits(int kt, uint64_t n);
 
 // Carve the workspace and run the statistics pass; synchronises the stream
@@ -67,7 +76,7 @@ int sort_prepare(SortCtx& ctx, int kt, const void* keys_in, uint64_t n, int pay_
 // host read, every digit below key_bits is sorted (graph-capturable).
 int sort_prepare_known(SortCtx& c, int kt, const void* keys_in, uint64_t n, int pay_bytes,
                        int digit_bits, uint32_t key_bits, void* ws, uint64_t ws_bytes,
-                       hipStream_t s);
+                       hipStream_t s, int shape = 0, bool small_tiles = false);
 int sort_execute(SortCtx& ctx, const void* keys_in, void* keys_out, int want_keys,
                  const void* pay_in, void* pay_out, int iota, int descending);
 

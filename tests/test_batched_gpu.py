@@ -83,16 +83,6 @@ def test_batched_graph_replays_match_eager():
     _check(r, keys, vals, offs, ("sum", "count"))
 
 
-def test_autotuner_picks_and_caches(tmp_path):
-    keys, vals, offs = _frames(7, 20, 20000, 1000)
-    tuner = batched.Autotuner(str(tmp_path / "tune.json"))
-    plan = batched.BatchedGroupBy(cuda(keys), cuda(vals), offs, tuner=tuner)
-    assert plan.digit_bits in (8, 11)
-    assert tuner.lookup(plan.n, plan.total_bits) == plan.digit_bits
-    again = batched.Autotuner(str(tmp_path / "tune.json"))
-    assert again.lookup(plan.n, plan.total_bits) == plan.digit_bits
-
-
 def test_key_outside_declared_range_raises_device_error():
     keys = np.array([0, 1, 2, 50, 3], dtype=np.int32)
     vals = np.ones(5, dtype=np.float32)
@@ -137,3 +127,33 @@ def test_oversize_batch_is_split_at_frame_boundaries():
         assert torch.all(d <= 1e-6 * (1 + whole.values["sum"].double().abs()))
     with pytest.raises(ValueError):
         batched.BatchedGroupBy(keys, vals, offs, digit_bits=8, tuner=None, max_rows=999)
+
+
+@pytest.mark.parametrize("digit_bits", [8, 11])
+@pytest.mark.parametrize("sort_shape", [0, 1, 2, 3])
+def test_batched_every_sort_configuration(digit_bits, sort_shape):
+    """Every (digit width, tile shape) the autotuner may choose gives the
+    per-frame oracle's groups, for 32-bit and 64-bit composite keys."""
+    keys, vals, offs = _frames(100 + sort_shape, 50, 60000, 5000)
+    plan = batched.BatchedGroupBy(cuda(keys), cuda(vals), offs, aggs=("sum", "count", "max"),
+                                  digit_bits=digit_bits, sort_shape=sort_shape)
+    _check(plan.run(), keys, vals, offs, ("sum", "count", "max"))
+    wide = keys.astype(np.int64) * 3_000_000_000
+    plan = batched.BatchedGroupBy(cuda(wide), cuda(vals), offs, aggs=("sum", "count"),
+                                  digit_bits=digit_bits, sort_shape=sort_shape)
+    assert plan.total_bits > 32
+    _check(plan.run(), wide, vals, offs, ("sum", "count"))
+    assert _lib.device_error() == 0
+
+
+def test_autotuner_picks_and_caches(tmp_path):
+    keys, vals, offs = _frames(7, 20, 20000, 300)
+    path = tmp_path / "tune.json"
+    tuner = batched.Autotuner(str(path))
+    plan = batched.BatchedGroupBy(cuda(keys), cuda(vals), offs, aggs=("sum",), tuner=tuner)
+    assert (plan.digit_bits, plan.sort_shape) in batched.Autotuner.CANDIDATES
+    assert len(tuner.timings[batched.Autotuner.key(plan.n, plan.total_bits)]) == len(
+        batched.Autotuner.CANDIDATES)
+    again = batched.Autotuner(str(path))
+    assert again.lookup(plan.n, plan.total_bits) == (plan.digit_bits, plan.sort_shape)
+    _check(plan.run(), keys, vals, offs, ("sum",))
