@@ -225,39 +225,60 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
 
 
 def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
-                    budget_bytes: int = 64 << 30, reps: int = 5):
+                    budget_bytes: int = 64 << 30, reps: int = 5,
+                    switch_interval: float | None = None):
     """The served stream: 8 resident frames (1M..50M rows, int64 keys of
     cardinality 10..1M, float32 values), 70% filter -> groupby(sum, count,
     mean) and 30% inner joins with 1e5..1e6-row dimension tables, Poisson
     arrivals at each rate in qps_levels through the request loop (slots
     workers, HBM budget), against the plain path (the same queries one at a
-    time with no resident index and no batching). Also the hash partition of
-    2e8 uniform int64 keys into 256 buckets."""
+    time with no resident index and no batching) and against the served
+    executors run one query at a time on the resident indexes (so the gain
+    of the indexes and the gain of batching and overlap are reported
+    apart), with each frame's index build time. switch_interval, when set,
+    is the interpreter's GIL switch interval while the server is open (a
+    process-wide setting, restored when it closes; by default untouched).
+    Also the hash partition of 2e8 uniform int64 keys into 256 buckets."""
     import torch
     from jaxframes_amd import datagen, partition, serving
     frames = datagen.served_frames()
     store = serving.FrameStore(budget_bytes)
+    index_build = []
     for name, k, v, _ in frames:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         store.load(name, k, v)
+        torch.cuda.synchronize()
+        index_build.append({"frame": name, "rows": int(k.shape[0]),
+                            "groups": store[name].ngroups,
+                            "ms": round((time.perf_counter() - t0) * 1e3, 2)})
     qs = datagen.served_queries(frames, nq)
     rows = sum(store[q.frame].n for q in qs)
     warm = qs[:8]
     for q in warm:
         serving.run_plain(store, q)
+        serving.execute(store, q)
     torch.cuda.synchronize()
-    plain_runs = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        for q in qs:
-            serving.run_plain(store, q)
-        torch.cuda.synchronize()
-        plain_runs.append(time.perf_counter() - t0)
-    plain_s = sorted(plain_runs)[len(plain_runs) // 2]
+
+    def median_pass(run):
+        runs = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for q in qs:
+                run(store, q)
+            torch.cuda.synchronize()
+            runs.append(time.perf_counter() - t0)
+        return sorted(runs)[len(runs) // 2]
+    # the plain path: no resident index, one query at a time
+    plain_s = median_pass(serving.run_plain)
+    # the served executors on the resident indexes, still one query at a
+    # time on one stream: the gain of the indexes alone
+    seq_s = median_pass(serving.execute)
     levels = []
     for qps in qps_levels:
         runs = []
         for rep in range(reps):   # the replay with the median wall time is reported
-            srv = serving.Server(store, slots=slots)
+            srv = serving.Server(store, slots=slots, switch_interval=switch_interval)
             srv.warmup(qs)   # every query once per slot stream: allocator pools are per stream
             arr = serving.poisson_arrivals(len(qs), qps, seed=3 + rep)
             t0 = time.perf_counter()
@@ -275,7 +296,8 @@ def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
                        "p99_ms": round(float(np.percentile(lat, 99)), 3),
                        "peak_running": srv.peak_running,
                        "mean_batch": round(float(np.mean(srv.batches)), 2),
-                       "speedup_vs_plain": round(plain_s / wall, 2)})
+                       "speedup_vs_plain": round(plain_s / wall, 2),
+                       "speedup_vs_indexed_sequential": round(seq_s / wall, 2)})
     pk = datagen.partition_keys(200_000_000)
     partition.hash_partition(pk, 256, return_keys=True, return_rows=False)
     torch.cuda.synchronize()
@@ -291,8 +313,13 @@ def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
                         "-> groupby(k).agg(sum, count, mean), 30% inner join with a "
                         "1e5..1e6-row dimension table; Poisson arrivals",
             "queries": len(qs), "rows": rows, "slots": slots, "hbm_budget_bytes": budget_bytes,
+            "gil_switch_interval_s": switch_interval,
+            "index_build": index_build,
             "plain_rows_per_s": round(rows / plain_s, 1),
-            "plain_qps": round(len(qs) / plain_s, 1), "levels": levels,
+            "plain_qps": round(len(qs) / plain_s, 1),
+            "indexed_sequential_rows_per_s": round(rows / seq_s, 1),
+            "index_gain": round(plain_s / seq_s, 2),
+            "levels": levels,
             "hash_partition": {"workload": "2e8 uniform int64 keys -> 256 buckets (fmix32)",
                                "ms": round(pdt * 1e3, 3),
                                "keys_per_s": round(2e8 / pdt, 1)}}

@@ -173,10 +173,14 @@ def test_batch_failure_reaches_every_member():
     srv.close()
 
 
-def test_switch_interval_set_while_serving_and_restored():
+def test_switch_interval_untouched_by_default_set_only_when_asked_and_restored():
     import sys
     before = sys.getswitchinterval()
     srv, _ = _make(slots=1)
+    assert sys.getswitchinterval() == before
+    srv.close()
+    srv = Server(FrameStore(100), slots=1, hbm_budget_bytes=100, executor=lambda st, q: None,
+                 use_streams=False, footprint_fn=lambda st, q: 1, switch_interval=1e-4)
     assert abs(sys.getswitchinterval() - 1e-4) < 1e-9
     srv.close()
     assert sys.getswitchinterval() == before
