@@ -355,6 +355,12 @@ def _sort_pass_bytes(n: int, kb: int, pb: int, passes: int) -> int:
     return n * kb + passes * n * (kb + pb) + (passes - 1) * n * (kb + pb)
 
 
+def _sort_pass_bytes_pay(n: int, kb: int, pb: int, passes: int) -> int:
+    """Algorithmic bytes of `passes` scatter passes over n keys of kb bytes
+    carrying an existing pb-byte payload: every pass reads and writes both."""
+    return passes * 2 * n * (kb + pb)
+
+
 def _measure(step, reps: int, bytes_of, traffic_key: str | None = None) -> dict:
     """Wall time per call of step (no events), then the HIP-event time of
     every kernel launch over reps more calls; bytes_of(name, launches) gives
@@ -450,19 +456,22 @@ def workload_join(reps: int) -> dict:
         return joining.sort_merge_join(lk, lv, rk, rv, "inner")
 
     def bytes_of(name, launches):
+        # each side's two float32 payloads ride through its sort packed in
+        # one 8-byte record (read and written once by the packing)
+        if name == "pack_cols":
+            return (L + R) * (8 + 8)
         if name.startswith("sort_stats") or name == "sort_hist":
             return (L + R) * 8
         if name.startswith("onesweep"):
             if launches != pl + pr:
                 return None
-            return _sort_pass_bytes(L, 8, 4, pl) + _sort_pass_bytes(R, 8, 4, pr)
+            return _sort_pass_bytes_pay(L, 8, 8, pl) + _sort_pass_bytes_pay(R, 8, 8, pr)
         if name == "join_count":
             return (L + R) * 8
         if name == "join_fill":
-            # sorted keys and row indices of both sides, each side's first
-            # payload read once; keys, two payloads and both row columns
-            # (for the second payloads) written per pair
-            return (L + R) * (8 + 4 + 4) + m * (8 + 4 + 4 + 4 + 4)
+            # sorted keys and payload records of both sides read once; the key
+            # and four payload columns written per pair
+            return (L + R) * (8 + 8) + m * (8 + 4 * 4)
         if name == "gather":
             return launches * m * (4 + 4 + 4)
         return None
