@@ -595,6 +595,9 @@ def main() -> int:
     ap.add_argument("--batched-rows", type=int, default=400_000_000,
                     help="rows of the batched groupby (the 4096 frames' row counts are scaled "
                          "to it); 0 = the unscaled distribution (about 2e9 rows)")
+    ap.add_argument("--no-batched-unscaled", action="store_true",
+                    help="skip the second batched entry, the unscaled distribution (about "
+                         "1.9e9 rows in two pipelines of at most 2^30 rows)")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the sort, join and partition workloads")
@@ -730,6 +733,9 @@ def main() -> int:
                      "hash_partition": workload_partition(reps)}
         torch.cuda.empty_cache()
     batched_line = None if args.no_batched else batched_vs_plain(args.batched_rows or None)
+    batched_unscaled = None
+    if not (args.no_batched or args.no_batched_unscaled or args.batched_rows == 0):
+        batched_unscaled = batched_vs_plain(None)
     served_line = None if args.no_served else served_vs_plain()
 
     line = {
@@ -758,6 +764,7 @@ def main() -> int:
         "check": check,
         "workloads": workloads,
         "batched": batched_line,
+        "batched_unscaled": batched_unscaled,
         "served": served_line,
         "kernels": kernels,
         "cpu_baseline": cpu,
