@@ -880,11 +880,20 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   const uint64_t cap = std::min<uint64_t>(256 * per_cu, (uint64_t)OP_GROUP * OP_L1);
   const uint64_t want = (n + (uint64_t)OP_W * STEP - 1) / ((uint64_t)OP_W * STEP);
   uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, want));
-  a.rows_per_wave = ((n + (uint64_t)grid * OP_W - 1) / ((uint64_t)grid * OP_W) + STEP - 1) / STEP * STEP;
-  grid = (uint32_t)((n + a.rows_per_wave * OP_W - 1) / (a.rows_per_wave * OP_W));
   const bool vec = (uintptr_t)keys % 16 == 0 && (uintptr_t)vals % 16 == 0;
   const char* nm = TE == OP_TE ? "onepass_table" : "onepass_table_2k";
-  if (vec) {
+  // float32 sums alone: the count-free tables (group presence from the sums)
+  constexpr bool NC_OK = std::is_same<V, float>::value && !MM;
+  const bool nc = NC_OK && sc.out_count == nullptr && sc.out_mean == nullptr;
+  if (nc) {
+    if constexpr (NC_OK) {
+      if (vec) {
+        JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, true, TE, true><<<grid, OP_W * 64, 0, s>>>(a)));
+      } else {
+        JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, false, TE, true><<<grid, OP_W * 64, 0, s>>>(a)));
+      }
+    }
+  } else if (vec) {
     JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, true, TE><<<grid, OP_W * 64, 0, s>>>(a)));
   } else {
     JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, false, TE><<<grid, OP_W * 64, 0, s>>>(a)));

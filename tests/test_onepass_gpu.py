@@ -346,6 +346,54 @@ def test_onepass_min_max(vdt, R):
         assert np.array_equal(to_numpy(r1["v"]), ref.reduceat(sv, starts), equal_nan=True)
 
 
+@pytest.mark.parametrize("R", [8, 1000, 1500])
+def test_onepass_sum_only_presence_from_sums(R):
+    """float32 sums alone run the count-free tables (group presence from the
+    sum entries, which start at -0.0): groups whose rows are all -0.0, a
+    single -0.0 or +0.0 row, NaN and infinities are all present, keys with no
+    rows are absent, and every sum has the bits of the counted tables'
+    (groupby_stats) -- +0.0 for the -0.0 groups, as tables started at +0.0
+    give."""
+    rng = np.random.default_rng(R + 5)
+    keys = rng.integers(0, R, N).astype(np.int32)
+    keys[(keys % 7 == 5) & (keys > 2)] = 0             # keys with no rows inside the span
+    vals = (rng.standard_normal(N) * 10).astype(np.float32)
+    g = np.unique(keys)
+    special = {int(g[1]): "all-neg-zero", int(g[-2]): "mixed-zeros"}
+    vals[keys == g[1]] = -0.0
+    mz = np.flatnonzero(keys == g[-2])
+    vals[mz] = np.where(np.arange(mz.size) % 2 == 0, -0.0, 0.0).astype(np.float32)
+    # one row each of its own key: -0.0, +0.0 (R >= 1000: keys beyond the rest)
+    extra_k, extra_v = np.array([], np.int32), np.array([], np.float32)
+    if R >= 1000:
+        extra_k = np.array([R, R + 1], np.int32)
+        extra_v = np.array([-0.0, 0.0], np.float32)
+        special.update({R: "single-neg-zero", R + 1: "single-pos-zero"})
+    keys = np.concatenate([keys, extra_k])
+    vals = np.concatenate([vals, extra_v])
+    for with_specials in (False, True):
+        v = vals.copy()
+        if with_specials:
+            v[np.flatnonzero(keys == g[2])[:2]] = np.nan
+            v[np.flatnonzero(keys == g[3])[:1]] = np.inf
+            v[np.flatnonzero(keys == g[4])[:1]] = -np.inf
+        (uk, res), ran = _kernels_run(lambda: grouping.groupby_aggregate(
+            cuda(keys), {"v": cuda(v)}, {"v": "sum"}))
+        assert "onepass_table" in ran or "onepass_table_2k" in ran, ran
+        sk, sv, starts = _groups(keys, v)
+        assert np.array_equal(to_numpy(uk), sk[starts])
+        uk2, res2 = grouping.groupby_stats(cuda(keys), cuda(v))
+        assert torch.equal(uk, uk2)
+        got = to_numpy(res["v"])
+        assert np.array_equal(got.view(np.int32), to_numpy(res2["sum"]).view(np.int32))
+        for k, what in special.items():
+            i = int(np.searchsorted(sk[starts], k))
+            assert sk[starts][i] == k, what
+            assert got[i] == 0.0 and not np.signbit(got[i]), (what, got[i])
+        if not with_specials:
+            _check(keys, v, uk, res["v"])
+
+
 @pytest.mark.parametrize("R", [2, 1000, 1700])
 def test_onepass_float64_sums(R):
     """float64 values: sums compensated through ds_add_rtn_f64 (TwoSum of the
