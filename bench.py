@@ -164,18 +164,39 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - a) * 1e3)
     dt = (time.perf_counter() - t0) / reps
-    groups = int(plan.result().keys.shape[0])
+    res = plan.result()
+    groups = int(res.keys.shape[0])
     digit_bits, pipelines = plan.digit_bits, (1 if plan.parts is None else len(plan.parts))
-    cb = 4 if (plan.parts or [plan])[0].total_bits <= 32 else 8
+    parts = plan.parts or [plan]
+    cb = 4 if parts[0].total_bits <= 32 else 8
+    # rows and groups of the frames on the one-pass tables (routed plans) and
+    # of the sorted frames
+    gcount = torch.diff(res.offsets).cpu().numpy()
+    routed = [p for p in parts if p.route is not None]
+    n_narrow = sum(p.narrow_rows for p in routed)
+    f0 = np.cumsum([0] + [p.nframes for p in parts])
+    g_narrow = sum(int(gcount[f0[i] + p.narrow_ids].sum()) for i, p in enumerate(parts)
+                   if p.route is not None)
+    n_sorted, g_sorted = n - n_narrow, groups - g_narrow
+    routing = {"routed_pipelines": len(routed), "narrow_frames": sum(p.narrow_frames for p in routed),
+               "narrow_rows": n_narrow, "narrow_groups": g_narrow, "sorted_rows": n_sorted,
+               "sorted_groups": g_sorted}
+    del res
 
     def bytes_of(name, launches):
         # int32 keys and float32 values; composite keys of cb bytes
         if name == "batch_compose":
-            return n * (4 + cb)
+            # routed: keys and values of the sorted frames read, composites and
+            # values written compacted; otherwise keys read, composites written
+            return n_sorted * (4 + 4 + cb + 4) if routed else n * (4 + cb)
         if name.startswith("onesweep"):
-            return launches * 2 * n * (cb + 4) // pipelines
+            return launches * 2 * n_sorted * (cb + 4) // pipelines
         if name == "segreduce_tiles":
-            return n * (cb + 4) + groups * (cb + 4 + 8)
+            return n_sorted * (cb + 4) + g_sorted * (cb + 4 + 8)
+        if name == "bn_table":
+            return n_narrow * 8
+        if name == "bw_copy":
+            return g_sorted * 2 * (cb + 4 + 8)
         return None
 
     # per-kernel HIP-event times of eager launches of the same pipeline
@@ -189,6 +210,7 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
                             f"log-uniform [10, 1M]), float32 values; sum + count per frame",
                 "rows": n, "groups": groups, "ms_per_batch": round(dt * 1e3, 3),
                 "rows_per_s": round(n / dt, 1), "tuned": tuned, "pipelines": pipelines,
+                "routing": routing,
                 "eager_ms_per_batch": eager["ms_per_call"],
                 "alg_bytes_per_batch": eager["alg_bytes_per_call"],
                 "batch_hbm_frac": round(eager["alg_bytes_per_call"] / dt / 1e9 / PEAK_HBM_GBPS, 4),
@@ -214,7 +236,7 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
            "plain_rows_per_s": round(plain_rps, 1),
            "plain_sample": f"first {plain_frames} frames, {rows_plain} rows",
            "speedup_vs_plain": round(n / dt / plain_rps, 2),
-           "hip_graph": True, "tuned": tuned, "pipelines": pipelines,
+           "hip_graph": True, "tuned": tuned, "pipelines": pipelines, "routing": routing,
            "eager_ms_per_batch": eager["ms_per_call"],
            "alg_bytes_per_batch": eager["alg_bytes_per_call"],
            "batch_hbm_frac": round(eager["alg_bytes_per_call"] / dt / 1e9 / PEAK_HBM_GBPS, 4),

@@ -319,6 +319,66 @@ int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void
                         void* out_mean, uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes,
                         void* stream);
 
+/* Routed batched groupby: frames whose keys span fewer than
+ * JFX_BATCH_NARROW_SPAN values skip the composite sort. Their rows are
+ * reduced in one-pass LDS tables indexed by ordered(key) - ordered(frame
+ * minimum), chunk by chunk, and the chunk tables merged per frame in chunk
+ * order; only the other ("wide") frames' rows are composed, compacted, sorted
+ * and reduced as in jfx_batched_groupby. Groups of both kinds are then
+ * placed frame-major, as jfx_batched_groupby writes them (same outputs, same
+ * composite keys, *out_ngroups a device word). Each frame's groups equal
+ * SortBasedGroupBy.groupby_aggregate (parallel_algorithms.py:733-803) of that
+ * frame; no counterpart in the reference.
+ *
+ * The split is fixed when the plan is made. jfx_batched_key_ranges reads the
+ * keys once and writes the min and max ordered key image of each row chunk
+ * (chunk c: rows [chunk_row0[c], chunk_row0[c] + chunk_rows[c]); device
+ * arrays; out_minmax: device uint64[2 * nchunks]), from which the caller
+ * builds a jfx_batched_routing of device arrays. A key outside its narrow
+ * frame's measured range, or outside [key_min, key_min + 2^key_bits), raises
+ * the device error word. float64 values are not routed (JFX_ERR_UNSUPPORTED). */
+#define JFX_BATCH_NARROW_SPAN 1024
+#define JFX_BATCH_CHUNK_ROWS 65536
+int jfx_batched_key_ranges(int key_type, const void* keys, const uint64_t* chunk_row0,
+                           const uint32_t* chunk_rows, uint32_t nchunks, uint64_t* out_minmax,
+                           void* stream);
+
+typedef struct jfx_batched_routing {
+  uint32_t n_narrow;             /* narrow frames */
+  uint32_t n_chunks;             /* row chunks of the narrow frames, frame by frame */
+  uint32_t n_wide;               /* the other frames (empty frames among them) */
+  uint32_t reserved;
+  uint64_t n_wide_rows;          /* rows of the wide frames */
+  uint64_t part_entries;         /* sum over chunks of their frame's span + 1 */
+  uint64_t merged_entries;       /* sum over narrow frames of span + 1 */
+  const uint32_t* narrow_frame;  /* [n_narrow] frame ids, ascending */
+  const uint64_t* narrow_omin;   /* [n_narrow] ordered image of the frame's minimum key */
+  const uint32_t* narrow_span;   /* [n_narrow] its max - min (ordered images), below 1024 */
+  const uint32_t* narrow_chunk0; /* [n_narrow + 1] first chunk of each narrow frame */
+  const uint64_t* narrow_mbase;  /* [n_narrow] first entry of its merged table */
+  const uint64_t* chunk_row0;    /* [n_chunks] first row */
+  const uint32_t* chunk_rows;    /* [n_chunks] rows, at most JFX_BATCH_CHUNK_ROWS */
+  const uint32_t* chunk_narrow;  /* [n_chunks] narrow index of the chunk's frame */
+  const uint64_t* chunk_part;    /* [n_chunks] first entry of its table; a frame's chunk
+                                    tables follow each other, span + 1 entries apart */
+  const uint32_t* wide_frame;    /* [n_wide] frame ids, ascending */
+  const int64_t* wide_offsets;   /* [n_wide + 1] row offsets of the wide frames, compacted */
+  const uint32_t* frame_narrow;  /* [nframes] narrow index of each frame, or 0xFFFFFFFF */
+} jfx_batched_routing;
+
+/* outputs: the outputs the call will write, bit 0 sum, 1 count, 2 min, 3 max,
+ * 4 mean. */
+int jfx_batched_routed_workspace_size(uint32_t total_bits, int val_type, uint32_t nframes,
+                                      const jfx_batched_routing* route, int outputs,
+                                      uint64_t* out_bytes);
+int jfx_batched_groupby_routed(int key_type, const void* keys, int val_type, const void* vals,
+                               uint64_t n, const int64_t* frame_offsets, uint32_t nframes,
+                               uint64_t key_min, uint32_t key_bits, int digit_bits, int sort_shape,
+                               const jfx_batched_routing* route, void* out_keys, void* out_sum,
+                               int64_t* out_count, void* out_min, void* out_max, void* out_mean,
+                               uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes,
+                               void* stream);
+
 /* ---- indexed filter -> groupby (serving) ------------------------------- */
 /* Per group of a frame indexed by (key, value) -- values ascending inside each
  * group by their order-preserving image, NaN last (nan_start[g] = first NaN

@@ -44,6 +44,24 @@ class JoinSide(ctypes.Structure):
                 ("out_cols", ctypes.c_void_p * JOIN_MAXCOLS)]
 
 
+BATCH_NARROW_SPAN = 1024
+BATCH_CHUNK_ROWS = 65536
+
+
+class BatchedRouting(ctypes.Structure):
+    """jfx_batched_routing (include/jfx.h): device arrays of a routed plan."""
+    _fields_ = [("n_narrow", ctypes.c_uint32), ("n_chunks", ctypes.c_uint32),
+                ("n_wide", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("n_wide_rows", ctypes.c_uint64), ("part_entries", ctypes.c_uint64),
+                ("merged_entries", ctypes.c_uint64),
+                ("narrow_frame", ctypes.c_void_p), ("narrow_omin", ctypes.c_void_p),
+                ("narrow_span", ctypes.c_void_p), ("narrow_chunk0", ctypes.c_void_p),
+                ("narrow_mbase", ctypes.c_void_p), ("chunk_row0", ctypes.c_void_p),
+                ("chunk_rows", ctypes.c_void_p), ("chunk_narrow", ctypes.c_void_p),
+                ("chunk_part", ctypes.c_void_p), ("wide_frame", ctypes.c_void_p),
+                ("wide_offsets", ctypes.c_void_p), ("frame_narrow", ctypes.c_void_p)]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -88,6 +106,12 @@ _SIGNATURES = {
     "jfx_batched_groupby": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, ctypes.c_uint32, c_u64,
                             ctypes.c_uint32, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_vp, c_u64, c_vp],
+    "jfx_batched_key_ranges": [c_int, c_vp, c_vp, c_vp, ctypes.c_uint32, c_vp, c_vp],
+    "jfx_batched_routed_workspace_size": [ctypes.c_uint32, c_int, ctypes.c_uint32,
+                                          ctypes.POINTER(BatchedRouting), c_int, c_u64p],
+    "jfx_batched_groupby_routed": [c_int, c_vp, c_int, c_vp, c_u64, c_vp, ctypes.c_uint32, c_u64,
+                                   ctypes.c_uint32, c_int, c_int, ctypes.POINTER(BatchedRouting),
+                                   c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_vp],
     "jfx_served_join_workspace_size": [c_u64, ctypes.c_uint32, c_u64p],
     "jfx_served_join_count": [c_int, c_vp, c_vp, ctypes.c_uint32, c_vp, c_u64, c_u64p, c_vp, c_u64,
                               c_vp],

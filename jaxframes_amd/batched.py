@@ -9,7 +9,10 @@ The device work is jfx_batched_groupby (composite (frame, key) keys, one radix
 sort, one segmented reduction) and has no host synchronisation, so a plan can
 be captured once in a HIP graph and replayed; the digit width and the tile
 shape of the sort are picked per size class by a small autotuner that times
-the candidates on the first call.
+the candidates on the first call. When enough rows belong to frames whose keys
+span fewer than 1024 values (measured once when the plan is made), the plan
+runs jfx_batched_groupby_routed instead: those frames are reduced in one-pass
+LDS tables and only the other frames' rows are sorted.
 """
 
 from __future__ import annotations
@@ -20,6 +23,7 @@ import os
 import threading
 import time
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -110,6 +114,23 @@ class BatchResult:
 # rows one device pipeline takes (its row indices and lookback counts are 30-bit)
 MAX_ROWS = (1 << 30) - 1
 
+# Narrow frames take the one-pass tables when they hold at least this share of
+# the rows: routing saves their sort passes (about 80 bytes of traffic per row
+# at 4 passes) but copies the wide rows' values once more and places the wide
+# groups after the reduction (about 8 bytes per wide row and 32 per wide group).
+ROUTE_MIN_SHARE = 0.15
+_OUT_BITS = {"sum": 1, "count": 2, "min": 4, "max": 8, "mean": 16}
+
+
+def _key_max(kbytes: int, signed: bool) -> int:
+    return (1 << (8 * kbytes - 1)) - 1 if signed else (1 << (8 * kbytes)) - 1
+
+
+def _ordered_image(x: int, kbytes: int, signed: bool) -> int:
+    """Order-preserving unsigned image of an integer key (the kernels' ordered())."""
+    w = 8 * kbytes
+    return ((x + (1 << (w - 1))) if signed else x) & ((1 << w) - 1)
+
 
 class BatchedGroupBy:
     """A reusable plan for one table shape: frame offsets, key range, value
@@ -124,7 +145,7 @@ class BatchedGroupBy:
 
     def __init__(self, keys, values, frame_offsets, *, key_range=None, aggs=("sum",),
                  digit_bits: int = 0, sort_shape: int = 0, tuner: Autotuner | None = TUNER,
-                 max_rows: int = MAX_ROWS):
+                 max_rows: int = MAX_ROWS, route: bool | None = None):
         _lib.require_gpu()
         for a in aggs:
             if a not in AGGS:
@@ -138,16 +159,24 @@ class BatchedGroupBy:
         if self.vals.shape[0] != self.n or int(self.offsets[-1]) != self.n or int(self.offsets[0]) != 0:
             raise ValueError("frame offsets must run from 0 to the number of rows")
         self.kc = key_code(self.keys)
+        kb = self.keys.element_size()
+        signed = self.keys.dtype in (torch.int32, torch.int64)
+        # per-chunk key ranges (one read of the keys): the key range when none
+        # is given, and the routing plan below
+        ranges = None
+        if self.n > 0 and (key_range is None or (route is not False and self.n <= max_rows)):
+            ranges = self._chunk_ranges()
         if key_range is None:
             if self.n == 0:
                 key_range = (0, 0)
             else:
-                mn, mx = torch.aminmax(self.keys)
-                key_range = (int(mn), int(mx))
+                off = (1 << (8 * kb - 1)) if signed else 0
+                key_range = (int(ranges[3][:, 0].min()) - off, int(ranges[3][:, 1].max()) - off)
         self.parts = None
         self.aggs = tuple(aggs)
+        self.route = None
         if self.n > max_rows:
-            self._split(key_range, digit_bits, sort_shape, tuner, max_rows)
+            self._split(key_range, digit_bits, sort_shape, tuner, max_rows, route)
             return
         self.kmin, kmax = int(key_range[0]), int(key_range[1])
         span = kmax - self.kmin
@@ -163,9 +192,16 @@ class BatchedGroupBy:
         self.out_keys = torch.empty(cap, dtype=self.comp_dtype, device=self.dev)
         self.outs = {a: _alloc_out(a, self.vals, cap, self.dev) for a in self.aggs}
         self.ngroups = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        if route is not False and self.vals.dtype != torch.float64 and ranges is not None:
+            self._plan_route(ranges, force=route is True)
         nb = ctypes.c_uint64(0)
-        _lib.call("jfx_batched_groupby_workspace_size", self.total_bits, self.vcode, self.n,
-                  ctypes.byref(nb))
+        if self.route is not None:
+            outs = sum(_OUT_BITS[a] for a in self.aggs)
+            _lib.call("jfx_batched_routed_workspace_size", self.total_bits, self.vcode,
+                      self.nframes, ctypes.byref(self.route), outs, ctypes.byref(nb))
+        else:
+            _lib.call("jfx_batched_groupby_workspace_size", self.total_bits, self.vcode, self.n,
+                      ctypes.byref(nb))
         self.ws = torch.empty(nb.value, dtype=torch.uint8, device=self.dev)
         self.graph = None
         if digit_bits:
@@ -175,7 +211,102 @@ class BatchedGroupBy:
         else:
             self.digit_bits, self.sort_shape = 8, sort_shape
 
-    def _split(self, key_range, digit_bits, sort_shape, tuner, max_rows):
+    def _chunk_ranges(self):
+        """Row chunks of every frame (at most BATCH_CHUNK_ROWS rows each) and
+        the min / max ordered key image of each: (frame of each chunk, first
+        rows, row counts, uint64[chunks, 2])."""
+        offs = self.offsets.cpu().numpy().astype(np.int64)
+        rows = np.diff(offs)
+        CH = _lib.BATCH_CHUNK_ROWS
+        nch = (rows + CH - 1) // CH
+        nc = int(nch.sum())
+        fr = np.repeat(np.arange(self.nframes, dtype=np.int64), nch)
+        first = np.cumsum(nch) - nch
+        kin = np.arange(nc, dtype=np.int64) - np.repeat(first, nch)
+        c_row0 = offs[fr] + kin * CH
+        c_rows = np.minimum(CH, offs[fr + 1] - c_row0)
+        d_row0 = torch.from_numpy(c_row0).to(self.dev)
+        d_rows = torch.from_numpy(c_rows.astype(np.int32)).to(self.dev)
+        mm = torch.empty(2 * nc, dtype=torch.int64, device=self.dev)
+        s = torch.cuda.current_stream(self.dev)
+        _lib.call("jfx_batched_key_ranges", self.kc, _lib.ptr(self.keys), _lib.ptr(d_row0),
+                  _lib.ptr(d_rows), nc, _lib.ptr(mm), ctypes.c_void_p(s.cuda_stream))
+        return fr, c_row0, c_rows, mm.cpu().numpy().view(np.uint64).reshape(-1, 2)
+
+    def _plan_route(self, ranges, force: bool):
+        """From the per-chunk key ranges: when frames spanning fewer than
+        1024 values hold at least ROUTE_MIN_SHARE of the rows (or force),
+        build the jfx_batched_routing arrays: those frames' row chunks and
+        table offsets, the wide frames' compacted row offsets and each
+        frame's narrow index."""
+        fr, c_row0, c_rows, mmh = ranges
+        offs = self.offsets.cpu().numpy().astype(np.int64)
+        rows = np.diff(offs)
+        F = self.nframes
+        CH = _lib.BATCH_CHUNK_ROWS
+        nch = (rows + CH - 1) // CH
+        fmin = np.full(F, np.iinfo(np.uint64).max, dtype=np.uint64)
+        fmax = np.zeros(F, dtype=np.uint64)
+        np.minimum.at(fmin, fr, mmh[:, 0])
+        np.maximum.at(fmax, fr, mmh[:, 1])
+        kb = self.keys.element_size()
+        signed = self.keys.dtype in (torch.int32, torch.int64)
+        lo = np.uint64(_ordered_image(self.kmin, kb, signed))
+        hi = np.uint64(_ordered_image(self.kmin + (1 << self.key_bits) - 1, kb, signed)
+                       if self.kmin + (1 << self.key_bits) - 1 <= _key_max(kb, signed)
+                       else _ordered_image(_key_max(kb, signed), kb, signed))
+        has = rows > 0
+        span = np.where(has, fmax - np.where(has, fmin, 0), 0)
+        narrow = has & (span < _lib.BATCH_NARROW_SPAN) & (fmin >= lo) & (fmax <= hi)
+        nrows = int(rows[narrow].sum())
+        if nrows == 0 or (not force and nrows < ROUTE_MIN_SHARE * self.n):
+            return
+        nidx = np.flatnonzero(narrow)
+        widx = np.flatnonzero(~narrow)
+        nspan = span[nidx].astype(np.uint64)
+        csel = narrow[fr]
+        chunk_narrow = np.searchsorted(nidx, fr[csel])
+        cent = nspan[chunk_narrow] + 1
+        chunk_part = np.cumsum(cent) - cent
+        mbase = np.cumsum(nspan + 1) - (nspan + 1)
+        chunk0 = np.zeros(nidx.size + 1, dtype=np.int64)
+        chunk0[1:] = np.cumsum(nch[nidx])
+        woffs = np.zeros(widx.size + 1, dtype=np.int64)
+        woffs[1:] = np.cumsum(rows[widx])
+        frame_narrow = np.full(F, 0xFFFFFFFF, dtype=np.uint32)
+        frame_narrow[nidx] = np.arange(nidx.size, dtype=np.uint32)
+        arrays = {
+            "narrow_frame": nidx.astype(np.uint32), "narrow_omin": fmin[nidx],
+            "narrow_span": nspan.astype(np.uint32), "narrow_chunk0": chunk0.astype(np.uint32),
+            "narrow_mbase": mbase.astype(np.uint64), "chunk_row0": c_row0[csel].astype(np.uint64),
+            "chunk_rows": c_rows[csel].astype(np.uint32),
+            "chunk_narrow": chunk_narrow.astype(np.uint32),
+            "chunk_part": chunk_part.astype(np.uint64), "wide_frame": widx.astype(np.uint32),
+            "wide_offsets": woffs, "frame_narrow": frame_narrow,
+        }
+        # one device buffer, every array at an 8-byte boundary
+        spans, pos = {}, 0
+        for name, a in arrays.items():
+            spans[name] = (pos, a.nbytes)
+            pos += (a.nbytes + 7) // 8 * 8
+        host = np.zeros(max(pos, 8), dtype=np.uint8)
+        for name, a in arrays.items():
+            o, nb = spans[name]
+            host[o:o + nb] = np.ascontiguousarray(a).view(np.uint8)
+        self._route_buf = torch.from_numpy(host).to(self.dev)
+        base = self._route_buf.data_ptr()
+        r = _lib.BatchedRouting()
+        r.n_narrow, r.n_chunks, r.n_wide = nidx.size, int(csel.sum()), widx.size
+        r.n_wide_rows = int(woffs[-1])
+        r.part_entries = int(cent.sum())
+        r.merged_entries = int((nspan + 1).sum())
+        for name in arrays:
+            setattr(r, name, base + spans[name][0])
+        self.route = r
+        self.narrow_frames, self.narrow_rows = int(nidx.size), nrows
+        self.narrow_ids = nidx
+
+    def _split(self, key_range, digit_bits, sort_shape, tuner, max_rows, route=None):
         offs = self.offsets.cpu().tolist()
         cuts, start = [0], 0
         for f in range(self.nframes):
@@ -192,13 +323,22 @@ class BatchedGroupBy:
             self.parts.append(BatchedGroupBy(self.keys[r0:r1], self.vals[r0:r1], sub,
                                              key_range=key_range, aggs=self.aggs,
                                              digit_bits=digit_bits, sort_shape=sort_shape,
-                                             tuner=tuner, max_rows=max_rows))
+                                             tuner=tuner, max_rows=max_rows, route=route))
         self.digit_bits = self.parts[0].digit_bits
         self.sort_shape = self.parts[0].sort_shape
 
     def _launch(self, digit_bits: int | None = None, sort_shape: int | None = None, stream=None):
         s = stream if stream is not None else torch.cuda.current_stream(self.dev)
         ptr = lambda a: _lib.ptr(self.outs.get(a))
+        if self.route is not None:
+            _lib.call("jfx_batched_groupby_routed", self.kc, _lib.ptr(self.keys), self.vcode,
+                      _lib.ptr(self.vals), self.n, _lib.ptr(self.offsets), self.nframes,
+                      self.kmin & ((1 << 64) - 1), self.key_bits, digit_bits or self.digit_bits,
+                      self.sort_shape if sort_shape is None else sort_shape,
+                      ctypes.byref(self.route), _lib.ptr(self.out_keys), ptr("sum"), ptr("count"),
+                      ptr("min"), ptr("max"), ptr("mean"), _lib.ptr(self.ngroups),
+                      _lib.ptr(self.ws), self.ws.numel(), ctypes.c_void_p(s.cuda_stream))
+            return
         _lib.call("jfx_batched_groupby", self.kc, _lib.ptr(self.keys), self.vcode, _lib.ptr(self.vals),
                   self.n, _lib.ptr(self.offsets), self.nframes, self.kmin & ((1 << 64) - 1),
                   self.key_bits, digit_bits or self.digit_bits,

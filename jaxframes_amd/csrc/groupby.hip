@@ -1332,3 +1332,258 @@ int jfx_batched_groupby(int key_type, const void* keys, int val_type, const void
 }
 
 }  // extern "C"
+
+namespace jfx {
+namespace {
+
+// One narrow table entry of a value type, with or without min / max codes.
+size_t bn_part_bytes(int vt, bool mm) {
+  switch (vt) {
+    case JFX_F32: return mm ? sizeof(OpPart<float, true>) : sizeof(OpPart<float, false>);
+    case JFX_I32: return mm ? sizeof(OpPart<int32_t, true>) : sizeof(OpPart<int32_t, false>);
+    case JFX_U32: return mm ? sizeof(OpPart<uint32_t, true>) : sizeof(OpPart<uint32_t, false>);
+    case JFX_I64: return mm ? sizeof(OpPart<int64_t, true>) : sizeof(OpPart<int64_t, false>);
+    case JFX_U64: return mm ? sizeof(OpPart<uint64_t, true>) : sizeof(OpPart<uint64_t, false>);
+  }
+  return 0;
+}
+
+// Workspace of the routed pipeline (the same carving sizes it and uses it).
+struct RoutedWs {
+  void *comp = nullptr, *scomp = nullptr, *cvals = nullptr, *svals = nullptr;
+  SegWs w{};
+  void* sort_ws = nullptr;
+  uint64_t sws = 0;
+  void *tkeys = nullptr, *tsum = nullptr, *tmin = nullptr, *tmax = nullptr, *tmean = nullptr;
+  int64_t* tcount = nullptr;
+  void *part = nullptr, *mtab = nullptr;
+  uint32_t *gn = nullptr, *base = nullptr, *wstart = nullptr;
+};
+
+void carve_routed(Carver& c, uint32_t total, int vt, uint32_t nframes, const jfx_batched_routing& r,
+                  int outputs, RoutedWs& o) {
+  const int cb = total <= 32 ? 4 : 8, vb = key_bytes(vt);
+  const int mtb = (vt == JFX_I64 || vt == JFX_U64 || vt == JFX_F64) ? 8 : 4;
+  const uint64_t nw = r.n_wide_rows;
+  if (nw) {
+    o.comp = c.take<char>(nw * cb);
+    o.scomp = c.take<char>(nw * cb);
+    o.cvals = c.take<char>(nw * vb);
+    o.svals = c.take<char>(nw * vb);
+    o.w = carve_seg(c, vb, nw);
+    o.sws = sort_ws_bytes(cb == 4 ? JFX_U32 : JFX_U64, vb, nw, true);
+    o.sort_ws = c.take<char>(o.sws);
+    o.tkeys = c.take<char>(nw * cb);
+    if (outputs & 1) o.tsum = c.take<char>(nw * vb);
+    if (outputs & 2) o.tcount = c.take<int64_t>(nw);
+    if (outputs & 4) o.tmin = c.take<char>(nw * vb);
+    if (outputs & 8) o.tmax = c.take<char>(nw * vb);
+    if (outputs & 16) o.tmean = c.take<char>(nw * mtb);
+  }
+  const size_t pb = bn_part_bytes(vt, (outputs & 12) != 0);
+  o.part = c.take<char>(r.part_entries * pb);
+  o.mtab = c.take<char>(r.merged_entries * pb);
+  o.gn = c.take<uint32_t>((uint64_t)r.n_narrow + 1u);
+  o.base = c.take<uint32_t>(nframes);
+  o.wstart = c.take<uint32_t>(nframes);
+}
+
+template <typename K, typename V, typename MT, bool MM, typename CK>
+int routed_run(int oc, int vt, const void* keys, const void* vals, const int64_t* offsets,
+               uint32_t nframes, uint64_t omin, uint32_t key_bits, uint32_t total, int digit_bits,
+               int sort_shape, const jfx_batched_routing& r, const RoutedWs& ws, void* out_keys,
+               const GroupOut<V, MT>& out, uint32_t* out_ngroups, uint32_t* err, hipStream_t s) {
+  using P = OpPart<V, MM>;
+  using VU = typename std::conditional<sizeof(V) == 4, uint32_t, uint64_t>::type;
+  using MU = typename std::conditional<sizeof(MT) == 4, uint32_t, uint64_t>::type;
+  if (r.n_narrow) {
+    JFX_LAUNCH("bn_table", s, (k_bn_table<K, V, MM><<<r.n_chunks, BR_W * 64, 0, s>>>(
+                                  static_cast<const K*>(keys), static_cast<const V*>(vals), oc, r,
+                                  static_cast<P*>(ws.part), err)));
+    JFX_HIP(hipGetLastError());
+    JFX_LAUNCH("bn_merge", s, (k_bn_merge<V, MM><<<r.n_narrow, 1024, 0, s>>>(
+                                  r, static_cast<const P*>(ws.part), static_cast<P*>(ws.mtab), ws.gn)));
+    JFX_HIP(hipGetLastError());
+  }
+  const uint64_t nw = r.n_wide_rows;
+  if (nw) {
+    const uint32_t blocks = (uint32_t)((nw + BC_ROWS - 1) / BC_ROWS);
+    JFX_LAUNCH("batch_compose", s, (k_batch_compose_wide<K, CK, VU><<<blocks, BC_BLOCK, 0, s>>>(
+                                       static_cast<const K*>(keys), static_cast<const VU*>(vals), oc,
+                                       omin, key_bits, offsets, r, static_cast<CK*>(ws.comp),
+                                       static_cast<VU*>(ws.cvals), err)));
+    JFX_HIP(hipGetLastError());
+    const int ckt = sizeof(CK) == 4 ? JFX_U32 : JFX_U64;
+    SortCtx ctx;
+    int rc = sort_prepare_known(ctx, ckt, ws.comp, nw, (int)sizeof(V), digit_bits, total, ws.sort_ws,
+                                ws.sws, s, sort_shape, true);
+    if (rc != JFX_OK) return rc;
+    rc = sort_execute(ctx, ws.comp, ws.scomp, 1, ws.cvals, ws.svals, 0, 0);
+    if (rc != JFX_OK) return rc;
+    SegCall c{ws.scomp, ORD_UNSIGNED, ws.svals, nullptr, nw, ws.tkeys,
+              out.sum ? ws.tsum : nullptr, out.count ? ws.tcount : nullptr,
+              out.mn ? ws.tmin : nullptr, out.mx ? ws.tmax : nullptr,
+              out.mean ? ws.tmean : nullptr, false, 0u};
+    rc = segreduce_dispatch(ws.w, ckt, vt, c, s);
+    if (rc != JFX_OK) return rc;
+  }
+  JFX_LAUNCH("batch_counts", s, (k_batch_counts<CK><<<(nframes + 255) / 256, 256, 0, s>>>(
+                                    r, nframes, static_cast<const CK*>(ws.tkeys),
+                                    nw ? ws.w.ngroups : nullptr, key_bits, ws.gn, ws.base,
+                                    ws.wstart)));
+  JFX_HIP(hipGetLastError());
+  JFX_LAUNCH("batch_bases", s, (k_batch_bases<<<1, 1024, 0, s>>>(nframes, ws.base, out_ngroups)));
+  JFX_HIP(hipGetLastError());
+  if (r.n_narrow) {
+    JFX_LAUNCH("bn_write", s, (k_bn_write<V, MT, MM, CK><<<r.n_narrow, 1024, 0, s>>>(
+                                  r, static_cast<const P*>(ws.mtab), ws.base, omin, key_bits,
+                                  static_cast<CK*>(out_keys), out)));
+    JFX_HIP(hipGetLastError());
+  }
+  if (nw) {
+    JFX_LAUNCH("bw_copy", s, (k_bw_copy<CK, VU, MU><<<2048, 256, 0, s>>>(
+                                 ws.w.ngroups, key_bits, ws.base, ws.wstart,
+                                 static_cast<const CK*>(ws.tkeys), static_cast<const VU*>(ws.tsum),
+                                 ws.tcount, static_cast<const VU*>(ws.tmin),
+                                 static_cast<const VU*>(ws.tmax), static_cast<const MU*>(ws.tmean),
+                                 static_cast<CK*>(out_keys), reinterpret_cast<VU*>(out.sum), out.count,
+                                 reinterpret_cast<VU*>(out.mn), reinterpret_cast<VU*>(out.mx),
+                                 reinterpret_cast<MU*>(out.mean))));
+    JFX_HIP(hipGetLastError());
+  }
+  return JFX_OK;
+}
+
+template <typename K, typename V, typename MT>
+int routed_vt(int oc, int vt, const void* keys, const void* vals, const int64_t* offsets,
+              uint32_t nframes, uint64_t omin, uint32_t key_bits, uint32_t total, int digit_bits,
+              int sort_shape, const jfx_batched_routing& r, const RoutedWs& ws, void* out_keys,
+              void* out_sum, int64_t* out_count, void* out_min, void* out_max, void* out_mean,
+              uint32_t* out_ngroups, uint32_t* err, hipStream_t s) {
+  const GroupOut<V, MT> o{static_cast<V*>(out_sum), out_count, static_cast<V*>(out_min),
+                          static_cast<V*>(out_max), static_cast<MT*>(out_mean)};
+  const bool mm = out_min || out_max;
+  const bool c4 = total <= 32;
+#define JFX_ROUTED(MMV, CKT)                                                                     \
+  routed_run<K, V, MT, MMV, CKT>(oc, vt, keys, vals, offsets, nframes, omin, key_bits, total,    \
+                                 digit_bits, sort_shape, r, ws, out_keys, o, out_ngroups, err, s)
+  if (mm) return c4 ? JFX_ROUTED(true, uint32_t) : JFX_ROUTED(true, uint64_t);
+  return c4 ? JFX_ROUTED(false, uint32_t) : JFX_ROUTED(false, uint64_t);
+#undef JFX_ROUTED
+}
+
+template <typename K>
+int routed_key(int oc, int vt, const void* keys, const void* vals, const int64_t* offsets,
+               uint32_t nframes, uint64_t omin, uint32_t key_bits, uint32_t total, int digit_bits,
+               int sort_shape, const jfx_batched_routing& r, const RoutedWs& ws, void* out_keys,
+               void* out_sum, int64_t* out_count, void* out_min, void* out_max, void* out_mean,
+               uint32_t* out_ngroups, uint32_t* err, hipStream_t s) {
+#define JFX_ROUTED_VT(VT, MTT)                                                                   \
+  routed_vt<K, VT, MTT>(oc, vt, keys, vals, offsets, nframes, omin, key_bits, total, digit_bits, \
+                        sort_shape, r, ws, out_keys, out_sum, out_count, out_min, out_max,       \
+                        out_mean, out_ngroups, err, s)
+  switch (vt) {
+    case JFX_F32: return JFX_ROUTED_VT(float, float);
+    case JFX_I32: return JFX_ROUTED_VT(int32_t, float);
+    case JFX_U32: return JFX_ROUTED_VT(uint32_t, float);
+    case JFX_I64: return JFX_ROUTED_VT(int64_t, double);
+    case JFX_U64: return JFX_ROUTED_VT(uint64_t, double);
+  }
+#undef JFX_ROUTED_VT
+  return set_error(JFX_ERR_UNSUPPORTED, "the routed batched groupby takes no float64 values");
+}
+
+}  // namespace
+}  // namespace jfx
+
+extern "C" {
+
+int jfx_batched_key_ranges(int key_type, const void* keys, const uint64_t* chunk_row0,
+                           const uint32_t* chunk_rows, uint32_t nchunks, uint64_t* out_minmax,
+                           void* stream) {
+  JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64 && jfx::int_key(key_type), JFX_ERR_INVALID,
+            "batched keys must be integers");
+  hipStream_t s = jfx::as_stream(stream);
+  if (nchunks == 0) return JFX_OK;
+  JFX_CHECK(keys && chunk_row0 && chunk_rows && out_minmax, JFX_ERR_INVALID, "null argument");
+  const int oc = jfx::order_code(key_type);
+  if (jfx::key_bytes(key_type) == 4) {
+    JFX_LAUNCH("batch_ranges", s, (jfx::k_batch_ranges<uint32_t><<<nchunks, 256, 0, s>>>(
+                                      static_cast<const uint32_t*>(keys), oc, chunk_row0, chunk_rows,
+                                      out_minmax)));
+  } else {
+    JFX_LAUNCH("batch_ranges", s, (jfx::k_batch_ranges<uint64_t><<<nchunks, 256, 0, s>>>(
+                                      static_cast<const uint64_t*>(keys), oc, chunk_row0, chunk_rows,
+                                      out_minmax)));
+  }
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
+
+int jfx_batched_routed_workspace_size(uint32_t total_bits, int val_type, uint32_t nframes,
+                                      const jfx_batched_routing* route, int outputs,
+                                      uint64_t* out_bytes) {
+  JFX_CHECK(out_bytes != nullptr && route != nullptr, JFX_ERR_INVALID, "null argument");
+  JFX_CHECK(jfx::valid_vt(val_type) && val_type != JFX_F64, JFX_ERR_UNSUPPORTED,
+            "the routed batched groupby takes integer or float32 values");
+  JFX_CHECK(total_bits >= 1 && total_bits <= 64, JFX_ERR_INVALID, "total_bits must be 1..64");
+  jfx::Carver c(nullptr, 0);
+  jfx::RoutedWs o;
+  jfx::carve_routed(c, total_bits, val_type, nframes, *route, outputs, o);
+  *out_bytes = c.used();
+  return JFX_OK;
+}
+
+int jfx_batched_groupby_routed(int key_type, const void* keys, int val_type, const void* vals,
+                               uint64_t n, const int64_t* frame_offsets, uint32_t nframes,
+                               uint64_t key_min, uint32_t key_bits, int digit_bits, int sort_shape,
+                               const jfx_batched_routing* route, void* out_keys, void* out_sum,
+                               int64_t* out_count, void* out_min, void* out_max, void* out_mean,
+                               uint32_t* out_ngroups, void* workspace, uint64_t ws_bytes,
+                               void* stream) {
+  JFX_CHECK(key_type >= JFX_U32 && key_type <= JFX_F64 && jfx::int_key(key_type), JFX_ERR_INVALID,
+            "batched keys must be integers");
+  JFX_CHECK(jfx::valid_vt(val_type) && val_type != JFX_F64, JFX_ERR_UNSUPPORTED,
+            "the routed batched groupby takes integer or float32 values");
+  JFX_CHECK(route != nullptr, JFX_ERR_INVALID, "route is null");
+  JFX_CHECK(n >= 1 && n < (1ull << 30), JFX_ERR_UNSUPPORTED, "n must be 1 .. 2^30 - 1 rows");
+  JFX_CHECK(nframes >= 1, JFX_ERR_INVALID, "at least one frame");
+  const jfx_batched_routing& r = *route;
+  JFX_CHECK((uint64_t)r.n_narrow + r.n_wide == nframes, JFX_ERR_INVALID,
+            "every frame must be narrow or wide");
+  JFX_CHECK(r.n_wide_rows <= n, JFX_ERR_INVALID, "wide rows exceed the table");
+  JFX_CHECK(r.n_narrow == 0 || r.n_chunks >= r.n_narrow, JFX_ERR_INVALID,
+            "every narrow frame needs a chunk");
+  uint32_t fbits = 0;
+  while ((1ull << fbits) < nframes) ++fbits;
+  const uint32_t total = fbits + key_bits;
+  JFX_CHECK(key_bits >= 1 && total <= 64, JFX_ERR_UNSUPPORTED,
+            "frame and key bits exceed 64 (narrow the key range)");
+  JFX_CHECK(out_keys && out_ngroups && frame_offsets && keys && vals && workspace, JFX_ERR_INVALID,
+            "null input, output or workspace");
+  JFX_CHECK(digit_bits == 0 || digit_bits == 8 || digit_bits == 11, JFX_ERR_INVALID,
+            "digit_bits must be 0, 8 or 11");
+  JFX_CHECK(sort_shape >= 0 && sort_shape < jfx::SORT_SHAPES, JFX_ERR_INVALID,
+            "sort_shape must be 0 to 3");
+  const int outputs = (out_sum ? 1 : 0) | (out_count ? 2 : 0) | (out_min ? 4 : 0) |
+                      (out_max ? 8 : 0) | (out_mean ? 16 : 0);
+  jfx::Carver cv(workspace, ws_bytes);
+  jfx::RoutedWs ws;
+  jfx::carve_routed(cv, total, val_type, nframes, r, outputs, ws);
+  JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "routed batched groupby workspace too small");
+  uint32_t* err = jfx::device_error_word();
+  JFX_CHECK(err != nullptr, JFX_ERR_HIP, "cannot resolve the device error word");
+  hipStream_t s = jfx::as_stream(stream);
+  const int oc = jfx::order_code(key_type);
+  uint64_t omin = jfx::key_bytes(key_type) == 4 ? (key_min & 0xFFFFFFFFull) : key_min;
+  if (oc == jfx::ORD_SIGNED) omin ^= jfx::key_bytes(key_type) == 4 ? 0x80000000ull : 0x8000000000000000ull;
+  if (jfx::key_bytes(key_type) == 4)
+    return jfx::routed_key<uint32_t>(oc, val_type, keys, vals, frame_offsets, nframes, omin, key_bits,
+                                     total, digit_bits, sort_shape, r, ws, out_keys, out_sum,
+                                     out_count, out_min, out_max, out_mean, out_ngroups, err, s);
+  return jfx::routed_key<uint64_t>(oc, val_type, keys, vals, frame_offsets, nframes, omin, key_bits,
+                                   total, digit_bits, sort_shape, r, ws, out_keys, out_sum, out_count,
+                                   out_min, out_max, out_mean, out_ngroups, err, s);
+}
+
+}  // extern "C"

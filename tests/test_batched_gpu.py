@@ -157,3 +157,117 @@ def test_autotuner_picks_and_caches(tmp_path):
     again = batched.Autotuner(str(path))
     assert again.lookup(plan.n, plan.total_bits) == (plan.digit_bits, plan.sort_shape)
     _check(plan.run(), keys, vals, offs, ("sum",))
+
+
+def _mixed_frames(seed, nframes, max_rows, dtype=np.int32, neg=False, wide_share=0.5):
+    """Frames whose keys span below 1024 values (narrow) mixed with wider ones,
+    plus empty and single-row frames."""
+    rng = np.random.default_rng(seed)
+    rows = np.floor(np.exp(rng.uniform(0, np.log(max_rows), nframes))).astype(np.int64)
+    rows[::13] = 0
+    rows[1::17] = 1
+    wide = rng.random(nframes) < wide_share
+    cards = np.where(wide, rng.integers(2000, 200_000, nframes), rng.integers(1, 1024, nframes))
+    base = rng.integers(-5000 if neg else 0, 5000, nframes)
+    keys = np.concatenate([base[f] + rng.integers(0, cards[f], rows[f]) for f in range(nframes)])
+    keys = keys.astype(dtype)
+    vals = rng.standard_normal(keys.shape[0]).astype(np.float32)
+    offs = np.zeros(nframes + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(rows)
+    return keys, vals, offs
+
+
+@pytest.mark.parametrize("dtype,neg", [(np.int32, False), (np.int32, True), (np.int64, True),
+                                       (np.uint32, False)])
+@pytest.mark.parametrize("wide_share", [0.0, 0.5, 1.0])
+def test_routed_matches_per_frame_oracle(dtype, neg, wide_share):
+    """Narrow frames on the one-pass tables, wide frames sorted: every frame's
+    groups equal the oracle's (keys, counts, min, max exact; sums and means
+    to the tolerance), and the routed and unrouted plans agree."""
+    keys, vals, offs = _mixed_frames(17 + int(wide_share * 10), 300, 60_000, dtype, neg,
+                                     wide_share)
+    aggs = ("sum", "count", "min", "max", "mean")
+    plan = batched.BatchedGroupBy(cuda(keys), cuda(vals), offs, aggs=aggs, digit_bits=8,
+                                  route=True)
+    rows = np.diff(offs)
+    if wide_share < 1.0:
+        assert plan.route is not None and plan.narrow_frames > 0
+    res = plan.run()
+    _check(res, keys, vals, offs, aggs)
+    plain = batched.BatchedGroupBy(cuda(keys), cuda(vals), offs, aggs=aggs, digit_bits=8,
+                                   route=False).run()
+    assert plain is not None
+    assert torch.equal(res.offsets, plain.offsets)
+    assert torch.equal(res.keys, plain.keys)
+    for a in ("count", "min", "max"):
+        assert torch.equal(res.values[a], plain.values[a]), a
+    assert _lib.device_error() == 0
+    assert rows.sum() == keys.shape[0]
+
+
+@pytest.mark.parametrize("vdt", [np.int32, np.int64, np.uint32, np.uint64])
+def test_routed_integer_values(vdt):
+    """Integer values: wrapping sums in the value type, exact on both paths."""
+    keys, _, offs = _mixed_frames(5, 120, 30_000, np.int32, True, 0.4)
+    rng = np.random.default_rng(3)
+    info = np.iinfo(vdt)
+    vals = rng.integers(info.min, info.max, keys.shape[0], dtype=vdt, endpoint=True)
+    aggs = ("sum", "count", "min", "max")
+    r1 = batched.BatchedGroupBy(cuda(keys), cuda(vals), offs, aggs=aggs, digit_bits=8,
+                                route=True).run()
+    r2 = batched.BatchedGroupBy(cuda(keys), cuda(vals), offs, aggs=aggs, digit_bits=8,
+                                route=False).run()
+    assert torch.equal(r1.offsets, r2.offsets) and torch.equal(r1.keys, r2.keys)
+    for a in aggs:
+        assert torch.equal(r1.values[a], r2.values[a]), a
+    got_off = r1.offsets.cpu().numpy()
+    gsum = r1.values["sum"].cpu().numpy()
+    for f in range(offs.shape[0] - 1):
+        a, b = offs[f], offs[f + 1]
+        if a == b:
+            continue
+        ek, idx = np.unique(keys[a:b], return_inverse=True)
+        es = np.zeros(ek.shape[0], dtype=vdt)
+        with np.errstate(over="ignore"):
+            np.add.at(es, idx, vals[a:b])
+        assert np.array_equal(gsum[got_off[f]:got_off[f + 1]], es), f
+
+
+def test_routed_graph_replays_and_large_frames():
+    """Frames above one 65536-row chunk (several chunk tables merged in
+    order), few distinct keys (lane-group tables), HIP-graph replays equal to
+    the eager run."""
+    rng = np.random.default_rng(11)
+    rows = np.array([300_000, 70_000, 5, 0, 200_000, 131_072, 1, 90_000])
+    cards = np.array([3, 1000, 2, 1, 50_000, 17, 1, 1023])
+    keys = np.concatenate([rng.integers(0, c, r) for r, c in zip(rows, cards)]).astype(np.int32)
+    vals = rng.standard_normal(keys.shape[0]).astype(np.float32)
+    offs = np.zeros(rows.size + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(rows)
+    plan = batched.BatchedGroupBy(cuda(keys), cuda(vals), offs, aggs=("sum", "count", "mean"),
+                                  digit_bits=8, route=True)
+    assert plan.narrow_frames == 6
+    eager = plan.run(graph=False)
+    ek, es = eager.keys.clone(), eager.values["sum"].clone()
+    for _ in range(3):
+        junk = torch.randn(1 << 20, device="cuda").sum()
+        torch.cuda.synchronize()
+        r = plan.run(graph=True)
+        assert torch.equal(r.keys, ek) and torch.equal(r.values["sum"], es)
+    _check(r, keys, vals, offs, ("sum", "count", "mean"))
+
+
+def test_routed_key_outside_measured_range_raises():
+    """The split is fixed when the plan is made: a narrow frame's key moved
+    outside its measured range afterwards raises the device error."""
+    keys = np.array([5, 6, 7, 5, 100, 20000, 3], dtype=np.int32)
+    vals = np.ones(7, dtype=np.float32)
+    kt = cuda(keys)
+    plan = batched.BatchedGroupBy(kt, cuda(vals), np.array([0, 4, 7]), aggs=("sum",),
+                                  digit_bits=8, route=True)
+    assert plan.narrow_frames == 1
+    _check(plan.run(), keys, vals, np.array([0, 4, 7]), ("sum",))
+    kt[1] = 900      # frame 0 was measured as [5, 7]
+    with pytest.raises(_lib.JfxError, match="device error"):
+        plan.run()
+    assert _lib.device_error() == 0
