@@ -354,15 +354,18 @@ __device__ __forceinline__ uint64_t join_load_rec(const JoinChunkSide& c, int mo
   return 0;
 }
 
+// Non-temporal stores: the outputs are written once and not read again here
+// (1.58-1.60 ms against 1.72 ms for the fill at 1.6e8 pairs; non-temporal
+// record loads on top gave nothing more).
 __device__ __forceinline__ void join_store_rec(const JoinChunkSide& c, int mode, uint32_t q,
                                                uint64_t r) {
   if (mode == JP_REC8_2X4) {
-    gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out0), q) = (uint32_t)r;
-    gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out1), q) = (uint32_t)(r >> 32);
+    __builtin_nontemporal_store((uint32_t)r, &gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out0), q));
+    __builtin_nontemporal_store((uint32_t)(r >> 32), &gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out1), q));
   } else if (mode == JP_REC8_1X8) {
-    gat(reinterpret_cast<JFX_GLOBAL uint64_t*>(c.out0), q) = r;
+    __builtin_nontemporal_store(r, &gat(reinterpret_cast<JFX_GLOBAL uint64_t*>(c.out0), q));
   } else if (mode == JP_REC4) {
-    gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out0), q) = (uint32_t)r;
+    __builtin_nontemporal_store((uint32_t)r, &gat(reinterpret_cast<JFX_GLOBAL uint32_t*>(c.out0), q));
   }
 }
 
@@ -456,7 +459,7 @@ k_join_fill(const K* sl, uint64_t L, const K* sr, int oc, const uint64_t* wlo, c
         for (int u = 0; u < JFU; ++u) {
           const uint32_t q = q0 + u * JB;
           if (q < nout) {
-            if (okey) gat(okey, q) = key[u];
+            if (okey) __builtin_nontemporal_store(key[u], &gat(okey, q));
             join_store_rec(lc, lmode, q, lr[u]);
             join_store_rec(rc, rmode, q, rr[u]);
           }
