@@ -461,7 +461,58 @@ def workload_sort(kind: str, reps: int, n: int = 100_000_000) -> dict:
         "gather_equal": bool(torch.equal(keys.view(torch.int64)[il], sk.view(torch.int64))),
         "ties": int(ties.sum()),
         "ties_in_row_order": bool((il[1:][ties] > il[:-1][ties]).all())}
+    del ok, il, ties
+    out["rocprim"] = rocprim_sort(keys, sk, idx, reps)
     return out
+
+
+def rocprim_sort(keys, sk, idx, reps: int) -> dict | None:
+    """The vendor-library baseline: rocPRIM's radix_sort_pairs (stable, all
+    64 key bits) on the same device-resident uint64 keys carrying the uint32
+    row index, timed per call like jfx_sort; its output must equal jfx_sort's
+    (keys and row order: both sorts are stable)."""
+    import ctypes
+
+    import torch
+    from jaxframes_amd import _lib
+    from jaxframes_amd.build import CMP_LIB
+    if not CMP_LIB.exists():
+        return None
+    cmp = ctypes.CDLL(str(CMP_LIB))
+    f = cmp.jfx_cmp_rocprim_sort_u64_u32
+    f.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_uint64, ctypes.c_void_p,
+                                          ctypes.POINTER(ctypes.c_uint64), ctypes.c_void_p]
+    n = int(keys.shape[0])
+    dev = keys.device
+    rows = torch.arange(n, dtype=torch.int32, device=dev)
+    kout = torch.empty_like(keys)
+    vout = torch.empty_like(rows)
+    tb = ctypes.c_uint64(0)
+    stream = ctypes.c_void_p(_lib.stream_handle(dev))
+    args = (keys.data_ptr(), kout.data_ptr(), rows.data_ptr(), vout.data_ptr(), n)
+    if f(*args, None, ctypes.byref(tb), stream) != 0:
+        return None
+    temp = torch.empty(tb.value, dtype=torch.uint8, device=dev)
+
+    def call():
+        st = f(*args, temp.data_ptr(), ctypes.byref(tb), stream)
+        if st != 0:
+            raise RuntimeError(f"rocprim radix_sort_pairs failed: {st}")
+    call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    same = bool(torch.equal(kout.view(torch.int64), sk.view(torch.int64))
+                and torch.equal(vout, idx.to(torch.int32)))
+    del temp, kout, vout, rows
+    torch.cuda.empty_cache()
+    return {"what": "rocprim::radix_sort_pairs (ROCm 7.2 rocPRIM, bits 0-64) on the same keys "
+                    "with the uint32 row index", "ms_per_call": round(dt * 1e3, 3),
+            "rows_per_s": round(n / dt, 1), "temp_bytes": int(tb.value),
+            "output_equal_jfx_sort": same}
 
 
 def workload_join(reps: int) -> dict:

@@ -18,6 +18,10 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "_lib"
 OBJDIR = LIBDIR / "obj"
 LIB = LIBDIR / "libjfx.so"
+# the benchmark's vendor-library baseline (rocPRIM's radix sort), kept out of
+# libjfx.so
+CMP_SRC = PKG / "cmp" / "rocprim_sort.hip"
+CMP_LIB = LIBDIR / "libjfx_cmp.so"
 INCLUDE = PKG.parent / "include"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -51,13 +55,29 @@ def _compile(src: Path, force: bool) -> Path:
     return obj
 
 
+def _build_cmp(force: bool) -> Path:
+    """libjfx_cmp.so: rocPRIM's radix_sort_pairs behind one C entry point."""
+    if not force and CMP_LIB.exists() and CMP_LIB.stat().st_mtime >= CMP_SRC.stat().st_mtime:
+        return CMP_LIB
+    tmp = CMP_LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, *FLAGS, "-shared", str(CMP_SRC), "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {CMP_SRC.name}:\n{r.stderr[-6000:]}")
+    os.replace(tmp, CMP_LIB)
+    return CMP_LIB
+
+
 def build(force: bool = True, jobs: int | None = None) -> Path:
-    """Compile all HIP sources and link libjfx.so; returns its path."""
+    """Compile all HIP sources and link libjfx.so (and the benchmark's
+    libjfx_cmp.so); returns the path of libjfx.so."""
     OBJDIR.mkdir(parents=True, exist_ok=True)
     srcs = _sources()
-    jobs = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    jobs = jobs or min(len(srcs) + 1, max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        cmp_job = ex.submit(_build_cmp, force)
         objs = list(ex.map(lambda s: _compile(s, force), srcs))
+        cmp_job.result()
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -111,20 +111,32 @@ __device__ __forceinline__ void lookback_publish(uint32_t* desc, uint32_t tile, 
 // Decoupled lookback for one counter column: after lookback_publish, walks
 // back over earlier tiles, returns the exclusive prefix over tiles < tile and
 // publishes this tile's inclusive prefix. desc is laid out [tile][stride]
-// with this column at offset `col`. Descriptors are fetched four tiles at a
-// time (independent loads, one round trip) and consumed in order.
+// with this column at offset `col` (tiles * stride < 2^32). Descriptors are
+// fetched four tiles at a time and consumed in order. The four loads must
+// be independent so that a round costs one round trip: their addresses are
+// 32-bit offsets clamped to tile 0 (always valid) and the loads are
+// unconditional, the value of a tile below 0 being replaced afterwards. (With
+// 64-bit index products under per-load branches the compiler gave the
+// products a high half that aliased the previous load's destination
+// register and waited for that load before issuing the next: three round
+// trips per round.)
 __device__ __forceinline__ uint32_t lookback_u32(uint32_t* desc, uint32_t tile, uint32_t stride,
                                                  uint32_t col, uint32_t count, uint32_t* err) {
   if (tile == 0) return 0u;
   uint32_t excl = 0;
-  int64_t t = (int64_t)tile - 1;
+  int32_t t = (int32_t)tile - 1;
   uint32_t spins = 0;
   bool done = false;
   while (!done) {
     uint32_t v[4];
 #pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int32_t tq = t - q;
+      v[q] = ld_acq_word(desc + ((uint32_t)(tq > 0 ? tq : 0) * stride + col));
+    }
+#pragma unroll
     for (int q = 0; q < 4; ++q)
-      v[q] = (t - q >= 0) ? ld_acq_word(desc + (size_t)(t - q) * stride + col) : LB_PFX;
+      if (t - q < 0) v[q] = LB_PFX;
     int used = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {

@@ -31,20 +31,47 @@ struct ScatterArgs {
 };
 
 // Wave64 multisplit match: the mask of lanes in `valid` whose D-bit digit
-// equals this lane's, using one ballot per digit bit. Per bit: a
-// sign-extended bit (0 or ~0), one ballot, and an XNOR-AND on each 32-bit
-// half of the mask (one v_bitop3 each on gfx950).
+// equals this lane's, using one ballot per digit bit. Per bit: the bit
+// sign-extended (v_bfe_i32: 0 or ~0), one ballot of it, and on each 32-bit
+// half of the mask m := m & ~(ballot ^ bit) as one v_bitop3 (truth table
+// 0x90 over (m, ballot, bit)): four VALU instructions per bit. The asm
+// barrier keeps the ballot on the extracted bit; otherwise the compiler
+// re-derives it with a shift and a signed compare (five per bit).
 template <int D>
 __device__ __forceinline__ uint64_t match_digit(uint32_t dg, uint64_t valid) {
   uint32_t lo = (uint32_t)valid, hi = (uint32_t)(valid >> 32);
 #pragma unroll
   for (int b = 0; b < D; ++b) {
-    const uint32_t x = (uint32_t)((int32_t)(dg << (31 - b)) >> 31);
-    const uint64_t bb = __ballot(x != 0u);
-    lo &= ~((uint32_t)bb ^ x);
-    hi &= ~((uint32_t)(bb >> 32) ^ x);
+    uint32_t x = (uint32_t)__builtin_amdgcn_sbfe((int)dg, b, 1);
+    asm volatile("" : "+v"(x));
+    const uint64_t bb = ballot64(x != 0u);
+    lo = __builtin_amdgcn_bitop3_b32(lo, (uint32_t)bb, x, 0x90);
+    hi = __builtin_amdgcn_bitop3_b32(hi, (uint32_t)(bb >> 32), x, 0x90);
   }
   return ((uint64_t)hi << 32) | lo;
+}
+
+// The same for two digits at once, their instructions interleaved (no
+// dependent pair of VALU instructions back to back).
+template <int D>
+__device__ __forceinline__ void match_digit2(uint32_t d0, uint32_t d1, uint64_t v0, uint64_t v1,
+                                             uint64_t& m0, uint64_t& m1) {
+  uint32_t lo0 = (uint32_t)v0, hi0 = (uint32_t)(v0 >> 32);
+  uint32_t lo1 = (uint32_t)v1, hi1 = (uint32_t)(v1 >> 32);
+#pragma unroll
+  for (int b = 0; b < D; ++b) {
+    uint32_t x0 = (uint32_t)__builtin_amdgcn_sbfe((int)d0, b, 1);
+    uint32_t x1 = (uint32_t)__builtin_amdgcn_sbfe((int)d1, b, 1);
+    asm volatile("" : "+v"(x0), "+v"(x1));
+    const uint64_t b0 = ballot64(x0 != 0u);
+    const uint64_t b1 = ballot64(x1 != 0u);
+    lo0 = __builtin_amdgcn_bitop3_b32(lo0, (uint32_t)b0, x0, 0x90);
+    hi0 = __builtin_amdgcn_bitop3_b32(hi0, (uint32_t)(b0 >> 32), x0, 0x90);
+    lo1 = __builtin_amdgcn_bitop3_b32(lo1, (uint32_t)b1, x1, 0x90);
+    hi1 = __builtin_amdgcn_bitop3_b32(hi1, (uint32_t)(b1 >> 32), x1, 0x90);
+  }
+  m0 = ((uint64_t)hi0 << 32) | lo0;
+  m1 = ((uint64_t)hi1 << 32) | lo1;
 }
 
 // Bits needed to represent digits below r (at least 1).
@@ -80,9 +107,15 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
   constexpr uint32_t MASK = RADIX - 1;
 
   // Per-wave digit counters; 16 bits suffice (a wave ranks at most 64*KPT
-  // keys) and halve the LDS footprint of the 11-bit radix.
+  // keys) and halve the LDS footprint of the 11-bit radix. Each wave's row
+  // has one spare entry (index RADIX) that lanes past the tile's end count
+  // into when keys are ranked in pairs.
   using HT = typename std::conditional<(D > 8), uint16_t, uint32_t>::type;
-  __shared__ HT s_whist[W * RADIX];
+  constexpr int WS = RADIX + 1;
+  // tiles whose staging buffer takes over half the LDS hold a CU alone
+  constexpr bool RANK2 = TILE * XB > 80 * 1024;
+  static_assert(!RANK2 || KPT % 2 == 0, "keys per lane are ranked in pairs");
+  __shared__ HT s_whist[W * WS];
   __shared__ uint32_t s_tstart[RADIX];
   __shared__ uint32_t s_delta[RADIX];
   __shared__ uint32_t s_tmp[W + 1];
@@ -96,7 +129,7 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
   const uint32_t R = a.radix_eff;
 
   if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
-  for (int i = tid; i < W * RADIX; i += BLOCK) s_whist[i] = HT(0);
+  for (int i = tid; i < W * WS; i += BLOCK) s_whist[i] = HT(0);
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint64_t tbase = (uint64_t)tile * TILE;
@@ -122,27 +155,62 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
       else pay[i] = li < tn ? a.pay_in[tbase + li] : PT(0);
     }
   }
-#pragma unroll
-  for (int i = 0; i < KPT; ++i) dg[i] = dig(key[i]) & MASK;
 
   // Wave-level multisplit ranking: peers = lanes with the same digit.
-  HT* wh = s_whist + wid * RADIX;
+  HT* wh = s_whist + wid * WS;
+  if constexpr (RANK2) {
+    // Two keys per lane at a time. Each pair's digits are taken inside the
+    // loop, so the first pairs are ranked while the later keys are still
+    // loading (the loads complete in order). Every lane of a digit group
+    // writes the same new count; lanes past the tile's end count into the
+    // spare entry (no branch). At 100M uint64 keys + uint32 rows, 16384-key
+    // tiles: 544-574 us per pass against 584-630 us for one key at a time.
+    // It takes 14-30 more VGPRs, which would halve the occupancy of the
+    // smaller tiles (two workgroups per CU), so only tiles that hold a CU
+    // alone by their LDS use it.
 #pragma unroll
-  for (int i = 0; i < KPT; ++i) {
-    const uint32_t li = wb + (uint32_t)i * 64u + lane;
-    const bool valid = li < tn;
-    const uint64_t peers = match_digit<D>(dg[i], ballot64(valid));
-    if (valid) {
-      const uint32_t below = mbcnt(peers);
-      const uint32_t cnt = wh[dg[i]];
-      slot[i] = cnt + below;
+    for (int i = 0; i < KPT; i += 2) {
+      const uint32_t li = wb + (uint32_t)i * 64u + lane;
+      const bool v0 = li < tn, v1 = li + 64u < tn;
+      dg[i] = dig(key[i]) & MASK;
+      dg[i + 1] = dig(key[i + 1]) & MASK;
+      uint64_t p0, p1;
+      match_digit2<D>(dg[i], dg[i + 1], ballot64(v0), ballot64(v1), p0, p1);
+      const uint32_t e0 = v0 ? dg[i] : (uint32_t)RADIX;
+      const uint32_t e1 = v1 ? dg[i + 1] : (uint32_t)RADIX;
+      const uint32_t c0 = wh[e0];
+      slot[i] = c0 + mbcnt(p0);
       lds_order();
-      if (below == 0u) wh[dg[i]] = (HT)(cnt + (uint32_t)__popcll(peers));
+      wh[e0] = (HT)(c0 + (uint32_t)__popcll(p0));
       lds_order();
+      const uint32_t c1 = wh[e1];
+      slot[i + 1] = c1 + mbcnt(p1);
+      lds_order();
+      wh[e1] = (HT)(c1 + (uint32_t)__popcll(p1));
+      lds_order();
+      // keep each pair's ballots next to their use: hoisting the ballots of
+      // all keys ahead of the LDS chain exhausts the SGPRs
+      __builtin_amdgcn_sched_barrier(0);
     }
-    // keep each item's ballots next to their use: hoisting the ballots of
-    // all items ahead of the LDS chain exhausts the SGPRs
-    __builtin_amdgcn_sched_barrier(0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) dg[i] = dig(key[i]) & MASK;
+#pragma unroll
+    for (int i = 0; i < KPT; ++i) {
+      const uint32_t li = wb + (uint32_t)i * 64u + lane;
+      const bool valid = li < tn;
+      const uint64_t peers = match_digit<D>(dg[i], ballot64(valid));
+      if (valid) {
+        const uint32_t below = mbcnt(peers);
+        const uint32_t cnt = wh[dg[i]];
+        slot[i] = cnt + below;
+        lds_order();
+        if (below == 0u) wh[dg[i]] = (HT)(cnt + (uint32_t)__popcll(peers));
+        lds_order();
+      }
+      // keep each key's ballots next to their use
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
   __syncthreads();
 
@@ -151,8 +219,8 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
     uint32_t sum = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
-      const uint32_t c = s_whist[w * RADIX + d];
-      s_whist[w * RADIX + d] = (HT)sum;
+      const uint32_t c = s_whist[w * WS + d];
+      s_whist[w * WS + d] = (HT)sum;
       sum += c;
     }
     s_tstart[d] = sum;
