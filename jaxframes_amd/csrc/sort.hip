@@ -89,33 +89,31 @@ struct Stats4 {
 // pass into g_hist[pass][256], so a multi-pass sort reads its keys once
 // before the scatter passes instead of twice: digit 0 is folded from the
 // low-11-bit histogram, the higher digits take one LDS atomic each (one per
-// wave when the whole wave agrees on the digit). The top digit of keys with a
-// narrow range often takes two or three values in a wave, and atomics of one
-// instruction to one address serialise: it is counted in TOP_REP copies, lane
-// l adding to copy l % TOP_REP (float keys in [0, 14285], 1e8 keys: 596 us
-// with one table, 385 us with 8 copies, 345 us with 16; uniform int64 keys
-// 231-238 us either way).
-constexpr int TOP_REP = 16;
-constexpr int TOP_STRIDE = 257;  // copies of one bin fall in different banks
+// wave when the whole wave agrees on the digit). These atomics bound the
+// pass, so each higher digit is counted in HREP interleaved copies: lane l
+// adds bin b at b * HREP + l % HREP. The lanes of one copy (l, l + 16, l +
+// 32, l + 48) then share one group of 4 LDS banks and the copies never share
+// a bank, so a wave instruction meets at most 4-way bank conflicts (two on
+// average) where one table per digit met 4-5 on uniform digits and up to
+// 64-way same-address conflicts on skewed ones (keys of a narrow range, or
+// Zipf keys, give a digit two or three values in a wave).
+constexpr int HREP = 16;
 
 template <typename K, bool HIST>
 __global__ void __launch_bounds__(STATS_BLOCK)
 k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uint32_t* counts,
              unsigned long long* g_stats, uint32_t* g_maxbin, uint32_t* g_hist) {
   constexpr int NP = (int)sizeof(K);  // 8-bit layout digits
-  constexpr int NH = HIST ? (NP - 2) * 256 : 1;      // digits 1 .. NP-2
-  constexpr int NT = HIST ? TOP_REP * TOP_STRIDE : 1;  // digit NP-1
+  constexpr int NH = HIST ? (NP - 1) * 256 * HREP : 1;  // digits 1 .. NP-1, HREP copies
   __shared__ uint32_t s_low[2048];
   __shared__ uint32_t s_h[NH];
-  __shared__ uint32_t s_top[NT];
   for (int i = threadIdx.x; i < 2048; i += STATS_BLOCK) s_low[i] = 0u;
   if (HIST) {
     for (int i = threadIdx.x; i < NH; i += STATS_BLOCK) s_h[i] = 0u;
-    for (int i = threadIdx.x; i < NT; i += STATS_BLOCK) s_top[i] = 0u;
   }
   __syncthreads();
   const uint32_t lane = lane_id();
-  uint32_t* const my_top = s_top + (lane % TOP_REP) * TOP_STRIDE;
+  uint32_t* const my_h = s_h + (HIST ? (lane % HREP) : 0u);
   auto digits = [&](K o) {
     if (!HIST) return;
     const uint64_t act = ballot64(true);
@@ -123,11 +121,11 @@ k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uin
     for (int p = 1; p < NP; ++p) {
       const uint32_t bin = (uint32_t)((o >> (p * 8)) & (K)255);
       const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
-      uint32_t* const tab = p == NP - 1 ? s_top : s_h + (p - 1) * 256;
+      uint32_t* const tab = s_h + (p - 1) * 256 * HREP;
       if (ballot64(bin == b0) == act) {
-        if (lane == (uint32_t)__builtin_ctzll(act)) atomicAdd(&tab[b0], (uint32_t)__popcll(act));
+        if (lane == (uint32_t)__builtin_ctzll(act)) atomicAdd(&tab[b0 * HREP], (uint32_t)__popcll(act));
       } else {
-        atomicAdd(p == NP - 1 ? &my_top[bin] : &tab[bin], 1u);
+        atomicAdd(&my_h[(p - 1) * 256 * HREP + bin * HREP], 1u);
       }
     }
   };
@@ -235,15 +233,11 @@ k_sort_stats(const K* __restrict__ keys, uint64_t n, uint64_t chunk, int oc, uin
       for (int j = 0; j < 8; ++j) d0 += s_low[threadIdx.x + 256 * j];
       if (d0) atomicAdd(&g_hist[threadIdx.x], d0);
     }
-    for (int t = threadIdx.x; t < NH; t += STATS_BLOCK) {
-      const uint32_t v = s_h[t];
-      if (v) atomicAdd(&g_hist[256 + t], v);
-    }
-    if (threadIdx.x < 256) {
+    for (int t = threadIdx.x; t < (NP - 1) * 256; t += STATS_BLOCK) {
       uint32_t v = 0u;
 #pragma unroll
-      for (int c = 0; c < TOP_REP; ++c) v += s_top[c * TOP_STRIDE + threadIdx.x];
-      if (v) atomicAdd(&g_hist[256 * (NP - 1) + threadIdx.x], v);
+      for (int c = 0; c < HREP; ++c) v += s_h[t * HREP + c];
+      if (v) atomicAdd(&g_hist[256 + t], v);
     }
   }
 }
@@ -295,14 +289,18 @@ __global__ void __launch_bounds__(HIST_BLOCK)
 k_sort_hist(const K* __restrict__ keys, uint64_t n, int oc, uint32_t vary, uint32_t* g_hist) {
   constexpr int RADIX = 1 << D;
   constexpr int NP = (int)(sizeof(K) * 8 + D - 1) / D;
-  __shared__ uint32_t s_h[NP * RADIX];
-  for (int i = threadIdx.x; i < NP * RADIX; i += HIST_BLOCK) s_h[i] = 0u;
+  // interleaved copies against LDS bank conflicts, as in k_sort_stats (two
+  // for the 11-bit radix, whose tables are 8x larger)
+  constexpr int REP = D == 8 ? HREP : 2;
+  __shared__ uint32_t s_h[NP * RADIX * REP];
+  for (int i = threadIdx.x; i < NP * RADIX * REP; i += HIST_BLOCK) s_h[i] = 0u;
   __syncthreads();
   constexpr int VEC = 16 / sizeof(K);
   const uint64_t nvec = ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0) ? n / VEC : 0;
   const uint64_t stride = (uint64_t)gridDim.x * HIST_BLOCK;
   const uint4* kv = reinterpret_cast<const uint4*>(keys);
   const uint32_t lane = lane_id();
+  uint32_t* const my_h = s_h + lane % REP;
   // A digit on which the whole wave agrees (high digits of clustered or
   // frame-major keys) is added once with the wave's count instead of 64
   // colliding LDS atomics.
@@ -315,9 +313,9 @@ k_sort_hist(const K* __restrict__ keys, uint64_t n, int oc, uint32_t vary, uint3
       const uint32_t bin = (uint32_t)((o >> (p * D)) & (K)(RADIX - 1));
       const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
       if (ballot64(bin == b0) == act) {
-        if (lane == (uint32_t)__builtin_ctzll(act)) atomicAdd(&s_h[p * RADIX + b0], (uint32_t)__popcll(act));
+        if (lane == (uint32_t)__builtin_ctzll(act)) atomicAdd(&s_h[(p * RADIX + b0) * REP], (uint32_t)__popcll(act));
       } else {
-        atomicAdd(&s_h[p * RADIX + bin], 1u);
+        atomicAdd(&my_h[(p * RADIX + bin) * REP], 1u);
       }
     }
   };
@@ -343,7 +341,9 @@ k_sort_hist(const K* __restrict__ keys, uint64_t n, int oc, uint32_t vary, uint3
     one(keys[t]);
   __syncthreads();
   for (int t = threadIdx.x; t < NP * RADIX; t += HIST_BLOCK) {
-    const uint32_t v = s_h[t];
+    uint32_t v = 0u;
+#pragma unroll
+    for (int c = 0; c < REP; ++c) v += s_h[t * REP + c];
     if (v) atomicAdd(&g_hist[t], v);
   }
 }
