@@ -35,8 +35,8 @@ AGGS = ("sum", "count", "min", "max", "mean")
 
 # onesweep tile shapes of jfx_batched_groupby (sort_shape): (waves, keys per
 # lane) for 32-bit and for 64-bit composite keys
-SORT_SHAPES = {0: ((16, 8), (8, 16)), 1: ((8, 16), (16, 8)), 2: ((8, 8), (4, 16)),
-               3: ((4, 16), (8, 8))}
+SORT_SHAPES = {0: ((16, 8), (16, 16)), 1: ((8, 16), (16, 8)), 2: ((8, 8), (4, 16)),
+               3: ((4, 16), (8, 8)), 4: ((16, 16), (16, 16))}
 
 
 class Autotuner:
@@ -46,7 +46,9 @@ class Autotuner:
     on first use by timing every candidate. Thread-safe; optionally persisted
     as JSON (path from JFX_TUNE_FILE)."""
 
-    CANDIDATES = tuple((d, sh) for d in (8, 11) for sh in sorted(SORT_SHAPES))
+    # shape 4 (16384-key tiles) exists for 8-bit digits only
+    CANDIDATES = tuple((d, sh) for d in (8, 11) for sh in sorted(SORT_SHAPES)
+                       if not (d == 11 and sh == 4))
 
     def __init__(self, path: str | None = None):
         self._lock = threading.Lock()

@@ -112,8 +112,11 @@ __global__ void __launch_bounds__(W * 64) k_onesweep(ScatterArgs<K, PT> a, Digit
   // into when keys are ranked in pairs.
   using HT = typename std::conditional<(D > 8), uint16_t, uint32_t>::type;
   constexpr int WS = RADIX + 1;
-  // tiles whose staging buffer takes over half the LDS hold a CU alone
-  constexpr bool RANK2 = TILE * XB > 80 * 1024;
+  // Keys ranked in pairs: workgroups of 16 waves whose LDS holds a CU alone
+  // (a second workgroup would not fit in 160 KB), with payloads of at most 4
+  // bytes (the 8-byte ones leave no VGPRs for it at 4 waves per SIMD).
+  constexpr size_t LDS_BYTES = sizeof(HT) * W * WS + 8u * RADIX + 4u * (W + 2) + (size_t)TILE * XB;
+  constexpr bool RANK2 = W == 16 && 2 * LDS_BYTES > 160u * 1024u && sizeof(PT) <= 4;
   static_assert(!RANK2 || KPT % 2 == 0, "keys per lane are ranked in pairs");
   __shared__ HT s_whist[W * WS];
   __shared__ uint32_t s_tstart[RADIX];

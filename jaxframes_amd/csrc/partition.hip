@@ -174,8 +174,12 @@ struct PartWs {
 
 // onesweep scatter of the partition: the sort's tile of 8192 keys, 32-bit
 // keys as 16 waves x 8 keys and 64-bit keys as 8 x 16
-constexpr uint32_t PART_TILE = 8192;
+constexpr uint32_t PART_TILE = 8192;  // the smallest tile (sizes the descriptors)
 uint64_t part_tiles(uint64_t n) { return (n + PART_TILE - 1) / PART_TILE; }
+// Keys per onesweep tile: 16384 (16 waves x 16 keys, one workgroup per CU,
+// keys ranked in pairs) for 64-bit keys into at most 256 buckets, measured
+// at 2e8 int64 keys; 8192 otherwise.
+constexpr uint32_t part_tile(int kb, uint32_t nb) { return (kb == 8 && nb <= 256) ? 16384u : 8192u; }
 
 constexpr uint32_t PART_MAXCHUNKS = 512;
 
@@ -199,12 +203,13 @@ void part_carve(PartWs& w, Carver& c, uint32_t nb, uint64_t n) {
 
 template <typename K, typename PT, int D>
 void part_onesweep_launch(const ScatterArgs<K, PT>& a, HashDigit<K> hd, hipStream_t s) {
-  constexpr int W = sizeof(K) == 4 ? 16 : 8;
-  constexpr int KPT = PART_TILE / (W * 64);
-  static_assert(W * 64 * KPT == (int)PART_TILE, "partition tile shape");
+  constexpr uint32_t TILE = part_tile((int)sizeof(K), D == 8 ? 256u : 2048u);
+  constexpr int W = (sizeof(K) == 4 || TILE == 16384u) ? 16 : 8;
+  constexpr int KPT = TILE / (W * 64);
+  static_assert(W * 64 * KPT == (int)TILE, "partition tile shape");
+  const uint32_t tiles = (uint32_t)((a.n + TILE - 1) / TILE);
   JFX_LAUNCH(D == 8 ? "part_onesweep_d8" : "part_onesweep_d11", s,
-             (k_onesweep<K, PT, D, W, KPT, HashDigit<K>><<<(uint32_t)part_tiles(a.n), W * 64, 0, s>>>(
-                 a, hd)));
+             (k_onesweep<K, PT, D, W, KPT, HashDigit<K>><<<tiles, W * 64, 0, s>>>(a, hd)));
 }
 
 template <typename K, typename PT, int D>
@@ -239,7 +244,8 @@ int partition_typed(int kt, const void* keys, uint64_t n, uint32_t nb, int raw, 
                                      w.counts, nb, CM_DIRECT, 0u, nb, w.nchunks, w.colex, w.total)));
   JFX_HIP(hipGetLastError());
   const bool chunked = part_chunked();
-  const uint64_t ndesc = chunked ? 0 : part_tiles(n) * nb;
+  const uint32_t ptile = part_tile((int)sizeof(K), nb);
+  const uint64_t ndesc = chunked ? 0 : (n + ptile - 1) / ptile * nb;
   const uint32_t zero_blocks = ndesc ? (uint32_t)std::min<uint64_t>(256, (ndesc + 1023) / 1024) : 0u;
   JFX_LAUNCH("part_offsets", s, (k_part_offsets<<<1 + zero_blocks, 1024, 0, s>>>(
                                      w.total, nb, out_offsets, chunked ? nullptr : w.base, w.desc,

@@ -378,7 +378,7 @@ void geometry(SortCtx& c) {
 uint64_t os_tiles(uint64_t n, uint32_t tile = OS_TILE) { return (n + tile - 1) / tile; }
 
 // keys per tile of a shape
-uint32_t shape_tile(int shape) { return shape >= 2 ? 4096u : 8192u; }
+uint32_t shape_tile(int shape) { return shape == 4 ? 16384u : shape >= 2 ? 4096u : 8192u; }
 
 void carve(SortCtx& c, Carver& cv) {
   c.alt_keys = cv.take<char>(c.n * c.kb);
@@ -423,6 +423,12 @@ struct Passes {
         case 1: onesweep_wk<K4 ? 8 : 16, K4 ? 16 : 8>(D, a, dig, s); return hipGetLastError();
         case 2: onesweep_wk<K4 ? 8 : 4, K4 ? 8 : 16>(D, a, dig, s); return hipGetLastError();
         case 3: onesweep_wk<K4 ? 4 : 8, K4 ? 16 : 8>(D, a, dig, s); return hipGetLastError();
+        case 4:
+          if (D == 8) {
+            onesweep_d8_wk<16, 16>(a, dig, s);
+            return hipGetLastError();
+          }
+          break;
         default: break;
       }
     }
@@ -657,7 +663,7 @@ int sort_prepare_known(SortCtx& c, int kt, const void* keys_in, uint64_t n, int 
                        hipStream_t s, int shape, bool small_tiles) {
   JFX_CHECK(kt == JFX_U32 || kt == JFX_U64, JFX_ERR_INVALID, "known-range sorts take unsigned keys");
   JFX_CHECK(shape >= 0 && shape < SORT_SHAPES, JFX_ERR_INVALID, "bad sort shape");
-  JFX_CHECK(small_tiles || shape_tile(shape) == 8192u, JFX_ERR_INVALID,
+  JFX_CHECK(small_tiles || shape_tile(shape) >= 8192u, JFX_ERR_INVALID,
             "4096-key tiles need a workspace carved for them");
   c.shape = shape;
   c.desc_tile = small_tiles ? 4096u : 8192u;

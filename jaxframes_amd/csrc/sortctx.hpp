@@ -53,10 +53,12 @@ struct SortCtx {
 };
 
 // Onesweep tile shapes (waves x keys per lane): 0 = 32-bit keys 16 x 8,
-// 64-bit keys 8 x 16 (8192-key tiles, the default); 1 = 8 x 16 / 16 x 8
-// (8192); 2 = 8 x 8 / 4 x 16 (4096); 3 = 4 x 16 / 8 x 8 (4096). Shapes 1-3
-// are taken by sorts that carry a payload.
-constexpr int SORT_SHAPES = 4;
+// 64-bit keys 8 x 16 (8192-key tiles, the default; 64-bit keys with at most
+// a 4-byte payload and 8-bit digits take 16 x 16); 1 = 8 x 16 / 16 x 8
+// (8192); 2 = 8 x 8 / 4 x 16 (4096); 3 = 4 x 16 / 8 x 8 (4096); 4 = 16 x 16
+// for both widths (16384-key tiles, 8-bit digits only: 11-bit digits take
+// shape 0). Shapes 1-4 are taken by sorts that carry a payload.
+constexpr int SORT_SHAPES = 5;
 
 // small_tiles: carve lookback descriptors for 4096-key tiles (any shape).
 uint64_t sort_ws_bytes(int kt, int pay_bytes, uint64_t n, bool small_tiles = false);
