@@ -55,18 +55,6 @@ __device__ __forceinline__ uint64_t gbound(const K* a, uint64_t lo, uint64_t hi,
 }
 
 template <typename K>
-__device__ __forceinline__ uint32_t lbound(const K* s, uint32_t n, K x, bool upper) {
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    const K v = s[mid];
-    if (upper ? (v <= x) : (v < x)) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-
-template <typename K>
 __global__ void __launch_bounds__(256)
 k_join_windows(const K* sl, uint64_t L, const K* sr, uint64_t R, int oc, uint32_t ntiles,
                uint64_t* wlo, uint64_t* whi) {
@@ -78,31 +66,92 @@ k_join_windows(const K* sl, uint64_t L, const K* sr, uint64_t R, int oc, uint32_
   whi[t] = gbound(sr, 0, R, ordered(sl[b], oc), oc, true);
 }
 
+// First index i in [p, n) of sorted LDS array s with s[i] > x (upper) or
+// s[i] >= x (lower), n if none: probes p, p + 1, p + 3, p + 7, ... and
+// bisects the last bracket, so a bound d entries past p takes about
+// 2 log2(d) + 1 reads instead of log2(n - p).
+template <typename K>
+__device__ __forceinline__ uint32_t gallop(const K* s, uint32_t p, uint32_t n, K x, bool upper) {
+  uint32_t lo = p, hi = n, step = 1u;
+  while (true) {
+    const uint32_t q = lo + step - 1u;
+    if (q >= n) break;
+    const K v = s[q];
+    if (upper ? (v > x) : (v >= x)) {
+      hi = q;
+      break;
+    }
+    lo = q + 1u;
+    step <<= 1;
+  }
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const K v = s[mid];
+    if (upper ? (v <= x) : (v < x)) lo = mid + 1u;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The raw keys of this thread's JIPT consecutive rows of the tile starting
+// at tb (rows past L read as 0 and are not used): 16-byte loads when the
+// tile is whole and aligned.
+template <typename K>
+__device__ __forceinline__ void tile_rows(const K* sl, uint64_t L, uint64_t tb, K (&raw)[JIPT]) {
+  const uint64_t r0 = tb + (uint64_t)threadIdx.x * JIPT;
+  constexpr int PER = 16 / (int)sizeof(K);
+  static_assert(JIPT % PER == 0, "rows per thread fill whole 16-byte loads");
+  if (tb + JT <= L && (reinterpret_cast<uintptr_t>(sl + r0) & 15u) == 0) {
+    const uint4* q = reinterpret_cast<const uint4*>(sl + r0);
+#pragma unroll
+    for (int v = 0; v < JIPT / PER; ++v) {
+      const uint4 x = q[v];
+      __builtin_memcpy(&raw[v * PER], &x, 16);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < JIPT; ++j) raw[j] = r0 + j < L ? sl[r0 + j] : K(0);
+  }
+}
+
 // Bounds of the tile's rows: lo[j], cnt[j] for j in this thread's rows.
+// A thread's rows are consecutive and sorted, so with the window staged in
+// LDS a row equal to the previous one reuses its bounds and a larger one
+// gallops from the previous upper bound (the 1e8 x 1e7 join: about 10 left
+// rows per right key, so most bounds lie 0-2 entries on).
 template <typename K>
 __device__ __forceinline__ void tile_bounds(const K* sl, uint64_t L, const K* sr, int oc, uint64_t tb,
                                             uint64_t w0, uint64_t w1, const K* s_w, bool staged,
                                             uint64_t* lo, uint32_t* cnt) {
+  K raw[JIPT];
+  tile_rows(sl, L, tb, raw);
+  const uint32_t wn = (uint32_t)(w1 - w0);
+  bool have = false;
+  K px = K(0);
+  uint32_t pa = 0u, pb = 0u;
 #pragma unroll
   for (int j = 0; j < JIPT; ++j) {
     const uint64_t i = tb + (uint64_t)threadIdx.x * JIPT + j;
     lo[j] = 0;
     cnt[j] = 0;
     if (i < L) {
-      const K raw = sl[i];
-      if (is_nan_key(raw, oc)) continue;
-      const K x = ordered(raw, oc);
-      uint64_t a, b;
+      if (is_nan_key(raw[j], oc)) continue;
+      const K x = ordered(raw[j], oc);
       if (staged) {
-        const uint32_t wn = (uint32_t)(w1 - w0);
-        a = w0 + lbound(s_w, wn, x, false);
-        b = w0 + lbound(s_w, wn, x, true);
+        if (!(have && x == px)) {
+          pa = gallop(s_w, have ? pb : 0u, wn, x, false);
+          pb = gallop(s_w, pa, wn, x, true);
+          px = x;
+          have = true;
+        }
+        lo[j] = w0 + pa;
+        cnt[j] = pb - pa;
       } else {
-        a = gbound(sr, w0, w1, x, oc, false);
-        b = gbound(sr, a, w1, x, oc, true);
+        const uint64_t a = gbound(sr, w0, w1, x, oc, false);
+        const uint64_t b = gbound(sr, a, w1, x, oc, true);
+        lo[j] = a;
+        cnt[j] = (uint32_t)(b - a);
       }
-      lo[j] = a;
-      cnt[j] = (uint32_t)(b - a);
     }
   }
 }
