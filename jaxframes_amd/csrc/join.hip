@@ -9,13 +9,13 @@
 // [lower_bound(key_i), upper_bound(key_i)) of the sorted right keys:
 //   k_join_windows  one thread per left tile: the right-key window covering
 //                   the tile's first..last key (two global binary searches)
-//   k_join_count    per tile: each row's bounds by binary search inside the
-//                   window staged in LDS; the tile's pair count
+//   k_join_count    per tile: each row's bounds by a galloping search inside
+//                   the window staged in LDS; the tile's pair count
 //   k_join_scan     exclusive scan of tile counts (M read back by the host)
-//   k_join_fill     per tile: bounds again, an in-tile scan of counts, then
-//                   a load-balanced expansion: output position k of the tile
-//                   finds its left row by binary search over the scan, so a
-//                   row with many matches is spread over the workgroup
+//   k_join_fill     per chunk of at most JFCH outputs of a tile: bounds
+//                   again, an in-tile scan of counts, each row marking its
+//                   outputs in LDS, then the outputs written in order, so a
+//                   tile (or a row) with many matches is spread over chunks
 //   k_join_first    left/right joins: first match or -1 (:1000-1034)
 //   k_gather        payload gather by uint32 row, fill for row == ~0
 // Float keys: NaN never matches (== in the match matrix is false); -0.0
@@ -119,12 +119,12 @@ __device__ __forceinline__ void tile_rows(const K* sl, uint64_t L, uint64_t tb, 
 // LDS a row equal to the previous one reuses its bounds and a larger one
 // gallops from the previous upper bound (the 1e8 x 1e7 join: about 10 left
 // rows per right key, so most bounds lie 0-2 entries on).
+// The caller loads the rows (tile_rows) before staging the window, so the
+// two reads are in flight together.
 template <typename K>
-__device__ __forceinline__ void tile_bounds(const K* sl, uint64_t L, const K* sr, int oc, uint64_t tb,
-                                            uint64_t w0, uint64_t w1, const K* s_w, bool staged,
-                                            uint64_t* lo, uint32_t* cnt) {
-  K raw[JIPT];
-  tile_rows(sl, L, tb, raw);
+__device__ __forceinline__ void tile_bounds(const K (&raw)[JIPT], uint64_t L, const K* sr, int oc,
+                                            uint64_t tb, uint64_t w0, uint64_t w1, const K* s_w,
+                                            bool staged, uint64_t* lo, uint32_t* cnt) {
   const uint32_t wn = (uint32_t)(w1 - w0);
   bool have = false;
   K px = K(0);
@@ -172,11 +172,13 @@ k_join_count(const K* sl, uint64_t L, const K* sr, int oc, const uint64_t* wlo, 
   const uint32_t t = blockIdx.x;
   const uint64_t tb = (uint64_t)t * JT;
   const uint64_t w0 = wlo[t], w1 = whi[t];
+  K raw[JIPT];
+  tile_rows(sl, L, tb, raw);
   const bool staged = stage_window(sr, oc, w0, w1, s_w);
   __syncthreads();
   uint64_t lo[JIPT];
   uint32_t cnt[JIPT];
-  tile_bounds(sl, L, sr, oc, tb, w0, w1, s_w, staged, lo, cnt);
+  tile_bounds(raw, L, sr, oc, tb, w0, w1, s_w, staged, lo, cnt);
   uint64_t sum = 0;
 #pragma unroll
   for (int j = 0; j < JIPT; ++j) sum += cnt[j];
@@ -448,11 +450,13 @@ k_join_fill(const K* sl, uint64_t L, const K* sr, int oc, const uint64_t* wlo, c
     const uint32_t t = cmap[c];
     const uint64_t tb = (uint64_t)t * JT;
     const uint64_t w0 = wlo[t], w1 = whi[t];
+    K raw[JIPT];
+    tile_rows(sl, L, tb, raw);
     const bool staged = stage_window(sr, oc, w0, w1, s_w);
     __syncthreads();
     uint64_t lo[JIPT];
     uint32_t cnt[JIPT];
-    tile_bounds(sl, L, sr, oc, tb, w0, w1, s_w, staged, lo, cnt);
+    tile_bounds(raw, L, sr, oc, tb, w0, w1, s_w, staged, lo, cnt);
 #pragma unroll
     for (int j = 0; j < JIPT; ++j) {
       s_incl[threadIdx.x * JIPT + j] = cnt[j];
