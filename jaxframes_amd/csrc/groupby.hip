@@ -880,6 +880,7 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   const uint64_t cap = std::min<uint64_t>(256 * per_cu, (uint64_t)OP_GROUP * OP_L1);
   const uint64_t want = (n + (uint64_t)OP_W * STEP - 1) / ((uint64_t)OP_W * STEP);
   uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cap, want));
+  uint32_t group = OP_GROUP;
   const bool vec = (uintptr_t)keys % 16 == 0 && (uintptr_t)vals % 16 == 0;
   const char* nm = TE == OP_TE ? "onepass_table" : "onepass_table_2k";
   // float32 sums alone: the count-free tables (group presence from the sums)
@@ -887,10 +888,22 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   const bool nc = NC_OK && sc.out_count == nullptr && sc.out_mean == nullptr;
   if (nc) {
     if constexpr (NC_OK) {
+      // 32-bit keys, TE = 1024: 16 waves per block (128 KB of wave tables,
+      // one block per CU): the same 4096 waves as 1024 blocks of 4, each
+      // wave reading the same steps, but a quarter of the block tables to
+      // write and merge (3.1 MB instead of 12.6 MB). The 2048-residue tables
+      // (256 KB at 16 waves) and 64-bit keys (over the 128 VGPRs that 16
+      // waves per CU leave) keep 4 waves.
+      constexpr int WN = (TE == OP_TE && sizeof(K) == 4) ? 16 : OP_W;
+      if constexpr (WN != OP_W) {
+        const uint64_t want16 = (n + (uint64_t)WN * STEP - 1) / ((uint64_t)WN * STEP);
+        grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(256, want16));
+        group = (grid + OP_L1 - 1) / OP_L1;
+      }
       if (vec) {
-        JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, true, TE, true><<<grid, OP_W * 64, 0, s>>>(a)));
+        JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, true, TE, true, WN><<<grid, WN * 64, 0, s>>>(a)));
       } else {
-        JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, false, TE, true><<<grid, OP_W * 64, 0, s>>>(a)));
+        JFX_LAUNCH(nm, s, (k_onepass_table<K, V, MM, false, TE, true, WN><<<grid, WN * 64, 0, s>>>(a)));
       }
     }
   } else if (vec) {
@@ -903,14 +916,14 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
                     static_cast<V*>(sc.out_max), static_cast<MT*>(sc.out_mean)};
   // level-1 merges go to the part buffer after the block tables (it holds
   // SORT_MAXCHUNKS x 2048 min / max states, more than both need)
-  const uint32_t ng1 = (grid + OP_GROUP - 1) / OP_GROUP;
+  const uint32_t ng1 = (grid + group - 1) / group;
   JFX_CHECK(ng1 <= OP_L1, JFX_ERR_UNSUPPORTED, "one-pass grid exceeds the final merge");
   JFX_CHECK(((uint64_t)grid + ng1) * TE * sizeof(OpPart<V, MM>) <= w.part_bytes,
             JFX_ERR_WORKSPACE, "one-pass partials exceed the partial buffer");
   OpPart<V, MM>* l1 = a.part + (size_t)grid * TE;
   static_assert(TE % OP_MT == 0, "level-1 merge blocks cover whole tables");
   JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V, MM, TE><<<dim3(ng1, TE / OP_MT), OP_MT, 0, s>>>(
-                                      a.part, grid, a.ctl, a.epoch, l1)));
+                                      a.part, grid, group, a.ctl, a.epoch, l1)));
   JFX_HIP(hipGetLastError());
   // the final kernel publishes its result words to the host itself (no
   // separate read-back kernel), or to device words that are copied back

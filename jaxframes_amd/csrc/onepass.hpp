@@ -276,7 +276,9 @@ __device__ __forceinline__ void op_load_coal(const T* p, uint32_t lane, T (&o)[W
   }
 }
 
-// NC (float32 sums without counts, min / max or means): no count table. The
+// W waves per block (the workgroup's wave tables are merged into one block
+// table at its end). NC (float32 sums without counts, min / max or means):
+// no count table. The
 // wave tables start at -0.0 and every run is added as run + 0.0, which maps
 // -0.0 to +0.0 and keeps every other value: a round-to-nearest sum of
 // addends that are not all -0.0 is never -0.0, so an entry still holding
@@ -284,24 +286,24 @@ __device__ __forceinline__ void op_load_coal(const T* p, uint32_t lane, T (&o)[W
 // (0 or 1). The sums are those of tables started at +0.0 (adding -0.0 to a
 // sum changes nothing). This saves the count atomic of every run, a third
 // of the LDS atomic time, and 4 KB of LDS per block.
-template <typename K, typename V, bool MM, bool VEC, uint32_t TE, bool NC = false>
-__global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, MM> a) {
+template <typename K, typename V, bool MM, bool VEC, uint32_t TE, bool NC = false, int W = OP_W>
+__global__ void __launch_bounds__(W * 64) k_onepass_table(OnePassArgs<K, V, MM> a) {
   using T = typename OpT<V>::T;
   using U = typename ValTraits<V>::U;
   using C = MmCode<V>;
   constexpr bool DD = op_dd<V>;
   static_assert(!NC || (std::is_same<V, float>::value && !MM), "count-free tables: float32 sums only");
   constexpr uint64_t STEP = 64u * WT_RPL;
-  __shared__ T s_acc[OP_W * TE];
+  __shared__ T s_acc[W * TE];
   // float64 values: wave-private compensation of each sum entry
-  __shared__ T s_comp[DD ? OP_W * TE : 1];
+  __shared__ T s_comp[DD ? W * TE : 1];
   __shared__ uint32_t s_cnt[NC ? 1 : TE];
   // min / max codes: order-independent like the counts, one table per block
   __shared__ U s_min[MM ? TE : 1], s_max[MM ? TE : 1];
-  __shared__ K s_mn[OP_W], s_mx[OP_W];
+  __shared__ K s_mn[W], s_mx[W];
   __shared__ uint32_t s_stop;
   if (threadIdx.x == 0) s_stop = 0u;
-  for (uint32_t i = threadIdx.x; i < TE; i += OP_W * 64) {
+  for (uint32_t i = threadIdx.x; i < TE; i += W * 64) {
     if constexpr (!NC) s_cnt[i] = 0u;
     if constexpr (MM) {
       s_min[i] = ~U(0);
@@ -364,8 +366,8 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
   // W, w + 2W, ... of the W waves), so that at any time the waves read one
   // window of consecutive steps: 137-139 us at 100M rows against 140-141 us
   // for one contiguous range of steps per wave.
-  const uint64_t wave = (uint64_t)blockIdx.x * OP_W + wid;
-  const uint64_t nwaves = (uint64_t)gridDim.x * OP_W;
+  const uint64_t wave = (uint64_t)blockIdx.x * W + wid;
+  const uint64_t nwaves = (uint64_t)gridDim.x * W;
   const uint64_t nsteps = a.n / STEP;
   const uint64_t SW = STEP * nwaves;
 
@@ -502,7 +504,7 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
   __syncthreads();
   if (threadIdx.x == 0) {
     K bl = s_mn[0], bh = s_mx[0];
-    for (int w = 1; w < OP_W; ++w) {
+    for (int w = 1; w < W; ++w) {
       bl = s_mn[w] < bl ? s_mn[w] : bl;
       bh = s_mx[w] > bh ? s_mx[w] : bh;
     }
@@ -513,12 +515,12 @@ __global__ void __launch_bounds__(OP_W * 64) k_onepass_table(OnePassArgs<K, V, M
   // residue r < rt: its lane-group tables in (wave, group) order; the
   // entries from rt on are unused
   const uint32_t lg = TE / rt;
-  for (uint32_t r = threadIdx.x; r < TE; r += OP_W * 64) {
+  for (uint32_t r = threadIdx.x; r < TE; r += W * 64) {
     OpAcc<V, MM> st;
     st.init();
     if (r < rt) {
       bool seen = false;
-      for (int w = 0; w < OP_W; ++w)
+      for (int w = 0; w < W; ++w)
         for (uint32_t g = 0; g < lg; ++g) {
           const uint32_t e = w * TE + g * rt + ((r ^ g) & (rt - 1u));
           if constexpr (DD) st.add_pair(s_acc[e], s_comp[e]);
@@ -552,7 +554,8 @@ __device__ __forceinline__ void op_publish(const ReadBackSlot& rb, uint32_t stat
 
 // Two-level merge, thread = residue (coalesced reads): block (g, c) of
 // k_onepass_merge merges residues [OP_MT * c, OP_MT * (c + 1)) of block
-// tables [OP_GROUP * g, OP_GROUP * (g + 1)) in block order into l1[g];
+// tables [group * g, group * (g + 1)) in block order into l1[g] (group =
+// OP_GROUP, or 16 for the 256 blocks of 16 waves of the count-free tables);
 // k_onepass_final (one block) merges the at most OP_L1 level-1 tables in
 // order, checks the key range and writes the groups in key order. Each merge
 // issues all its loads before the first add: the level-1 blocks are small
@@ -583,15 +586,16 @@ __device__ __forceinline__ void merge_seq(OpAcc<V, MM>& st, const OpPart<V, MM>*
   st = x;
 }
 
-// grid (groups, TE / OP_MT)
+// grid (groups, TE / OP_MT); group (at most OP_GROUP) block tables per
+// level-1 table
 template <typename V, bool MM, uint32_t TE>
 __global__ void __launch_bounds__(OP_MT)
-k_onepass_merge(const OpPart<V, MM>* part, uint32_t nblocks, const uint32_t* ctl, uint32_t epoch,
-                OpPart<V, MM>* l1) {
+k_onepass_merge(const OpPart<V, MM>* part, uint32_t nblocks, uint32_t group, const uint32_t* ctl,
+                uint32_t epoch, OpPart<V, MM>* l1) {
   if (ctl[0] == epoch) return;  // stopped: the result is a miss
   constexpr uint32_t G = sizeof(OpPart<V, MM>) <= 24 ? OP_GROUP : OP_GROUP / 2;
-  const uint32_t c0 = blockIdx.x * OP_GROUP;
-  const uint32_t c1 = (c0 + OP_GROUP) < nblocks ? (c0 + OP_GROUP) : nblocks;
+  const uint32_t c0 = blockIdx.x * group;
+  const uint32_t c1 = (c0 + group) < nblocks ? (c0 + group) : nblocks;
   const uint32_t r = blockIdx.y * OP_MT + threadIdx.x;
   OpAcc<V, MM> st;
   st.init();
