@@ -213,6 +213,8 @@ def stream_handle(device=None) -> int:
         idx = torch.cuda.current_device()
     elif isinstance(device, int):
         idx = device
+    elif isinstance(device, torch.device) and device.index is not None:
+        idx = device.index
     else:
         idx = torch.device(device).index
         if idx is None:

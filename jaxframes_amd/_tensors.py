@@ -33,9 +33,10 @@ def device() -> torch.device:
 
 def to_device(x, dev: torch.device | None = None) -> torch.Tensor:
     """numpy / list / DLPack / torch -> 1-D torch tensor on the GPU."""
-    if (dev is None and isinstance(x, torch.Tensor) and x.is_cuda
-            and x.get_device() == torch.cuda.current_device()):
-        return x if x.is_contiguous() else x.contiguous()
+    if isinstance(x, torch.Tensor) and x.is_cuda:
+        # already resident: on the current device, or on dev when given
+        if x.get_device() == (torch.cuda.current_device() if dev is None else dev.index):
+            return x if x.is_contiguous() else x.contiguous()
     dev = dev or device()
     if isinstance(x, torch.Tensor):
         t = x
