@@ -67,30 +67,65 @@ struct HashDigit {
   }
 };
 
+// Each chunk's bucket histogram. Keys are read with 16-byte loads, four in
+// flight per thread. Up to 256 buckets are counted in 16 lane-interleaved
+// copies (bucket b of copy c at b * 16 + c, lane l adding to copy l % 16):
+// the lanes of one copy share a group of 4 LDS banks and copies never share
+// a bank, so an atomic instruction meets at most 4-way bank conflicts where
+// one table of uniform buckets met 4-5.
 template <typename K>
 __global__ void __launch_bounds__(1024)
 k_part_hist(const K* __restrict__ keys, uint64_t n, uint64_t chunk, HashDigit<K> hd,
             uint32_t* counts, uint32_t* hashes) {
-  __shared__ uint32_t s_h[2048];
-  for (uint32_t i = threadIdx.x; i < hd.nb; i += 1024) s_h[i] = 0u;
+  constexpr uint32_t REP = 16;
+  __shared__ uint32_t s_h[2048 > 256 * REP ? 2048 : 256 * REP];
+  const bool rep = hd.nb <= 256u;
+  const uint32_t ents = rep ? hd.nb * REP : hd.nb;
+  for (uint32_t i = threadIdx.x; i < ents; i += 1024) s_h[i] = 0u;
   __syncthreads();
+  uint32_t* const my = s_h + (rep ? (lane_id() % REP) : 0u);
+  const uint32_t sh = rep ? 4u : 0u;  // log2(REP)
+  auto add = [&](uint64_t i, K k) {
+    const uint32_t h = hd.hash(k);
+    if (hashes) hashes[i] = h;
+    const uint32_t b = hd.pow2_mask ? (h & hd.pow2_mask) : (h % hd.nb);
+    atomicAdd(&my[b << sh], 1u);
+  };
   const uint64_t cbeg = (uint64_t)blockIdx.x * chunk;
   const uint64_t cend = (cbeg + chunk) < n ? (cbeg + chunk) : n;
-  for (uint64_t i = cbeg + threadIdx.x; i < cend; i += 4096) {
-    K k[4];
+  constexpr int PER = 16 / (int)sizeof(K);
+  uint64_t tail = cbeg;
+  if ((reinterpret_cast<uintptr_t>(keys + cbeg) & 15u) == 0) {
+    const uint64_t nv = (cend - cbeg) / PER;
+    const uint4* kv = reinterpret_cast<const uint4*>(keys + cbeg);
+    for (uint64_t v = threadIdx.x; v < nv; v += 4096) {
+      uint4 x[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) k[u] = (i + u * 1024 < cend) ? keys[i + u * 1024] : K(0);
+      for (int u = 0; u < 4; ++u) x[u] = (v + u * 1024 < nv) ? kv[v + u * 1024] : make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (i + u * 1024 < cend) {
-        const uint32_t h = hd.hash(k[u]);
-        if (hashes) hashes[i + u * 1024] = h;
-        atomicAdd(&s_h[hd.pow2_mask ? (h & hd.pow2_mask) : (h % hd.nb)], 1u);
+      for (int u = 0; u < 4; ++u) {
+        if (v + u * 1024 < nv) {
+          K e[PER];
+          __builtin_memcpy(e, &x[u], 16);
+#pragma unroll
+          for (int j = 0; j < PER; ++j) add(cbeg + (v + u * 1024) * PER + j, e[j]);
+        }
       }
     }
+    tail = cbeg + nv * PER;
   }
+  for (uint64_t i = tail + threadIdx.x; i < cend; i += 1024) add(i, keys[i]);
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < hd.nb; i += 1024) counts[(size_t)blockIdx.x * hd.nb + i] = s_h[i];
+  for (uint32_t i = threadIdx.x; i < hd.nb; i += 1024) {
+    uint32_t c = 0u;
+    if (rep) {
+#pragma unroll
+      for (uint32_t r = 0; r < REP; ++r) c += s_h[i * REP + r];
+    } else {
+      c = s_h[i];
+    }
+    counts[(size_t)blockIdx.x * hd.nb + i] = c;
+  }
 }
 
 // Exclusive bucket offsets (int64, nb + 1 entries) from the bucket totals.
