@@ -123,3 +123,20 @@ def test_bucketize_padded_default_pad_matches_oracle(dt, name, pad):
         assert np.isnan(pad) and np.isnan(filler).all()
     else:
         assert (filler == pad).all()
+
+
+@pytest.mark.parametrize("dt", [np.int64, np.int32])
+def test_partition_unaligned_view(dt):
+    # a view one element into the buffer: the histogram pass takes its
+    # scalar-load path (16-byte loads need 16-byte aligned chunks)
+    rng = np.random.default_rng(11)
+    full = rng.integers(-2**30, 2**30, 200_001).astype(dt)
+    keys = full[1:]
+    t = cuda(full)[1:]
+    assert t.data_ptr() % 16 != 0
+    r = partition.hash_partition(t, 256, return_hashes=True)
+    perm, offs = oracle.hash_partition(keys, 256)
+    assert np.array_equal(r["offsets"].cpu().numpy(), offs)
+    assert np.array_equal(r["rows"].cpu().numpy(), perm)
+    h = r["hashes"].view(torch.int32).cpu().numpy().view(np.uint32)
+    assert np.array_equal(h, oracle.hash_multi_columns([keys]))
