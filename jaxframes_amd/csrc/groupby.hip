@@ -941,11 +941,13 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
                                       static_cast<K*>(out_keys), o, capacity, rb)));
   JFX_HIP(hipGetLastError());
   if (w.op_res) {
-    JFX_HIP(hipMemcpyAsync(res, w.op_res, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    uint64_t word = 0;
+    JFX_HIP(hipMemcpyAsync(&word, rb.seq_slot, sizeof(word), hipMemcpyDeviceToHost, s));
     JFX_HIP(hipStreamSynchronize(s));
+    JFX_CHECK(rb_decode(word, rb.seq, res), JFX_ERR_HIP, "one-pass result word missing");
     return JFX_OK;
   }
-  return read_back_wait(s, rb, 2 * sizeof(uint32_t), res);
+  return read_back_wait_word(s, rb, res);
 }
 
 // te = 1024 or 2048 residues; rt_min = te turns the lane-group tables off;

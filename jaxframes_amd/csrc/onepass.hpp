@@ -541,15 +541,14 @@ __global__ void __launch_bounds__(W * 64) k_onepass_table(OnePassArgs<K, V, MM> 
   }
 }
 
-// Result words that k_onepass_final publishes to the host (ReadBackSlot):
-// [0] status, [1] groups (on a miss: a lower bound of the key span max - min).
+// Result that k_onepass_final publishes to the host as one word (rb_word): a
+// status and the groups (on a miss: a lower bound of the key span max - min,
+// saturated to 2^30 - 1, which the retry decisions below 2048 do not see).
 constexpr uint32_t OP_MISS = 0u, OP_HIT = 1u, OP_CAPACITY = 2u;
 
 __device__ __forceinline__ void op_publish(const ReadBackSlot& rb, uint32_t status, uint32_t value) {
-  rb.dst[0] = status;
-  rb.dst[1] = value;
-  __threadfence_system();
-  __hip_atomic_store(rb.seq_slot, rb.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(rb.seq_slot, rb_word(rb.seq, status, value), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Two-level merge, thread = residue (coalesced reads): block (g, c) of
@@ -664,9 +663,9 @@ k_onepass_final(const OpPart<V, MM>* l1, uint32_t nl1, const uint64_t* mm, uint3
   if (!hit) {
     if (tid == 0) {
       // a lower bound of max - min over all keys (the keys that were read),
-      // saturated to 32 bits
+      // saturated to 30 bits
       const K span = hi >= lo ? hi - lo : K(0);
-      op_publish(rb, OP_MISS, span > (K)0xFFFFFFFFu ? 0xFFFFFFFFu : (uint32_t)span);
+      op_publish(rb, OP_MISS, span > (K)0x3FFFFFFFu ? 0x3FFFFFFFu : (uint32_t)span);
     }
     return;
   }

@@ -106,6 +106,29 @@ int read_back_wait(hipStream_t s, const ReadBackSlot& slot, size_t bytes, void* 
   return JFX_OK;
 }
 
+bool rb_decode(uint64_t word, uint64_t seq, uint32_t* res) {
+  if ((uint32_t)(word >> 32) != (uint32_t)seq) return false;
+  res[0] = (uint32_t)(word >> 30) & 3u;
+  res[1] = (uint32_t)word & ((1u << 30) - 1u);
+  return true;
+}
+
+int read_back_wait_word(hipStream_t s, const ReadBackSlot& slot, uint32_t* res) {
+  uint64_t* hseq = reinterpret_cast<uint64_t*>(t_coh.h) + RB_SEQ;
+  const auto t0 = std::chrono::steady_clock::now();
+  uint32_t polls = 0;
+  while (!rb_decode(__atomic_load_n(hseq, __ATOMIC_ACQUIRE), slot.seq, res)) {
+    if ((++polls & 255u) == 0 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+      JFX_HIP(hipStreamSynchronize(s));
+      JFX_CHECK(rb_decode(__atomic_load_n(hseq, __ATOMIC_ACQUIRE), slot.seq, res), JFX_ERR_HIP,
+                "read-back did not complete");
+      break;
+    }
+  }
+  return JFX_OK;
+}
+
 int read_back(hipStream_t s, const void* dev_src, size_t bytes, void* host_out) {
   JFX_CHECK(bytes % 4 == 0 && bytes <= READBACK_MAX, JFX_ERR_INVALID, "bad read-back size");
   ReadBackSlot slot{};

@@ -67,6 +67,22 @@ struct ReadBackSlot {
 int read_back_slot(ReadBackSlot& out);
 int read_back_wait(hipStream_t s, const ReadBackSlot& slot, size_t bytes, void* host_out);
 
+// A result of a 2-bit status and a value below 2^30, published as one 64-bit
+// word in the slot's sequence word: the low 32 bits of the sequence number
+// above the status and value. One relaxed system-scope store and no fence:
+// the host needs only this word (the kernel's other outputs are read later in
+// stream order), where a separate flag would need a system-scope release
+// ordering the words before it.
+__host__ __device__ inline uint64_t rb_word(uint64_t seq, uint32_t status, uint32_t value) {
+  return ((uint64_t)(uint32_t)seq << 32) | ((uint64_t)(status & 3u) << 30) |
+         (uint64_t)(value < (1u << 30) ? value : (1u << 30) - 1u);
+}
+// Spins for slot.seq's word (falling back to a stream sync after 2 ms) and
+// decodes it into res[0] = status, res[1] = value.
+int read_back_wait_word(hipStream_t s, const ReadBackSlot& slot, uint32_t* res);
+// Decodes a word copied back by the caller; false when it is not seq's.
+bool rb_decode(uint64_t word, uint64_t seq, uint32_t* res);
+
 // Device error word (one per device, in device memory owned by the library
 // image as a __device__ variable).
 uint32_t* device_error_word();
