@@ -554,7 +554,7 @@ struct SegWs {
   uint64_t* op_mm;  // [SORT_MAXCHUNKS * 2][2] one-pass: block min / max of the ordered keys
   uint32_t* op_ctl; // [4] one-pass: [0] stop word, [1] residues per table
   uint64_t part_bytes = 0;  // bytes at part
-  // one-pass result words in device memory (allocation-free calls: read
+  // one-pass result word in device memory (allocation-free calls: read
   // with a stream-ordered copy); null: the calling thread's pinned slot
   uint32_t* op_res = nullptr;
 };
@@ -840,7 +840,7 @@ int tile_reduce_key(const SortCtx& c, const SegWs& w, int vt, const void* keys, 
 // two small kernels and the result exchange) before the statistics-first
 // path, against 22-25 us saved by a hit (measured at 64K to 2M rows).
 // JFX_ONEPASS=0 turns the path off, =always tries it at any size.
-// res receives the final kernel's result words.
+// res receives the final kernel's result (status, value).
 
 std::atomic<uint32_t> g_op_epoch{0};
 constexpr uint64_t OP_MIN_ROWS = 1ull << 16;
@@ -925,8 +925,8 @@ int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const Se
   JFX_LAUNCH("onepass_merge", s, (k_onepass_merge<V, MM, TE><<<dim3(ng1, TE / OP_MT), OP_MT, 0, s>>>(
                                       a.part, grid, group, a.ctl, a.epoch, l1)));
   JFX_HIP(hipGetLastError());
-  // the final kernel publishes its result words to the host itself (no
-  // separate read-back kernel), or to device words that are copied back
+  // the final kernel publishes its result word to the host itself (no
+  // separate read-back kernel), or to a device word that is copied back
   ReadBackSlot rb{};
   if (w.op_res) {
     rb.dst = w.op_res;
@@ -1021,7 +1021,7 @@ int onepass_try(int key_type, int val_type, const void* keys, const void* vals, 
 // Scratch of jfx_groupby_noalloc, one copy per device, held until the process
 // ends: the one-pass block and level-1 tables (at most 780 tables of 1024
 // residues x 32 bytes, 25.6 MB, for 64-bit integer values with min / max),
-// the block min / max, the control words and the result words. Host code uses
+// the block min / max, the control words and the result word. Host code uses
 // the address from hipGetSymbolAddress, never the array's name.
 constexpr uint64_t JFX_NA_PART_BYTES = 26ull << 20;
 constexpr uint64_t JFX_NA_SCRATCH_BYTES = 27ull << 20;
