@@ -246,7 +246,7 @@ def batched_vs_plain(total_rows: int | None, reps: int = 5, plain_frames: int = 
     return out
 
 
-def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 1e6), slots: int = 4,
+def served_vs_plain(nq: int = 200, qps_levels=(2000.0, 10000.0, 1e6), slots: int = 4,
                     budget_bytes: int = 64 << 30, reps: int = 5,
                     switch_interval: float | None = None):
     """The served stream: 8 resident frames (1M..50M rows, int64 keys of
