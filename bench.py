@@ -523,35 +523,47 @@ def workload_join(reps: int) -> dict:
     from jaxframes_amd import datagen, joining
     lk, lv, rk, rv, m = datagen.join_tables()
     L, R = lk.shape[0], rk.shape[0]
-    pl, pr = _varying_digits(lk), _varying_digits(rk)
+    # the right side (the smaller) is sorted first; left rows outside its key
+    # range are dropped before the left sort (they have no pair): Lk rows
+    # remain, whose keys vary in pl digits
+    inr = (lk >= rk.min()) & (lk <= rk.max())
+    Lk = int(inr.sum())
+    pl, pr = _varying_digits(lk[inr]), _varying_digits(rk)
+    del inr
 
     def step():
         return joining.sort_merge_join(lk, lv, rk, rv, "inner")
 
     def bytes_of(name, launches):
-        # each side's two float32 payloads ride through its sort packed in
-        # one 8-byte record (read and written once by the packing)
+        # the right side's two float32 payloads are packed into one 8-byte
+        # record that rides through its sort (read and written once by the
+        # packing); the left side's are packed by the pre-filter's scatter
         if name == "pack_cols":
-            return (L + R) * (8 + 8)
+            return R * (8 + 8)
+        if name == "pf_count":
+            return L * 8
+        if name == "pf_scatter":
+            # every key read; the kept rows' payloads read, key and record written
+            return L * 8 + Lk * (8 + 8 + 8)
         if name.startswith("sort_stats") or name == "sort_hist":
-            return (L + R) * 8
+            return (Lk + R) * 8
         if name.startswith("onesweep"):
             if launches != pl + pr:
                 return None
-            return _sort_pass_bytes_pay(L, 8, 8, pl) + _sort_pass_bytes_pay(R, 8, 8, pr)
+            return _sort_pass_bytes_pay(Lk, 8, 8, pl) + _sort_pass_bytes_pay(R, 8, 8, pr)
         if name == "join_count":
-            return (L + R) * 8
+            return (Lk + R) * 8
         if name == "join_fill":
             # sorted keys and payload records of both sides read once; the key
             # and four payload columns written per pair
-            return (L + R) * (8 + 8) + m * (8 + 4 * 4)
+            return (Lk + R) * (8 + 8) + m * (8 + 4 * 4)
         if name == "gather":
             return launches * m * (4 + 4 + 4)
         return None
 
     out = {"workload": f"inner join of {L} x {R} int64 keys (right keys twice each, 80% of left "
                        f"rows match by Zipf(1.2)), 2 float32 payloads per side, {m} pairs",
-           "rows": L + R, "pairs": m, "passes": [pl, pr],
+           "rows": L + R, "pairs": m, "left_rows_in_right_range": Lk, "passes": [pl, pr],
            **_measure(step, reps, bytes_of, "join_1e8x1e7")}
     out["rows_per_s"] = round((L + R) / (out["ms_per_call"] / 1e3), 1)
     keys, cols = step()
@@ -622,9 +634,10 @@ def only_workload(name: str, reps: int, batched_rows: int = 400_000_000) -> dict
 
 def bandwidth_probe(nbytes: int = 800_000_000, reps: int = 5) -> dict:
     """HIP-event time of jfx_bandwidth_probe's read-only and copy kernels
-    over nbytes, for every grid / loads-in-flight / non-temporal setting;
-    returns the best of each mode (GB/s counts each byte read or written
-    once) and the whole sweep."""
+    over nbytes, for every grid / loads-in-flight / non-temporal setting,
+    and of its scatter copy (a radix pass's writes without the ranking) over
+    run lengths; returns the best of each mode (GB/s counts each byte read or
+    written once) and the whole sweep."""
     import ctypes
     import torch
     from jaxframes_amd import _lib
@@ -649,10 +662,39 @@ def bandwidth_probe(nbytes: int = 800_000_000, reps: int = 5) -> dict:
                     sweep.append({"mode": "read" if mode == 0 else "copy", "blocks": blocks,
                                   "unroll": unroll, "nontemporal": nt, "us": round(us, 1),
                                   "GBps": round(moved / (us * 1e-6) / 1e9, 1)})
+    # a radix pass's scattered writes without its ranking: 100M uint64 keys
+    # (with and without uint32 payloads) in 16384-key tiles written as runs
+    # of 16..256 keys (64 = an 8-bit digit of a uniform key at 16384-key
+    # tiles, 32 = 9 bits, 16 = 10 bits)
+    tiles = 6104
+    n = tiles * 16384
+    del src, dst
+    ssrc = torch.ones(n * 3, dtype=torch.int32, device=dev)
+    sdst = torch.empty_like(ssrc)
+    for mode in (2, 3):
+        moved = n * (12 if mode == 2 else 8)
+        for runlen in (16, 32, 64, 128, 256):
+            for nt in (0, 1):
+                args = (mode, ssrc.data_ptr(), sdst.data_ptr(), moved, 0, runlen, nt, stream)
+                _lib.call("jfx_bandwidth_probe", *args)
+                with _lib.KernelTimer() as kt:
+                    for _ in range(reps):
+                        _lib.call("jfx_bandwidth_probe", *args)
+                    torch.cuda.synchronize()
+                ms, cnt = next(iter(kt.stats().values()))
+                us = ms / cnt * 1e3
+                sweep.append({"mode": "scatter_pay" if mode == 2 else "scatter_keys",
+                              "keys": n, "run_keys": runlen, "nontemporal": nt,
+                              "us": round(us, 1),
+                              "GBps": round(2 * moved / (us * 1e-6) / 1e9, 1)})
     best = {}
-    for m in ("read", "copy"):
+    for m in ("read", "copy", "scatter_pay", "scatter_keys"):
         b = max((e for e in sweep if e["mode"] == m), key=lambda e: e["GBps"])
         best[m] = {**b, "frac_of_peak": round(b["GBps"] / PEAK_HBM_GBPS, 4)}
+    for m in ("scatter_pay", "scatter_keys"):
+        b = max((e for e in sweep if e["mode"] == m and e["run_keys"] == 64),
+                key=lambda e: e["GBps"])
+        best[m + "_run64"] = {**b, "frac_of_peak": round(b["GBps"] / PEAK_HBM_GBPS, 4)}
     return {"bytes": nbytes, "best": best, "sweep": sweep}
 
 

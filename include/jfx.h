@@ -278,6 +278,21 @@ int jfx_join_fill_columns(int key_type, const void* sorted_left, uint64_t n_left
 int jfx_pack_columns(int ncols, const void* const* cols, const int* col_bytes, uint64_t n,
                      void* out, int* out_record_bytes, void* stream);
 
+/* Inner-join range pre-filter of one side, run before that side's sort. The
+ * reference pairs only equal keys (parallel_algorithms.py:989-994), so a row
+ * whose key lies outside [sorted_other[0], sorted_other[n_other - 1]] (the
+ * other side sorted ascending) has no pair, and an inner join is unchanged
+ * without it. Keeps the other rows in row order: their keys to out_keys, their
+ * payload columns packed into one record per row (as jfx_pack_columns) to
+ * out_records when ncols > 0, their row indices to out_rows when non-null,
+ * and the kept count to *out_count (synchronises the stream once). Integer key
+ * types only (JFX_ERR_UNSUPPORTED for floats); outputs hold n rows;
+ * workspace of jfx_compact_workspace_size bytes. */
+int jfx_join_prefilter(int key_type, const void* keys, uint64_t n, const void* sorted_other,
+                       uint64_t n_other, int ncols, const void* const* cols, const int* col_bytes,
+                       void* out_keys, void* out_records, uint32_t* out_rows, uint64_t* out_count,
+                       void* workspace, uint64_t ws_bytes, void* stream);
+
 /* For each sorted probe key, build_rows[first index of an equal key in
  * sorted_build], or 0xFFFFFFFF when none: the searchsorted + equality test
  * of the left / right / outer branches (parallel_algorithms.py:1014-1047). */
@@ -500,7 +515,13 @@ int jfx_filter_rows(int col_type, const void* col, int cmp, const void* value, u
  * of `blocks` workgroups of 256 threads, non-temporal loads when
  * nontemporal != 0) and stores one word per workgroup to dst (dst holds
  * `blocks` uint32); mode 1 copies src to dst (both `bytes`). Buffers and
- * bytes 16-byte aligned. Asynchronous on `stream`. */
+ * bytes 16-byte aligned. Modes 2 and 3 are a radix pass's memory pattern
+ * without its ranking: n = bytes / 12 (mode 2: n uint64 keys followed by n
+ * uint32 payloads) or bytes / 8 (mode 3: keys only), n a multiple of 16384;
+ * each 16384-key tile is read in order and written as runs of `unroll` keys
+ * (a divisor of 16384), run d of tile t to [d*T*unroll + t*unroll, +unroll)
+ * of the T = n / 16384 tiles; `blocks` is unused and nontemporal selects
+ * non-temporal stores. Asynchronous on `stream`. */
 int jfx_bandwidth_probe(int mode, const void* src, void* dst, uint64_t bytes, uint32_t blocks,
                         uint32_t unroll, int nontemporal, void* stream);
 

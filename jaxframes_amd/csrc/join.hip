@@ -18,6 +18,13 @@
 //                   tile (or a row) with many matches is spread over chunks
 //   k_join_first    left/right joins: first match or -1 (:1000-1034)
 //   k_gather        payload gather by uint32 row, fill for row == ~0
+//   k_pf_count / k_pf_scan / k_pf_scatter
+//                   inner-join range pre-filter of one side before its sort:
+//                   a row whose key lies outside [first, last] of the other
+//                   side's sorted keys has no match (:989-994 pairs only
+//                   equal keys), so it is dropped; the kept rows keep their
+//                   order, and their payload columns are packed into the
+//                   sort's records in the same pass
 // Float keys: NaN never matches (== in the match matrix is false); -0.0
 // matches +0.0. Comparisons run on the order-preserving unsigned image.
 #include <algorithm>
@@ -622,6 +629,141 @@ k_pack_cols(JoinSideOut p, uint64_t first, uint64_t n) {
   }
 }
 
+// ---- inner-join range pre-filter ------------------------------------------
+// Chunks of whole 4096-row tiles (at most PF_MAXCHUNKS); within a tile each
+// wave owns 1024 consecutive rows, walked 64 at a time, so ballot order is
+// row order.
+constexpr int PF_BLOCK = 256;
+constexpr int PF_W = PF_BLOCK / 64;
+constexpr int PF_IPT = 16;
+constexpr uint32_t PF_TILE = PF_BLOCK * PF_IPT;
+constexpr uint32_t PF_MAXCHUNKS = 1024;
+
+__device__ __forceinline__ uint64_t pf_row(uint64_t t, int w, int j, uint32_t lane) {
+  return t + (uint64_t)w * 64u * PF_IPT + (uint64_t)j * 64u + lane;
+}
+
+// Keys compare in their own type (signed or unsigned integers): the other
+// side's first and last sorted keys bound every key that can match.
+template <typename K>
+__global__ void __launch_bounds__(PF_BLOCK)
+k_pf_count(const K* keys, uint64_t n, uint64_t chunk, const K* other, uint64_t n_other,
+           uint32_t* counts) {
+  __shared__ uint32_t s_w[PF_W];
+  const K lo = other[0], hi = other[n_other - 1];
+  const int wid = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
+  const uint64_t cbeg = (uint64_t)blockIdx.x * chunk;
+  const uint64_t cend = (cbeg + chunk) < n ? (cbeg + chunk) : n;
+  uint32_t cnt = 0;
+  for (uint64_t t = cbeg; t < cend; t += PF_TILE) {
+    K kv[PF_IPT];
+#pragma unroll
+    for (int j = 0; j < PF_IPT; ++j) {
+      const uint64_t r = pf_row(t, wid, j, lane);
+      kv[j] = r < cend ? keys[r] : K(0);
+    }
+#pragma unroll
+    for (int j = 0; j < PF_IPT; ++j) {
+      const uint64_t r = pf_row(t, wid, j, lane);
+      cnt += (uint32_t)__popcll(ballot64(r < cend && lo <= kv[j] && kv[j] <= hi));
+    }
+  }
+  if (lane == 0) s_w[wid] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t c = 0;
+    for (int w = 0; w < PF_W; ++w) c += s_w[w];
+    counts[blockIdx.x] = c;
+  }
+}
+
+// Exclusive scan of the chunk counts in place; the kept total to *total.
+__global__ void __launch_bounds__(1024) k_pf_scan(uint32_t* counts, uint32_t nchunks,
+                                                  uint64_t* total) {
+  __shared__ uint32_t s[PF_MAXCHUNKS];
+  __shared__ uint32_t s_tmp[1024 / 64 + 1];
+  for (uint32_t i = threadIdx.x; i < PF_MAXCHUNKS; i += 1024) s[i] = i < nchunks ? counts[i] : 0u;
+  __syncthreads();
+  const uint32_t tot = block_excl_scan_lds<PF_MAXCHUNKS, 1024>(s, s_tmp);
+  for (uint32_t i = threadIdx.x; i < nchunks; i += 1024) counts[i] = s[i];
+  if (threadIdx.x == 0) *total = tot;
+}
+
+// Kept row i at output q: its key, its payload columns packed into one
+// record (column j at byte offset off[j], as k_pack_cols) and, when
+// out_rows is set, i itself.
+template <typename K>
+__global__ void __launch_bounds__(PF_BLOCK)
+k_pf_scatter(const K* keys, uint64_t n, uint64_t chunk, const K* other, uint64_t n_other,
+             const uint32_t* offsets, K* out_keys, JoinSideOut p, uint32_t* out_rows) {
+  __shared__ uint32_t s_w[PF_W + 1];
+  const K lo = other[0], hi = other[n_other - 1];
+  const int wid = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
+  const uint64_t cbeg = (uint64_t)blockIdx.x * chunk;
+  const uint64_t cend = (cbeg + chunk) < n ? (cbeg + chunk) : n;
+  const bool two4 = p.rec_bytes == 8 && p.ncols == 2 && p.bytes[0] == 4 && p.bytes[1] == 4;
+  uint32_t run = offsets[blockIdx.x];
+  for (uint64_t t = cbeg; t < cend; t += PF_TILE) {
+    K kv[PF_IPT];
+    uint64_t m[PF_IPT];
+    uint32_t wcnt = 0;
+#pragma unroll
+    for (int j = 0; j < PF_IPT; ++j) {
+      const uint64_t r = pf_row(t, wid, j, lane);
+      kv[j] = r < cend ? keys[r] : K(0);
+    }
+#pragma unroll
+    for (int j = 0; j < PF_IPT; ++j) {
+      const uint64_t r = pf_row(t, wid, j, lane);
+      m[j] = ballot64(r < cend && lo <= kv[j] && kv[j] <= hi);
+      wcnt += (uint32_t)__popcll(m[j]);
+    }
+    if (lane == 0) s_w[wid] = wcnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t c = 0;
+      for (int w = 0; w < PF_W; ++w) {
+        const uint32_t x = s_w[w];
+        s_w[w] = c;
+        c += x;
+      }
+      s_w[PF_W] = c;
+    }
+    __syncthreads();
+    uint32_t pos = run + s_w[wid];
+#pragma unroll
+    for (int j = 0; j < PF_IPT; ++j) {
+      if ((m[j] >> lane) & 1ull) {
+        const uint64_t r = pf_row(t, wid, j, lane);
+        const uint32_t q = pos + mbcnt(m[j]);
+        out_keys[q] = kv[j];
+        if (out_rows) out_rows[q] = (uint32_t)r;
+        if (two4) {
+          const uint64_t rec = (uint64_t)static_cast<const uint32_t*>(p.src[0])[r] |
+                               ((uint64_t)static_cast<const uint32_t*>(p.src[1])[r] << 32);
+          static_cast<uint64_t*>(p.out[0])[q] = rec;
+        } else if (p.ncols) {
+          uint64_t rec = 0;
+#pragma unroll
+          for (int c = 0; c < JFX_JOIN_MAXCOLS; ++c) {
+            if (c >= p.ncols) break;
+            const uint64_t v = p.bytes[c] == 8 ? static_cast<const uint64_t*>(p.src[c])[r]
+                                               : (uint64_t)static_cast<const uint32_t*>(p.src[c])[r];
+            rec |= v << (8 * p.off[c]);
+          }
+          if (p.rec_bytes == 8) static_cast<uint64_t*>(p.out[0])[q] = rec;
+          else static_cast<uint32_t*>(p.out[0])[q] = (uint32_t)rec;
+        }
+      }
+      pos += (uint32_t)__popcll(m[j]);
+    }
+    run += s_w[PF_W];
+    __syncthreads();
+  }
+}
+
 // First match of every sorted probe row in the sorted build side: build row
 // (through perm) or ~0.
 template <typename K>
@@ -765,11 +907,83 @@ int side_from_abi(const jfx_join_side* a, JoinSideOut& o) {
   return JFX_OK;
 }
 
+template <typename K>
+int prefilter_typed(const void* keys, uint64_t n, const void* other, uint64_t n_other,
+                    const JoinSideOut& p, void* out_keys, uint32_t* out_rows, uint64_t* out_count,
+                    void* ws, uint64_t ws_bytes, hipStream_t s) {
+  Carver cv(ws, ws_bytes);
+  uint32_t* counts = cv.take<uint32_t>(PF_MAXCHUNKS);
+  uint64_t* total = cv.take<uint64_t>(1);
+  JFX_CHECK(cv.fits(), JFX_ERR_WORKSPACE, "pre-filter workspace too small");
+  const uint64_t tiles = (n + PF_TILE - 1) / PF_TILE;
+  const uint64_t nch = std::max<uint64_t>(1, std::min<uint64_t>(PF_MAXCHUNKS, tiles));
+  const uint64_t chunk = (tiles + nch - 1) / nch * PF_TILE;
+  const uint32_t nchunks = (uint32_t)((n + chunk - 1) / chunk);
+  const K* k = static_cast<const K*>(keys);
+  const K* o = static_cast<const K*>(other);
+  JFX_LAUNCH("pf_count", s, (k_pf_count<K><<<nchunks, PF_BLOCK, 0, s>>>(k, n, chunk, o, n_other,
+                                                                          counts)));
+  JFX_LAUNCH("pf_scan", s, (k_pf_scan<<<1, 1024, 0, s>>>(counts, nchunks, total)));
+  JFX_LAUNCH("pf_scatter", s, (k_pf_scatter<K><<<nchunks, PF_BLOCK, 0, s>>>(
+                                  k, n, chunk, o, n_other, counts, static_cast<K*>(out_keys), p,
+                                  out_rows)));
+  JFX_HIP(hipGetLastError());
+  return read_back(s, total, sizeof(uint64_t), out_count);
+}
+
 }  // namespace
 
 }  // namespace jfx
 
 extern "C" {
+
+int jfx_join_prefilter(int key_type, const void* keys, uint64_t n, const void* sorted_other,
+                       uint64_t n_other, int ncols, const void* const* cols, const int* col_bytes,
+                       void* out_keys, void* out_records, uint32_t* out_rows, uint64_t* out_count,
+                       void* workspace, uint64_t ws_bytes, void* stream) {
+  JFX_CHECK(out_count != nullptr, JFX_ERR_INVALID, "out_count is null");
+  JFX_CHECK(key_type == JFX_U32 || key_type == JFX_I32 || key_type == JFX_U64 ||
+                key_type == JFX_I64,
+            JFX_ERR_UNSUPPORTED, "the range pre-filter takes integer keys");
+  JFX_CHECK(n < (1ull << 32) && n_other < (1ull << 32), JFX_ERR_UNSUPPORTED,
+            "each side must have fewer than 2^32 rows");
+  JFX_CHECK(ncols >= 0 && ncols <= JFX_JOIN_MAXCOLS && (ncols == 0 || (cols && col_bytes)),
+            JFX_ERR_INVALID, "0 to 8 payload columns");
+  jfx::JoinSideOut p{};
+  p.ncols = ncols;
+  int off = 0;
+  for (int j = 0; j < ncols; ++j) {
+    JFX_CHECK(col_bytes[j] == 4 || col_bytes[j] == 8, JFX_ERR_UNSUPPORTED,
+              "columns must be 4 or 8 bytes");
+    JFX_CHECK(n == 0 || cols[j] != nullptr, JFX_ERR_INVALID, "null column");
+    p.src[j] = cols[j];
+    p.bytes[j] = col_bytes[j];
+    p.off[j] = off;
+    off += col_bytes[j];
+  }
+  JFX_CHECK(off <= 8, JFX_ERR_UNSUPPORTED, "packed records hold at most 8 bytes");
+  p.rec_bytes = off;
+  p.out[0] = out_records;
+  if (n == 0 || n_other == 0) {
+    *out_count = 0;
+    return JFX_OK;
+  }
+  JFX_CHECK(keys && sorted_other && out_keys && workspace && (ncols == 0 || out_records),
+            JFX_ERR_INVALID, "null argument");
+  hipStream_t s = jfx::as_stream(stream);
+  if (jfx::key_bytes(key_type) == 4) {
+    if (key_type == JFX_I32)
+      return jfx::prefilter_typed<int32_t>(keys, n, sorted_other, n_other, p, out_keys, out_rows,
+                                           out_count, workspace, ws_bytes, s);
+    return jfx::prefilter_typed<uint32_t>(keys, n, sorted_other, n_other, p, out_keys, out_rows,
+                                          out_count, workspace, ws_bytes, s);
+  }
+  if (key_type == JFX_I64)
+    return jfx::prefilter_typed<int64_t>(keys, n, sorted_other, n_other, p, out_keys, out_rows,
+                                         out_count, workspace, ws_bytes, s);
+  return jfx::prefilter_typed<uint64_t>(keys, n, sorted_other, n_other, p, out_keys, out_rows,
+                                        out_count, workspace, ws_bytes, s);
+}
 
 int jfx_join_workspace_size(uint64_t n_left, uint64_t* out_bytes) {
   JFX_CHECK(out_bytes != nullptr, JFX_ERR_INVALID, "out_bytes is null");

@@ -5,7 +5,14 @@
 //                 16-byte loads in flight per step, and folds them into an
 //                 XOR that each workgroup stores once (so no load is dead);
 // k_stream_copy   the same walk storing what it loads.
-// NT selects non-temporal loads (streamed past the caches' reuse policy).
+// k_scatter_copy  a radix pass's memory pattern without its ranking: tiles
+//                 of 16384 uint64 keys (and uint32 payloads) read in order
+//                 and written as 16384 / runlen runs per tile, run d of tile
+//                 t to [d * T * runlen + t * runlen, +runlen) of T tiles, so
+//                 that every run lands in a different region of the output
+//                 as a digit run of an ideally uniform radix pass does.
+// NT selects non-temporal loads (streamed past the caches' reuse policy); for
+// the scatter copy it selects non-temporal stores.
 #include "runtime.hpp"
 
 namespace jfx {
@@ -56,6 +63,65 @@ __global__ void __launch_bounds__(256) k_stream_copy(const v4u* src, v4u* dst, u
   for (; i < n16; i += stride) dst[i] = ld16<U, NT>(src + i);
 }
 
+constexpr uint32_t SC_TILE = 16384;  // keys per tile of the scatter copy
+constexpr int SC_KPT = 16;             // keys per lane (16 waves of 64 lanes)
+
+// One tile per workgroup; lane l of wave w holds keys w*1024 + i*64 + l
+// (the onesweep load order), so one wave store writes 64 consecutive keys of
+// a run (runlen >= 64) or 64 / runlen whole runs.
+template <bool PAY, bool NT>
+__global__ void __launch_bounds__(1024)
+k_scatter_copy(const uint64_t* kin, const uint32_t* pin, uint64_t* kout, uint32_t* pout,
+               uint32_t tiles, uint32_t runlen) {
+  const uint32_t t = blockIdx.x;
+  const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
+  const uint64_t tb = (uint64_t)t * SC_TILE;
+  const uint64_t region = (uint64_t)tiles * runlen;
+  uint64_t k[SC_KPT];
+  uint32_t p[SC_KPT];
+#pragma unroll
+  for (int i = 0; i < SC_KPT; ++i) {
+    const uint32_t j = w * 64u * SC_KPT + (uint32_t)i * 64u + l;
+    k[i] = kin[tb + j];
+    if constexpr (PAY) p[i] = pin[tb + j];
+  }
+#pragma unroll
+  for (int i = 0; i < SC_KPT; ++i) {
+    const uint32_t j = w * 64u * SC_KPT + (uint32_t)i * 64u + l;
+    const uint64_t pos = (uint64_t)(j / runlen) * region + (uint64_t)t * runlen + (j % runlen);
+    if constexpr (NT) __builtin_nontemporal_store(k[i], kout + pos);
+    else kout[pos] = k[i];
+  }
+  if constexpr (PAY) {
+#pragma unroll
+    for (int i = 0; i < SC_KPT; ++i) {
+      const uint32_t j = w * 64u * SC_KPT + (uint32_t)i * 64u + l;
+      const uint64_t pos = (uint64_t)(j / runlen) * region + (uint64_t)t * runlen + (j % runlen);
+      if constexpr (NT) __builtin_nontemporal_store(p[i], pout + pos);
+      else pout[pos] = p[i];
+    }
+  }
+}
+
+template <bool NT>
+int scatter_launch(bool pay, const void* src, void* dst, uint64_t n, uint32_t runlen,
+                   hipStream_t s) {
+  const uint32_t tiles = (uint32_t)(n / SC_TILE);
+  const uint64_t* kin = static_cast<const uint64_t*>(src);
+  uint64_t* kout = static_cast<uint64_t*>(dst);
+  const uint32_t* pin = reinterpret_cast<const uint32_t*>(kin + n);
+  uint32_t* pout = reinterpret_cast<uint32_t*>(kout + n);
+  if (pay) {
+    JFX_LAUNCH("scatter_copy", s, (k_scatter_copy<true, NT><<<tiles, 1024, 0, s>>>(
+                                       kin, pin, kout, pout, tiles, runlen)));
+  } else {
+    JFX_LAUNCH("scatter_copy", s, (k_scatter_copy<false, NT><<<tiles, 1024, 0, s>>>(
+                                       kin, nullptr, kout, nullptr, tiles, runlen)));
+  }
+  JFX_HIP(hipGetLastError());
+  return JFX_OK;
+}
+
 template <int U, bool NT>
 int probe_launch(int mode, const void* src, void* dst, uint64_t n16, uint32_t blocks, hipStream_t s) {
   if (mode == 0) {
@@ -86,8 +152,25 @@ int probe_unroll(int mode, const void* src, void* dst, uint64_t n16, uint32_t bl
 
 extern "C" int jfx_bandwidth_probe(int mode, const void* src, void* dst, uint64_t bytes,
                                    uint32_t blocks, uint32_t unroll, int nontemporal, void* stream) {
-  JFX_CHECK(mode == 0 || mode == 1, JFX_ERR_INVALID, "mode must be 0 (read) or 1 (copy)");
+  JFX_CHECK(mode >= 0 && mode <= 3, JFX_ERR_INVALID,
+            "mode must be 0 (read), 1 (copy), 2 (scatter copy of keys and payloads) or 3 "
+            "(scatter copy of keys)");
   JFX_CHECK(src != nullptr && dst != nullptr, JFX_ERR_INVALID, "null buffer");
+  if (mode >= 2) {
+    // unroll is the run length in keys; blocks is unused (one tile each)
+    const uint64_t rec = mode == 2 ? 12u : 8u;
+    JFX_CHECK(bytes % (rec * jfx::SC_TILE) == 0 && bytes / rec / jfx::SC_TILE <= (1u << 20),
+              JFX_ERR_INVALID, "scatter copy: bytes must be a multiple of 16384 records, "
+              "at most 2^20 tiles");
+    JFX_CHECK(unroll >= 1 && unroll <= jfx::SC_TILE && jfx::SC_TILE % unroll == 0,
+              JFX_ERR_INVALID, "scatter copy: run length must divide 16384");
+    JFX_CHECK((uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0, JFX_ERR_INVALID,
+              "buffers must be 16-byte aligned");
+    const uint64_t n = bytes / rec;
+    hipStream_t s = jfx::as_stream(stream);
+    return nontemporal ? jfx::scatter_launch<true>(mode == 2, src, dst, n, unroll, s)
+                       : jfx::scatter_launch<false>(mode == 2, src, dst, n, unroll, s);
+  }
   JFX_CHECK(bytes % 16 == 0 && (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0,
             JFX_ERR_INVALID, "buffers and size must be 16-byte aligned");
   JFX_CHECK(blocks >= 1 && blocks <= (1u << 20), JFX_ERR_INVALID, "blocks must be 1..2^20");

@@ -122,19 +122,66 @@ def pack_columns(cols: list) -> torch.Tensor:
     return out
 
 
+# Integer key types the range pre-filter takes, and the smallest side it runs
+# on (below it the count's host read costs more than the sort it can save).
+_PREFILTER_TYPES = (torch.int32, torch.int64, torch.uint32, torch.uint64)
+_PREFILTER_MIN_ROWS = 1 << 16
+
+
+def _prefilter(k, cols: list, other_sorted, want_rows: bool):
+    """Rows of k whose key lies within [other_sorted[0], other_sorted[-1]]
+    (jfx_join_prefilter): returns (keys, record or None, rows or None), in row
+    order; the record packs cols as pack_columns does."""
+    dev = k.device
+    n = k.shape[0]
+    width = sum(c.element_size() for c in cols)
+    out_k = torch.empty_like(k)
+    rec = (torch.empty(n, dtype=torch.int32 if width == 4 else torch.int64, device=dev)
+           if cols else None)
+    rows = torch.empty(n, dtype=torch.int32, device=dev) if want_rows else None
+    ptrs = (ctypes.c_void_p * max(1, len(cols)))(*[c.data_ptr() for c in cols])
+    nbytes = (ctypes.c_int * max(1, len(cols)))(*[c.element_size() for c in cols])
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_compact_workspace_size", n, ctypes.byref(nb))
+    ws = _lib.workspace(nb.value, dev)
+    kept = ctypes.c_uint64(0)
+    _lib.call("jfx_join_prefilter", key_code(k), _lib.ptr(k), n, _lib.ptr(other_sorted),
+              other_sorted.shape[0], len(cols), ptrs, nbytes, _lib.ptr(out_k), _lib.ptr(rec),
+              _lib.ptr(rows), ctypes.byref(kept), _lib.ptr(ws), ws.numel(), _stream(dev))
+    m = kept.value
+    return out_k[:m], (rec[:m] if rec is not None else None), (rows[:m] if rows is not None else None)
+
+
 class _JoinSide:
     """One side of the inner join, sorted: its payload either rides through
     the radix sort packed into one record per row (at most 8 bytes of
     columns: read in key order by the fill) or is gathered at the source row
-    the sort carried (wider payloads)."""
+    the sort carried (wider payloads). With other_sorted (the other side's
+    sorted keys, integer keys) the rows outside that side's key range are
+    dropped first (_prefilter): they have no pair, and the sort then covers
+    fewer rows, whose keys often vary in fewer digits."""
 
-    def __init__(self, keys, cols: list):
+    def __init__(self, keys, cols: list, other_sorted=None):
         self.k, self.orig = widen_key(to_device(keys))
         dev = self.k.device
         self.cols = [payload_source(c, dev) for c in cols]
         width = sum(c.element_size() for c in self.cols)
         self.record_bytes = width if self.cols and width <= 8 else 0
         self.perm = self.records = None
+        k = self.k
+        if other_sorted is not None:
+            k, rec, rows = _prefilter(self.k, self.cols if self.record_bytes else [],
+                                      other_sorted, bool(self.cols) and not self.record_bytes)
+            self.n = k.shape[0]
+            if not self.cols:
+                self.sorted, _ = radix_sort(k)
+            elif self.record_bytes:
+                self.sorted, self.records = radix_sort(k, rec)
+            else:
+                # the source rows ride through the sort as its payload
+                self.sorted, self.perm = radix_sort(k, rows)
+            return
+        self.n = k.shape[0]
         if not self.cols:
             self.sorted, _ = radix_sort(self.k)
         elif self.record_bytes:
@@ -168,12 +215,24 @@ def _inner_join_fused(left_keys, left_values: dict, right_keys, right_values: di
     lcols, rcols = list(left_values.items()), list(right_values.items())
     if len(lcols) > _lib.JOIN_MAXCOLS or len(rcols) > _lib.JOIN_MAXCOLS:
         return _inner_join_by_rows(left_keys, left_values, right_keys, right_values)
-    left = _JoinSide(left_keys, [v for _, v in lcols])
-    right = _JoinSide(right_keys, [v for _, v in rcols])
-    if left.k.dtype != right.k.dtype:
-        raise TypeError(f"join keys have different dtypes: {left.k.dtype} vs {right.k.dtype}")
+    lk, _ = widen_key(to_device(left_keys))
+    rk, _ = widen_key(to_device(right_keys))
+    if lk.dtype != rk.dtype:
+        raise TypeError(f"join keys have different dtypes: {lk.dtype} vs {rk.dtype}")
+    lvals, rvals = [v for _, v in lcols], [v for _, v in rcols]
+    if lk.dtype in _PREFILTER_TYPES and max(lk.shape[0], rk.shape[0]) >= _PREFILTER_MIN_ROWS:
+        # the smaller side is sorted first; its key range filters the larger
+        if lk.shape[0] >= rk.shape[0]:
+            right = _JoinSide(right_keys, rvals)
+            left = _JoinSide(left_keys, lvals, other_sorted=right.sorted)
+        else:
+            left = _JoinSide(left_keys, lvals)
+            right = _JoinSide(right_keys, rvals, other_sorted=left.sorted)
+    else:
+        left = _JoinSide(left_keys, lvals)
+        right = _JoinSide(right_keys, rvals)
     dev = left.k.device
-    L, R = left.k.shape[0], right.k.shape[0]
+    L, R = left.n, right.n
     kc = key_code(left.k)
     nb = ctypes.c_uint64(0)
     _lib.call("jfx_join_workspace_size", L, ctypes.byref(nb))

@@ -1,0 +1,185 @@
+"""The inner join's range pre-filter (jfx_join_prefilter): the kept rows,
+their packed records and row indices equal numpy's on the same inputs, and
+inner joins that take the filter (integer keys, a side of at least 2^16 rows)
+give the oracle's pairs and payloads bit for bit, for rows below, at and above
+the other side's key range, on either side, with every payload layout."""
+
+import ctypes
+import zlib
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from jaxframes_amd import _lib, joining
+from tests._util import cuda
+
+pytestmark = pytest.mark.gpu
+
+KT = {np.int32: _lib.JFX_I32, np.int64: _lib.JFX_I64, np.uint32: _lib.JFX_U32,
+      np.uint64: _lib.JFX_U64}
+
+
+_SIGNED = {torch.uint32: torch.int32, torch.uint64: torch.int64}
+
+
+def _np(t, dt):
+    """A device tensor's values as a numpy array of dt (unsigned types through
+    their signed view)."""
+    if t.dtype in _SIGNED:
+        t = t.view(_SIGNED[t.dtype])
+    return t.cpu().numpy().view(dt)
+
+
+def _prefilter(keys, other_sorted, cols, want_rows):
+    n = keys.size
+    k = cuda(keys)
+    o = cuda(other_sorted)
+    dcols = [cuda(c) for c in cols]
+    width = sum(c.dtype.itemsize for c in cols)
+    out_k = torch.empty_like(k)
+    rec = torch.empty(n, dtype=torch.int32 if width == 4 else torch.int64, device="cuda") \
+        if cols else None
+    rows = torch.empty(n, dtype=torch.int32, device="cuda") if want_rows else None
+    ptrs = (ctypes.c_void_p * max(1, len(cols)))(*[c.data_ptr() for c in dcols])
+    nbytes = (ctypes.c_int * max(1, len(cols)))(*[c.dtype.itemsize for c in cols])
+    nb = ctypes.c_uint64(0)
+    _lib.call("jfx_compact_workspace_size", n, ctypes.byref(nb))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
+    kept = ctypes.c_uint64(0)
+    _lib.call("jfx_join_prefilter", KT[keys.dtype.type], _lib.ptr(k), n, _lib.ptr(o),
+              other_sorted.size, len(cols), ptrs, nbytes, _lib.ptr(out_k), _lib.ptr(rec),
+              _lib.ptr(rows), ctypes.byref(kept), _lib.ptr(ws), ws.numel(),
+              ctypes.c_void_p(_lib.stream_handle(torch.device("cuda", 0))))
+    m = kept.value
+    got_k = _np(out_k[:m], keys.dtype)
+    got_r = rec[:m].cpu().numpy() if rec is not None else None
+    got_rows = rows[:m].cpu().numpy().view(np.uint32) if rows is not None else None
+    return got_k, got_r, got_rows
+
+
+def _record(cols, idx):
+    """pack_columns' layout: column j at the byte offset of the columns before it."""
+    width = sum(c.dtype.itemsize for c in cols)
+    buf = np.zeros((idx.size, width), dtype=np.uint8)
+    off = 0
+    for c in cols:
+        b = c.dtype.itemsize
+        buf[:, off:off + b] = c[idx].view(np.uint8).reshape(-1, b)
+        off += b
+    return buf.view(np.int32 if width == 4 else np.int64).ravel()
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.uint32, np.uint64])
+@pytest.mark.parametrize("layout", ["none", "2x4", "1x8", "1x4", "rows"])
+def test_prefilter_keeps_rows_in_range(dt, layout):
+    rng = np.random.default_rng(zlib.crc32(f"{dt.__name__}/{layout}".encode()))
+    n = 300_007
+    info = np.iinfo(dt)
+    keys = rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+    mid = rng.integers(info.min // 4 if info.min else info.max // 4, info.max // 2, 2000, dtype=dt)
+    other = np.sort(mid)
+    lo, hi = other[0], other[-1]
+    # rows exactly at both ends of the range, and a share inside it
+    keys[::97] = lo
+    keys[1::89] = hi
+    inside = rng.integers(0, 2000, n // 3)
+    keys[2::3][: inside.size] = mid[inside][: keys[2::3].size]
+    cols = {"none": [], "2x4": [rng.standard_normal(n).astype(np.float32),
+                                rng.integers(-9, 9, n).astype(np.int32)],
+            "1x8": [rng.standard_normal(n)], "1x4": [rng.standard_normal(n).astype(np.float32)],
+            "rows": []}[layout]
+    got_k, got_r, got_rows = _prefilter(keys, other, cols, layout == "rows")
+    idx = np.flatnonzero((keys >= lo) & (keys <= hi))
+    assert np.array_equal(got_k, keys[idx])
+    if cols:
+        assert np.array_equal(got_r, _record(cols, idx))
+    if layout == "rows":
+        assert np.array_equal(got_rows.astype(np.int64), idx)
+
+
+def test_prefilter_edges():
+    keys = np.arange(100_000, dtype=np.int64)
+    # no other rows: nothing can match
+    k, _, _ = _prefilter(keys, np.zeros(0, dtype=np.int64), [], False)
+    assert k.size == 0
+    # a one-key range
+    k, _, rows = _prefilter(keys, np.array([777], dtype=np.int64), [], True)
+    assert k.tolist() == [777] and rows.tolist() == [777]
+    # everything kept
+    k, _, _ = _prefilter(keys, np.array([-5, 10**6], dtype=np.int64), [], False)
+    assert np.array_equal(k, keys)
+    # float keys are not taken
+    with pytest.raises(_lib.JfxError):
+        _lib.call("jfx_join_prefilter", _lib.JFX_F32, None, 1, None, 1, 0, None, None, None,
+                  None, None, ctypes.byref(ctypes.c_uint64(0)), None, 0, None)
+
+
+PAYLOAD_LAYOUTS = [
+    ([], []),
+    ([np.float32, np.float32], [np.float32, np.int32]),
+    ([np.float64], [np.int64]),
+    ([np.float32, np.float32, np.float32], [np.float64, np.float32]),  # by row
+    ([np.float32] * 9, [np.int16]),                                   # beyond 8 columns
+]
+
+
+def _col(rng, dt, n):
+    return (rng.standard_normal(n) * 100).astype(dt) if np.dtype(dt).kind == "f" else \
+        rng.integers(-1000, 1000, n).astype(dt)
+
+
+def _check_join(lk, rk, lcols, rcols, rng):
+    lv = {f"l{j}": _col(rng, dt, lk.size) for j, dt in enumerate(lcols)}
+    rv = {f"r{j}": _col(rng, dt, rk.size) for j, dt in enumerate(rcols)}
+    ek, eout, _, _ = oracle.sort_merge_join(lk, lv, rk, rv, "inner")
+    k, out = joining.sort_merge_join(cuda(lk), {c: cuda(v) for c, v in lv.items()},
+                                     cuda(rk), {c: cuda(v) for c, v in rv.items()}, "inner")
+    got = _np(k, lk.dtype)
+    assert np.array_equal(got.view(np.uint8), ek.astype(lk.dtype).view(np.uint8))
+    assert sorted(out) == sorted(eout)
+    for c, e in eout.items():
+        g = out[c].cpu().numpy()
+        assert g.dtype == e.dtype, (c, g.dtype, e.dtype)
+        assert np.array_equal(g, e), c
+    return ek.size
+
+
+@pytest.mark.parametrize("lcols,rcols", PAYLOAD_LAYOUTS, ids=["none", "2+2", "8B", "rows", "9cols"])
+@pytest.mark.parametrize("left_larger", [True, False])
+def test_join_with_prefilter_matches_oracle(lcols, rcols, left_larger):
+    rng = np.random.default_rng(len(lcols) * 7 + len(rcols) + left_larger)
+    big, small = 200_000, 40_000
+    sk = rng.integers(10**6, 2 * 10**6, small).astype(np.int64)
+    lo, hi = sk.min(), sk.max()
+    bk = np.concatenate([sk[rng.integers(0, small, big // 2)],                 # matching
+                         rng.integers(-10**9, lo, big // 4),                    # below the range
+                         rng.integers(hi + 1, 10**9, big // 8),                 # above it
+                         np.full(big // 16, lo), np.full(big // 16, hi),        # at its ends
+                         rng.integers(lo, hi + 1, big - big // 2 - big // 4 - big // 8
+                                      - 2 * (big // 16))])                     # inside, unmatched or not
+    rng.shuffle(bk)
+    if left_larger:
+        m = _check_join(bk, sk, lcols, rcols, rng)
+    else:
+        m = _check_join(sk, bk, lcols, rcols, rng)
+    assert m > 0
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.uint32, np.uint64])
+def test_join_with_prefilter_key_types(dt):
+    rng = np.random.default_rng(5)
+    info = np.iinfo(dt)
+    rk = rng.integers(info.max // 3, info.max // 3 + 50_000, 30_000, dtype=dt)
+    lk = np.concatenate([rk[rng.integers(0, rk.size, 60_000)],
+                         rng.integers(info.min, info.max, 60_000, dtype=dt, endpoint=True)])
+    rng.shuffle(lk)
+    _check_join(lk, rk, [np.float32], [np.float32], rng)
+
+
+def test_join_with_prefilter_no_overlap():
+    rng = np.random.default_rng(8)
+    lk = rng.integers(0, 10**6, 100_000).astype(np.int64)
+    rk = rng.integers(2 * 10**6, 3 * 10**6, 70_000).astype(np.int64)
+    assert _check_join(lk, rk, [np.float32, np.float32], [np.float32], rng) == 0
