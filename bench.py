@@ -540,10 +540,8 @@ def workload_join(reps: int) -> dict:
         # packing); the left side's are packed by the pre-filter's scatter
         if name == "pack_cols":
             return R * (8 + 8)
-        if name == "pf_count":
-            return L * 8
-        if name == "pf_scatter":
-            # every key read; the kept rows' payloads read, key and record written
+        if name == "pf_onepass":
+            # every key read once; the kept rows' payloads read, key and record written
             return L * 8 + Lk * (8 + 8 + 8)
         if name.startswith("sort_stats") or name == "sort_hist":
             return (Lk + R) * 8

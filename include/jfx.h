@@ -287,7 +287,11 @@ int jfx_pack_columns(int ncols, const void* const* cols, const int* col_bytes, u
  * out_records when ncols > 0, their row indices to out_rows when non-null,
  * and the kept count to *out_count (synchronises the stream once). Integer key
  * types only (JFX_ERR_UNSUPPORTED for floats); outputs hold n rows;
- * workspace of jfx_compact_workspace_size bytes. */
+ * workspace of jfx_join_prefilter_workspace_size(n) bytes. Below 2^30 rows
+ * one pass (a decoupled lookback over the tiles' kept counts) reads the keys
+ * once; above it (or with JFX_PREFILTER=chunked) chunk counts, their scan and
+ * a scatter read them twice. */
+int jfx_join_prefilter_workspace_size(uint64_t n, uint64_t* out_bytes);
 int jfx_join_prefilter(int key_type, const void* keys, uint64_t n, const void* sorted_other,
                        uint64_t n_other, int ncols, const void* const* cols, const int* col_bytes,
                        void* out_keys, void* out_records, uint32_t* out_rows, uint64_t* out_count,

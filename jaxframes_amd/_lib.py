@@ -100,6 +100,7 @@ _SIGNATURES = {
     "jfx_join_fill_columns": [c_int, c_vp, c_u64, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u64, c_vp],
     "jfx_pack_columns": [c_int, ctypes.POINTER(c_vp), ctypes.POINTER(c_int), c_u64, c_vp,
                          ctypes.POINTER(c_int), c_vp],
+    "jfx_join_prefilter_workspace_size": [c_u64, c_u64p],
     "jfx_join_prefilter": [c_int, c_vp, c_u64, c_vp, c_u64, c_int, ctypes.POINTER(c_vp),
                            ctypes.POINTER(c_int), c_vp, c_vp, c_vp, ctypes.POINTER(c_u64), c_vp,
                            c_u64, c_vp],

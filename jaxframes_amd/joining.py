@@ -142,7 +142,7 @@ def _prefilter(k, cols: list, other_sorted, want_rows: bool):
     ptrs = (ctypes.c_void_p * max(1, len(cols)))(*[c.data_ptr() for c in cols])
     nbytes = (ctypes.c_int * max(1, len(cols)))(*[c.element_size() for c in cols])
     nb = ctypes.c_uint64(0)
-    _lib.call("jfx_compact_workspace_size", n, ctypes.byref(nb))
+    _lib.call("jfx_join_prefilter_workspace_size", n, ctypes.byref(nb))
     ws = _lib.workspace(nb.value, dev)
     kept = ctypes.c_uint64(0)
     _lib.call("jfx_join_prefilter", key_code(k), _lib.ptr(k), n, _lib.ptr(other_sorted),

@@ -45,7 +45,7 @@ def _prefilter(keys, other_sorted, cols, want_rows):
     ptrs = (ctypes.c_void_p * max(1, len(cols)))(*[c.data_ptr() for c in dcols])
     nbytes = (ctypes.c_int * max(1, len(cols)))(*[c.dtype.itemsize for c in cols])
     nb = ctypes.c_uint64(0)
-    _lib.call("jfx_compact_workspace_size", n, ctypes.byref(nb))
+    _lib.call("jfx_join_prefilter_workspace_size", n, ctypes.byref(nb))
     ws = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
     kept = ctypes.c_uint64(0)
     _lib.call("jfx_join_prefilter", KT[keys.dtype.type], _lib.ptr(k), n, _lib.ptr(o),
@@ -71,9 +71,20 @@ def _record(cols, idx):
     return buf.view(np.int32 if width == 4 else np.int64).ravel()
 
 
+@pytest.fixture(params=["onepass", "chunked"])
+def form(request, monkeypatch):
+    """Both forms of the pre-filter: the one-pass lookback kernel (below 2^30
+    rows) and the chunk-count / scan / scatter kernels (above it)."""
+    if request.param == "chunked":
+        monkeypatch.setenv("JFX_PREFILTER", "chunked")
+    else:
+        monkeypatch.delenv("JFX_PREFILTER", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("dt", [np.int32, np.int64, np.uint32, np.uint64])
 @pytest.mark.parametrize("layout", ["none", "2x4", "1x8", "1x4", "rows"])
-def test_prefilter_keeps_rows_in_range(dt, layout):
+def test_prefilter_keeps_rows_in_range(dt, layout, form):
     rng = np.random.default_rng(zlib.crc32(f"{dt.__name__}/{layout}".encode()))
     n = 300_007
     info = np.iinfo(dt)
@@ -99,7 +110,7 @@ def test_prefilter_keeps_rows_in_range(dt, layout):
         assert np.array_equal(got_rows.astype(np.int64), idx)
 
 
-def test_prefilter_edges():
+def test_prefilter_edges(form):
     keys = np.arange(100_000, dtype=np.int64)
     # no other rows: nothing can match
     k, _, _ = _prefilter(keys, np.zeros(0, dtype=np.int64), [], False)
