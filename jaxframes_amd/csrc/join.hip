@@ -778,12 +778,11 @@ k_pf_scatter(const K* keys, uint64_t n, uint64_t chunk, const K* other, uint64_t
 // tile counts its kept rows, takes the kept rows of the tiles before it by a
 // decoupled lookback over one counter per tile, and writes its kept rows
 // there. The keys are read once (the three-kernel form reads them twice).
-// The last tile writes the kept total. Tiles of 16384 rows (16 waves): every
-// tile takes its id with an atomic on one counter, and those serialise at
-// the coherence point, so 4096-row tiles (24415 atomics at 1e8 rows) took 779
-// us, 8192-row tiles 779 us and 16384-row tiles 686 us. (Loading the kept
-// rows' payloads before the lookback instead of after it measured 816
-// against 779 us with 4096-row tiles.)
+// The last tile writes the kept total. Tiles of 16384 rows (16 waves): at
+// 1e8 int64 keys 4096-row tiles took 779 us, 8192-row tiles 779 us and
+// 16384-row tiles 686 us. (Loading the kept rows' payloads before the
+// lookback instead of after it measured 816 against 779 us with 4096-row
+// tiles.)
 constexpr int PF1_BLOCK = 1024;
 constexpr int PF1_W = PF1_BLOCK / 64;
 constexpr uint32_t PF1_TILE = PF1_BLOCK * PF_IPT;
