@@ -194,3 +194,12 @@ def test_join_with_prefilter_no_overlap():
     lk = rng.integers(0, 10**6, 100_000).astype(np.int64)
     rk = rng.integers(2 * 10**6, 3 * 10**6, 70_000).astype(np.int64)
     assert _check_join(lk, rk, [np.float32, np.float32], [np.float32], rng) == 0
+
+
+def test_join_with_prefilter_empty_side():
+    """An empty smaller side leaves nothing for the larger side to match."""
+    rng = np.random.default_rng(12)
+    lk = rng.integers(0, 10**6, 100_000).astype(np.int64)
+    rk = np.zeros(0, dtype=np.int64)
+    assert _check_join(lk, rk, [np.float32, np.float32], [np.float32], rng) == 0
+    assert _check_join(rk, lk, [np.float32], [np.float32, np.float32], rng) == 0
