@@ -631,8 +631,8 @@ def only_workload(name: str, reps: int, batched_rows: int = 400_000_000) -> dict
 
 
 def bandwidth_probe(nbytes: int = 800_000_000, reps: int = 5) -> dict:
-    """HIP-event time of jfx_bandwidth_probe's read-only and copy kernels
-    over nbytes, for every grid / loads-in-flight / non-temporal setting,
+    """HIP-event time of jfx_bandwidth_probe's read-only, copy and write-only
+    kernels over nbytes, for every grid / loads-in-flight / non-temporal setting,
     and of its scatter copy (a radix pass's writes without the ranking) over
     run lengths; returns the best of each mode (GB/s counts each byte read or
     written once) and the whole sweep."""
@@ -644,7 +644,7 @@ def bandwidth_probe(nbytes: int = 800_000_000, reps: int = 5) -> dict:
     dst = torch.empty_like(src)
     stream = ctypes.c_void_p(_lib.stream_handle(dev))
     sweep = []
-    for mode in (0, 1):
+    for mode in (0, 1, 4):
         for blocks in (1024, 2048, 4096, 8192, 16384):
             for unroll in (1, 2, 4, 8):
                 for nt in (0, 1):
@@ -656,8 +656,8 @@ def bandwidth_probe(nbytes: int = 800_000_000, reps: int = 5) -> dict:
                         torch.cuda.synchronize()
                     ms, cnt = next(iter(kt.stats().values()))
                     us = ms / cnt * 1e3
-                    moved = nbytes * (1 if mode == 0 else 2)
-                    sweep.append({"mode": "read" if mode == 0 else "copy", "blocks": blocks,
+                    moved = nbytes * (2 if mode == 1 else 1)
+                    sweep.append({"mode": {0: "read", 1: "copy", 4: "write"}[mode], "blocks": blocks,
                                   "unroll": unroll, "nontemporal": nt, "us": round(us, 1),
                                   "GBps": round(moved / (us * 1e-6) / 1e9, 1)})
     # a radix pass's scattered writes without its ranking: 100M uint64 keys
@@ -686,7 +686,7 @@ def bandwidth_probe(nbytes: int = 800_000_000, reps: int = 5) -> dict:
                               "us": round(us, 1),
                               "GBps": round(2 * moved / (us * 1e-6) / 1e9, 1)})
     best = {}
-    for m in ("read", "copy", "scatter_pay", "scatter_keys"):
+    for m in ("read", "copy", "write", "scatter_pay", "scatter_keys"):
         b = max((e for e in sweep if e["mode"] == m), key=lambda e: e["GBps"])
         best[m] = {**b, "frac_of_peak": round(b["GBps"] / PEAK_HBM_GBPS, 4)}
     for m in ("scatter_pay", "scatter_keys"):
@@ -718,9 +718,11 @@ def main() -> int:
                                        "served"),
                     help="run one other workload by itself (for profiling) and print its line")
     ap.add_argument("--bandwidth-probe", action="store_true",
-                    help="only sweep the hand-written read-only and copy kernels over 800 MB "
-                         "(grid, loads in flight, non-temporal loads) and print the best "
-                         "achieved GB/s of each: the ceilings the kernels are read against")
+                    help="only sweep the hand-written read-only, copy and write-only kernels "
+                         "over 800 MB (grid, loads in flight, non-temporal accesses) and the "
+                         "scatter copy (a radix pass's writes without its ranking) and print "
+                         "the best achieved GB/s of each: the ceilings the kernels are read "
+                         "against")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 400 MB (float32[1e8]) so that PMC byte "
                          "counters can be scaled against a transfer of known size")

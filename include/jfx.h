@@ -525,7 +525,9 @@ int jfx_filter_rows(int col_type, const void* col, int cmp, const void* value, u
  * each 16384-key tile is read in order and written as runs of `unroll` keys
  * (a divisor of 16384), run d of tile t to [d*T*unroll + t*unroll, +unroll)
  * of the T = n / 16384 tiles; `blocks` is unused and nontemporal selects
- * non-temporal stores. Asynchronous on `stream`. */
+ * non-temporal stores. Mode 4 writes `bytes` of dst with 16-byte stores and
+ * no loads (the walk of mode 0; nontemporal selects non-temporal stores).
+ * Asynchronous on `stream`. */
 int jfx_bandwidth_probe(int mode, const void* src, void* dst, uint64_t bytes, uint32_t blocks,
                         uint32_t unroll, int nontemporal, void* stream);
 

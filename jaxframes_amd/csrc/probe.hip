@@ -5,6 +5,8 @@
 //                 16-byte loads in flight per step, and folds them into an
 //                 XOR that each workgroup stores once (so no load is dead);
 // k_stream_copy   the same walk storing what it loads.
+// k_stream_write  the same walk storing a value derived from the index (no
+//                 loads).
 // k_scatter_copy  a radix pass's memory pattern without its ranking: tiles
 //                 of 16384 uint64 keys (and uint32 payloads) read in order
 //                 and written as 16384 / runlen runs per tile, run d of tile
@@ -123,8 +125,32 @@ int scatter_launch(bool pay, const void* src, void* dst, uint64_t n, uint32_t ru
 }
 
 template <int U, bool NT>
+__global__ void __launch_bounds__(256) k_stream_write(v4u* dst, uint64_t n16) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < n16; i += U * stride) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t x = (uint32_t)(i + k * stride);
+      const v4u v = {x, x + 1u, x + 2u, x + 3u};
+      if constexpr (NT) __builtin_nontemporal_store(v, dst + i + k * stride);
+      else dst[i + k * stride] = v;
+    }
+  }
+  for (; i < n16; i += stride) {
+    const uint32_t x = (uint32_t)i;
+    const v4u v = {x, x + 1u, x + 2u, x + 3u};
+    if constexpr (NT) __builtin_nontemporal_store(v, dst + i);
+    else dst[i] = v;
+  }
+}
+
+template <int U, bool NT>
 int probe_launch(int mode, const void* src, void* dst, uint64_t n16, uint32_t blocks, hipStream_t s) {
-  if (mode == 0) {
+  if (mode == 4) {
+    JFX_LAUNCH("stream_write", s, (k_stream_write<U, NT><<<blocks, 256, 0, s>>>(
+                                       static_cast<v4u*>(dst), n16)));
+  } else if (mode == 0) {
     JFX_LAUNCH("stream_read", s, (k_stream_read<U, NT><<<blocks, 256, 0, s>>>(
                                       static_cast<const v4u*>(src), n16, static_cast<uint32_t*>(dst))));
   } else {
@@ -152,11 +178,11 @@ int probe_unroll(int mode, const void* src, void* dst, uint64_t n16, uint32_t bl
 
 extern "C" int jfx_bandwidth_probe(int mode, const void* src, void* dst, uint64_t bytes,
                                    uint32_t blocks, uint32_t unroll, int nontemporal, void* stream) {
-  JFX_CHECK(mode >= 0 && mode <= 3, JFX_ERR_INVALID,
-            "mode must be 0 (read), 1 (copy), 2 (scatter copy of keys and payloads) or 3 "
-            "(scatter copy of keys)");
+  JFX_CHECK(mode >= 0 && mode <= 4, JFX_ERR_INVALID,
+            "mode must be 0 (read), 1 (copy), 2 (scatter copy of keys and payloads), 3 "
+            "(scatter copy of keys) or 4 (write)");
   JFX_CHECK(src != nullptr && dst != nullptr, JFX_ERR_INVALID, "null buffer");
-  if (mode >= 2) {
+  if (mode == 2 || mode == 3) {
     // unroll is the run length in keys; blocks is unused (one tile each)
     const uint64_t rec = mode == 2 ? 12u : 8u;
     JFX_CHECK(bytes % (rec * jfx::SC_TILE) == 0 && bytes / rec / jfx::SC_TILE <= (1u << 20),

@@ -31,6 +31,18 @@ def test_copy_probe_is_exact():
         assert torch.equal(src, dst)
 
 
+def test_write_probe_values():
+    n16 = 1 << 16
+    dst = torch.zeros(n16 * 4, dtype=torch.int32, device="cuda")
+    for nt in (0, 1):
+        dst.zero_()
+        _lib.call("jfx_bandwidth_probe", 4, dst.data_ptr(), dst.data_ptr(), n16 * 16, 64, 4, nt,
+                  _stream())
+        torch.cuda.synchronize()
+        i = np.arange(n16, dtype=np.int64)[:, None] + np.arange(4)[None, :]
+        assert np.array_equal(dst.cpu().numpy().reshape(-1, 4).astype(np.int64), i)
+
+
 @pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("runlen", [16, 64, 256])
 def test_scatter_probe_positions(mode, runlen):
