@@ -858,7 +858,7 @@ template <typename K, typename V, typename MT, bool MM, uint32_t TE>
 int onepass_run(int oc, const void* keys, const void* vals, uint64_t n, const SegWs& w,
                 void* out_keys, const SegCall& sc, uint64_t capacity, uint32_t rt_min,
                 uint32_t* res, hipStream_t s) {
-  constexpr uint64_t STEP = 64u * WT_RPL;
+  constexpr uint64_t STEP = 64u * OP_RPL;
   // LDS per block: a sum table per wave, one count table and (MM) one min and
   // one max table
   constexpr uint64_t BLK = (uint64_t)OP_W * TE * sizeof(typename OpT<V>::T) * (op_dd<V> ? 2u : 1u) +

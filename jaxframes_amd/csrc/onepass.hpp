@@ -38,6 +38,7 @@
 namespace jfx {
 
 constexpr uint32_t OP_TE = 1024;  // table entries (residues) per wave; 2048 for the wider retry
+constexpr int OP_RPL = 16;        // rows per lane per step of k_onepass_table
 constexpr int OP_W = 4;           // waves per block
 constexpr uint32_t OP_NO_RUN = 0xFFFFFFFFu;
 
@@ -255,11 +256,11 @@ typedef uint32_t op_v2u __attribute__((ext_vector_type(2)));
 // instruction touches 64 lines in part). Runs of equal keys then fold in
 // registers over E consecutive rows instead of 16: keys in runs of 64 rows
 // measured 186 us against 152 us (16 lanes flush one residue together).
-template <int E, typename T>
-__device__ __forceinline__ void op_load_coal(const T* p, uint32_t lane, T (&o)[WT_RPL]) {
+template <int E, typename T, int N>
+__device__ __forceinline__ void op_load_coal(const T* p, uint32_t lane, T (&o)[N]) {
   constexpr int B = E * sizeof(T);
 #pragma unroll
-  for (int c = 0; c < WT_RPL / E; ++c) {
+  for (int c = 0; c < N / E; ++c) {
     const T* src = p + (uint64_t)c * 64 * E + (uint64_t)lane * E;
     T t[E];
     if constexpr (B == 16) {
@@ -293,7 +294,7 @@ __global__ void __launch_bounds__(W * 64) k_onepass_table(OnePassArgs<K, V, MM> 
   using C = MmCode<V>;
   constexpr bool DD = op_dd<V>;
   static_assert(!NC || (std::is_same<V, float>::value && !MM), "count-free tables: float32 sums only");
-  constexpr uint64_t STEP = 64u * WT_RPL;
+  constexpr uint64_t STEP = 64u * OP_RPL;
   __shared__ T s_acc[W * TE];
   // float64 values: wave-private compensation of each sum entry
   __shared__ T s_comp[DD ? W * TE : 1];
@@ -426,19 +427,18 @@ __global__ void __launch_bounds__(W * 64) k_onepass_table(OnePassArgs<K, V, MM> 
 
   uint64_t base = wave * STEP;
   if (!stop && VEC && wave < nsteps) {
-    // the loads are unconditional (clamped to the last whole step) so that
-    // the in-order wait counters let the next step stay in flight
+    // the loads are unconditional (clamped to the last whole step)
     const uint64_t last = base + ((nsteps - 1 - wave) / nwaves) * SW;
-    K ka[WT_RPL], kb[WT_RPL];
-    V va[WT_RPL], vb[WT_RPL];
-    auto load = [&](uint64_t b, K (&kk)[WT_RPL], V (&vv)[WT_RPL]) {
+    K ka[OP_RPL];
+    V va[OP_RPL];
+    auto load = [&](uint64_t b, K (&kk)[OP_RPL], V (&vv)[OP_RPL]) {
       const uint64_t bc = (b < last ? b : last);
       op_load_coal<OP_E<K, V>>(a.keys + bc, lane, kk);
       op_load_coal<OP_E<K, V>>(a.vals + bc, lane, vv);
     };
-    auto fold = [&](K (&kk)[WT_RPL], V (&vv)[WT_RPL]) {
+    auto fold = [&](K (&kk)[OP_RPL], V (&vv)[OP_RPL]) {
 #pragma unroll
-      for (int i = 0; i < WT_RPL; ++i) row(kk[i], vv[i]);
+      for (int i = 0; i < OP_RPL; ++i) row(kk[i], vv[i]);
     };
     bool first = true;
     // after a step: the first step's own span, then the stop word
@@ -452,20 +452,20 @@ __global__ void __launch_bounds__(W * 64) k_onepass_table(OnePassArgs<K, V, MM> 
       }
       return stopped();
     };
+    // One step of loads in flight per wave (8 KB of 4-byte keys and values):
+    // at 100M rows the 4096 waves then keep 32 MB in flight, and the table
+    // kernel took 128.1-128.9 us, against 134.1-134.7 us with the next step
+    // loaded while folding the current one (16 KB per wave), 132.5 us for
+    // single 512-row steps and 133.5 us for single 2048-row steps. The
+    // hand-written read stream of the bandwidth probe likewise runs fastest
+    // with one 16-byte load in flight per thread.
     load(base, ka, va);
-    __builtin_amdgcn_sched_barrier(0);
-    load(base + SW, kb, vb);
     __builtin_amdgcn_sched_barrier(0);
     while (true) {
       fold(ka, va);
       base += SW;
       if (base > last || (stop = check())) break;
-      load(base + SW, ka, va);
-      __builtin_amdgcn_sched_barrier(0);
-      fold(kb, vb);
-      base += SW;
-      if (base > last || (stop = check())) break;
-      load(base + SW, kb, vb);
+      load(base, ka, va);
       __builtin_amdgcn_sched_barrier(0);
     }
   }

@@ -2,9 +2,9 @@
 that small inputs never reach:
 
 - the one-pass groupby at 16M-100M rows, where every wave runs several
-  1024-row steps (double-buffered loads, the first-step span check and the
-  stop-word polling), including out-of-range keys placed after a wave's
-  first step, in its first step and in the last wave's last step;
+  1024-row steps (the step loop, the first-step span check and the stop-word
+  polling), including out-of-range keys placed after a wave's first step, in
+  its first step and in the last wave's last step;
 - the sort-based groupby and the batched groupby beyond 1024 tiles of 8192
   rows, where the segmented reduction's fix-up carries groups from one chunk
   of 1024 tile partials to the next;
@@ -101,18 +101,27 @@ def test_headline_groupby_stats_100m_min_max():
     assert np.array_equal(_host(res["max"]), df["max"].to_numpy())
 
 
-N16 = 16 * 1024 * 1024   # 4096 waves of 4096 rows: 4 one-pass steps per wave
-WAVE = 4096
+N16 = 16 * 1024 * 1024
+# The one-pass kernel deals 1024-row steps to its waves grid-stride (wave w
+# takes steps w, w + W, w + 2W, ...); with a sum and a count at 16M rows it
+# runs W = 4096 waves, so every wave takes 4 steps and the k-th step of wave w
+# holds rows [(k * W + w) * STEP, +STEP).
+STEP = 1024
+NWAVES = 4096
+
+
+def _row(wave, k, off):
+    return (k * NWAVES + wave) * STEP + off
 
 
 # name, row of the odd key, odd key, whether the one-pass path still hits
 ODD_CASES = [
-    ("after-first-step", 100 * WAVE + 1500, 5000, False),
-    ("in-first-step", 100 * WAVE + 10, 5000, False),
+    ("after-first-step", _row(100, 1, 500), 5000, False),
+    ("in-first-step", _row(100, 0, 10), 5000, False),
     ("last-wave-last-step", N16 - 7, 5000, False),
-    ("span-1500-retry", 2000 * WAVE + 3000, 1500, True),
-    ("negative-after-first-step", 777 * WAVE + 2222, -3000, False),
-    ("negative-span-1003-hit", 777 * WAVE + 2222, -3, True),
+    ("span-1500-retry", _row(2000, 2, 300), 1500, True),
+    ("negative-after-first-step", _row(777, 3, 222), -3000, False),
+    ("negative-span-1003-hit", _row(777, 3, 222), -3, True),
 ]
 
 
