@@ -13,7 +13,9 @@
 //                      are written here; each tile leaves its leading and
 //                      trailing partial.
 //   k_segreduce_fix    one workgroup scans the per-tile partials and writes
-//                      the groups that cross tile boundaries, plus G.
+//                      the groups that cross tile boundaries, plus G; above
+//                      1024 tiles its chunks of 1024 run in parallel
+//                      (k_segfix_chunks, k_segfix_carry, k_segfix_apply).
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -491,6 +493,107 @@ __global__ void __launch_bounds__(1024) k_segreduce_fix(SegArgs<K, V, MT, MM> a)
   }
 }
 
+// Above one chunk of 1024 tiles the fix-up runs its chunks in parallel:
+// k_segfix_chunks gives each chunk's (flag, state) as one element (the
+// workgroup segmented scan of k_segreduce_fix without a carry),
+// k_segfix_carry scans those elements in one workgroup (at most 1024 chunks:
+// 2^30 rows), leaving each chunk the state of the group open at its start
+// and the total after the last chunk, and k_segfix_apply does
+// k_segreduce_fix's per-chunk work from that carry. The order of the state
+// merges is fixed, so results repeat bit for bit from run to run.
+template <typename K, typename V, typename MT, bool MM>
+__device__ __forceinline__ void segfix_tile(const SegArgs<K, V, MT, MM>& a, uint32_t u,
+                                            uint2& info, AggState<V, MM>& lead, uint32_t& h,
+                                            AggState<V, MM>& el) {
+  info = make_uint2(0u, 0u);
+  lead = AggState<V, MM>::identity();
+  el = AggState<V, MM>::identity();
+  h = 0u;
+  if (u < a.num_tiles) {
+    info = a.tile_info[u];
+    lead = a.tile_lead[u];
+    if (info.y > 0u) {
+      h = 1u;
+      el = a.tile_trail[u];
+    } else {
+      el = lead;
+    }
+  }
+}
+
+template <typename K, typename V, typename MT, bool MM>
+__global__ void __launch_bounds__(1024)
+k_segfix_chunks(SegArgs<K, V, MT, MM> a, uint32_t* cflag, AggState<V, MM>* cstate) {
+  __shared__ AggState<V, MM> s_wst[16 + 1];
+  __shared__ uint32_t s_wf[16 + 1];
+  uint2 info;
+  AggState<V, MM> lead, el;
+  uint32_t h;
+  segfix_tile(a, blockIdx.x * 1024u + threadIdx.x, info, lead, h, el);
+  uint32_t ef, tf;
+  AggState<V, MM> est, tst;
+  block_seg_scan<V, MM, 16>(h, el, ef, est, tf, tst, s_wst, s_wf);
+  if (threadIdx.x == 0) {
+    cflag[blockIdx.x] = tf;
+    cstate[blockIdx.x] = tst;
+  }
+}
+
+template <typename V, bool MM>
+__global__ void __launch_bounds__(1024)
+k_segfix_carry(uint32_t nch, uint32_t* cflag, AggState<V, MM>* cstate) {
+  __shared__ AggState<V, MM> s_wst[16 + 1];
+  __shared__ uint32_t s_wf[16 + 1];
+  const uint32_t t = threadIdx.x;
+  uint32_t f = 0u;
+  AggState<V, MM> st = AggState<V, MM>::identity();
+  if (t < nch) {
+    f = cflag[t];
+    st = cstate[t];
+  }
+  uint32_t ef, tf;
+  AggState<V, MM> est, tst;
+  block_seg_scan<V, MM, 16>(f, st, ef, est, tf, tst, s_wst, s_wf);
+  if (t < nch) {  // each thread rewrites only the entry it read
+    cflag[t] = ef;
+    cstate[t] = est;
+  }
+  if (t == 0) {
+    cflag[nch] = tf;
+    cstate[nch] = tst;
+  }
+}
+
+template <typename K, typename V, typename MT, bool MM>
+__global__ void __launch_bounds__(1024)
+k_segfix_apply(SegArgs<K, V, MT, MM> a, const uint32_t* cflag, const AggState<V, MM>* cstate,
+               uint32_t nch) {
+  __shared__ AggState<V, MM> s_wst[16 + 1];
+  __shared__ uint32_t s_wf[16 + 1];
+  const uint32_t c = blockIdx.x;
+  uint2 info;
+  AggState<V, MM> lead, el;
+  uint32_t h;
+  segfix_tile(a, c * 1024u + threadIdx.x, info, lead, h, el);
+  uint32_t ef, tf;
+  AggState<V, MM> est, tst;
+  block_seg_scan<V, MM, 16>(h, el, ef, est, tf, tst, s_wst, s_wf);
+  // state of the group open at the tile's start: the chunk's carry, then est
+  uint32_t pf = cflag[c];
+  AggState<V, MM> pre = cstate[c];
+  seg_combine(pf, pre, ef, est);
+  if (h && info.x > 0u) {
+    pre.merge(lead);
+    finalize(a, info.x - 1u, pre);
+  }
+  if (c == nch - 1u && threadIdx.x == 0) {
+    const uint2 last = a.tile_info[a.num_tiles - 1];
+    const uint32_t G = last.x + last.y;
+    if (G > 0u) finalize(a, G - 1u, cstate[nch]);
+    *a.ngroups = G;
+  }
+}
+
 // Known-segment groups of a single range-reduced counting-sort pass over
 // integer keys: digit d = ordered(key) - min is one group exactly when its
 // total is non-zero. Writes G, each group's first row and its key.
@@ -557,6 +660,9 @@ struct SegWs {
   // one-pass result word in device memory (allocation-free calls: read
   // with a stream-ordered copy); null: the calling thread's pinned slot
   uint32_t* op_res = nullptr;
+  // chunk elements and carries of the parallel fix-up ([chunks + 1])
+  uint32_t* fix_flag = nullptr;
+  void* fix_state = nullptr;
 };
 
 uint64_t seg_tiles(uint64_t n) { return (n + SEG_TILE - 1) / SEG_TILE; }
@@ -577,6 +683,9 @@ SegWs carve_seg(Carver& c, int vb, uint64_t n) {
   w.part = c.take<char>(w.part_bytes);
   w.op_mm = c.take<uint64_t>((uint64_t)SORT_MAXCHUNKS * 2 * 2);
   w.op_ctl = c.take<uint32_t>(4);
+  const uint64_t nch = (t + 1023) / 1024 + 1;
+  w.fix_flag = c.take<uint32_t>(nch);
+  w.fix_state = c.take<char>(nch * st);
   return w;
 }
 
@@ -642,7 +751,16 @@ int segreduce_mm(const SegWs& w, const SegCall& c, hipStream_t s) {
     JFX_LAUNCH("segreduce_tiles", s, (k_segreduce_tiles<K, V, MT, MM, false, true><<<a.num_tiles, SEG_BLOCK, 0, s>>>(a)));
   }
   JFX_HIP(hipGetLastError());
-  JFX_LAUNCH("segreduce_fix", s, (k_segreduce_fix<K, V, MT, MM><<<1, 1024, 0, s>>>(a)));
+  const uint32_t nch = (a.num_tiles + 1023u) / 1024u;
+  if (nch <= 1u) {
+    JFX_LAUNCH("segreduce_fix", s, (k_segreduce_fix<K, V, MT, MM><<<1, 1024, 0, s>>>(a)));
+  } else {
+    JFX_CHECK(nch <= 1024u, JFX_ERR_UNSUPPORTED, "segmented reduction: more than 2^30 rows");
+    auto* fs = static_cast<AggState<V, MM>*>(w.fix_state);
+    JFX_LAUNCH("segreduce_fix", s, (k_segfix_chunks<K, V, MT, MM><<<nch, 1024, 0, s>>>(a, w.fix_flag, fs)));
+    JFX_LAUNCH("segreduce_fix", s, (k_segfix_carry<V, MM><<<1, 1024, 0, s>>>(nch, w.fix_flag, fs)));
+    JFX_LAUNCH("segreduce_fix", s, (k_segfix_apply<K, V, MT, MM><<<nch, 1024, 0, s>>>(a, w.fix_flag, fs, nch)));
+  }
   JFX_HIP(hipGetLastError());
   return JFX_OK;
 }
